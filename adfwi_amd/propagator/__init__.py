@@ -1,0 +1,5 @@
+"""Propagators of the hot path (ADFWI/propagator)."""
+from .acoustic_propagator import AcousticPropagator
+from .boundary_condition import bc_gerjan, bc_pml, bc_pml_xz, bc_sincos
+
+__all__ = ["AcousticPropagator", "bc_pml", "bc_pml_xz", "bc_sincos", "bc_gerjan"]

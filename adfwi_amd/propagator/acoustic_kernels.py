@@ -1,0 +1,366 @@
+"""Acoustic propagation on the MI355X: coefficient set-up in torch, time stepping in HIP.
+
+Drop-in for ADFWI/propagator/acoustic_kernels.py: ``pad_torchSingle`` (:16-37)
+and ``forward_kernel`` (:179-301) keep their signatures and results.  The
+per-step op sequence (:40-176) and its autograd replay under
+``torch.utils.checkpoint`` (:267-278) are replaced by ``AcousticWave``, a
+``torch.autograd.Function`` whose forward runs the fused HIP step and whose
+backward runs the hand-written discrete adjoint with sqrt(nt) wavefield
+checkpoints and segment recomputation.
+"""
+import ctypes
+import math
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from .. import _native as nat
+
+C1 = 9.0 / 8.0
+C2 = -1.0 / 24.0
+PITCH_ALIGN = 64  # floats: rows start on 256-byte boundaries
+
+
+def pad_torchSingle(v: torch.Tensor, pml: int, nz: int, nx: int, ns: int,
+                    device: torch.device = torch.device("cpu")) -> torch.Tensor:
+    """Edge-replicating pad (nz, nx) -> (nz+2*pml, nx+2*pml), float32 like the reference
+    (acoustic_kernels.py:16-37 allocates the destination without a dtype)."""
+    out = F.pad(v.reshape(1, 1, nz, nx).to(torch.float32), (pml, pml, pml, pml), mode="replicate")
+    return out.reshape(nz + 2 * pml, nx + 2 * pml)
+
+
+def checkpoint_interval(nt: int) -> int:
+    """Wavefield checkpoint spacing: ceil(sqrt(nt)) steps, so memory grows as sqrt(nt)."""
+    return max(1, int(math.ceil(math.sqrt(nt))))
+
+
+def last_chunk_start(nt: int, segments: int) -> int:
+    """First step of the last torch.chunk(.., segments) time chunk (acoustic_kernels.py:268)."""
+    segments = max(1, int(segments))
+    size = -(-nt // segments)
+    nchunks = -(-nt // size)
+    return (nchunks - 1) * size
+
+
+class Timeline:
+    """Optional HIP-event timing of every library call (on the stream the kernels run on).
+
+    When ``enabled``, each forward / recompute / adjoint call appends
+    (label, kernel, launches, start_event, end_event); ``summary()`` turns them
+    into total milliseconds and launches per kernel.
+    """
+
+    def __init__(self):
+        self.enabled = False
+        self.records = []
+
+    def call(self, label, kernel, launches, device, fn):
+        if not self.enabled:
+            return fn()
+        st = torch.cuda.current_stream(device)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        out = fn()
+        b.record(st)
+        self.records.append((label, kernel, launches, a, b))
+        return out
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for label, kernel, n, a, b in self.records:
+            d = out.setdefault(kernel, {"ms": 0.0, "launches": 0, "calls": {}})
+            ms = a.elapsed_time(b)
+            d["ms"] += ms
+            d["launches"] += n
+            d["calls"][label] = d["calls"].get(label, 0.0) + ms
+        return out
+
+
+TIMELINE = Timeline()
+
+
+class AcousticGeometry:
+    """Device-side description of one acoustic problem (grid, sources, receivers, tiling).
+
+    Holds the int32 index tables and receiver tile bins; builds the ctypes
+    descriptor for each call.  Padded indices are passed in.
+    """
+
+    def __init__(self, Z: int, X: int, nt: int, free_surface: bool, nabc: int,
+                 src_z: np.ndarray, src_x: np.ndarray, rcv_z: np.ndarray, rcv_x: np.ndarray,
+                 device, shots_per_block: int = 0):
+        nat.require_gpu(device)
+        self.device = torch.device(device)
+        self.Z, self.X, self.nt = int(Z), int(X), int(nt)
+        self.free_surface = bool(free_surface)
+        self.fs = int(nabc) if free_surface else 1
+        self.pitch = -(-self.X // PITCH_ALIGN) * PITCH_ALIGN
+        src_z = np.asarray(src_z, dtype=np.int64).reshape(-1)
+        src_x = np.asarray(src_x, dtype=np.int64).reshape(-1)
+        rcv_z = np.asarray(rcv_z, dtype=np.int64).reshape(-1)
+        rcv_x = np.asarray(rcv_x, dtype=np.int64).reshape(-1)
+        self.S, self.R = len(src_z), len(rcv_z)
+        zmin = self.fs - 1 if free_surface else 0
+        if self.S < 1:
+            raise ValueError("at least one shot is required")
+        if np.any(src_z < zmin) or np.any(src_z > self.Z - 1) or np.any(src_x < 0) or np.any(src_x > self.X - 1):
+            raise ValueError(f"source index outside the padded grid {self.Z} x {self.X}")
+        if self.R and (np.any(rcv_z < 0) or np.any(rcv_z > self.Z - 2) or np.any(rcv_x < 0)
+                       or np.any(rcv_x > self.X - 2)):
+            raise ValueError(f"receiver index outside the padded grid {self.Z} x {self.X} "
+                             "(u has X-1 columns and w has Z-1 rows)")
+        self.shots_per_block = int(shots_per_block)
+        i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
+        self.src_z, self.src_x = i32(src_z), i32(src_x)
+        self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
+        self._keep = []
+        desc = self.desc(None, None)
+        self.layout = nat.Layout()
+        nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),
+                  "adfwi_acoustic_get_layout")
+        start = np.zeros(self.layout.ntiles + 1, dtype=np.int32)
+        lst = np.zeros(max(self.R, 1), dtype=np.int32)
+        rz32 = np.ascontiguousarray(rcv_z.astype(np.int32))
+        rx32 = np.ascontiguousarray(rcv_x.astype(np.int32))
+        nat.check(nat.load().adfwi_acoustic_bin_receivers(
+            ctypes.byref(desc), rz32.ctypes.data_as(ctypes.c_void_p), rx32.ctypes.data_as(ctypes.c_void_p),
+            start.ctypes.data_as(ctypes.c_void_p), lst.ctypes.data_as(ctypes.c_void_p)),
+            "adfwi_acoustic_bin_receivers")
+        self.bin_start = torch.tensor(start, device=self.device)
+        self.bin_list = torch.tensor(lst, device=self.device)
+
+    def desc(self, coef: Optional[torch.Tensor], src: Optional[torch.Tensor]) -> nat.AcousticDesc:
+        d = nat.AcousticDesc()
+        d.abi_version = nat.ABI_VERSION
+        d.struct_size = ctypes.sizeof(nat.AcousticDesc)
+        d.nshots, d.nz, d.nx, d.pitch = self.S, self.Z, self.X, self.pitch
+        d.fs, d.free_surface, d.nt, d.nrec = self.fs, int(self.free_surface), self.nt, self.R
+        d.shots_per_block = self.shots_per_block
+        d.coef = None if coef is None else coef.data_ptr()
+        d.src = None if src is None else src.data_ptr()
+        d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
+        d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
+        if hasattr(self, "bin_start"):
+            d.rcv_bin_start, d.rcv_bin_list = self.bin_start.data_ptr(), self.bin_list.data_ptr()
+        return d
+
+    # -- buffers ----------------------------------------------------------
+    def state(self, n: int = 1) -> torch.Tensor:
+        return torch.zeros((n, 3, self.S, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+
+    def pack_coef(self, omk1, a1, omk2, a2, omk3) -> torch.Tensor:
+        coef = torch.zeros((5, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+        for k, f in enumerate((omk1, a1, omk2, a2, omk3)):
+            coef[k, :, :self.X] = f
+        return coef
+
+
+def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *, record: bool = True,
+                energy_u_from: Optional[int] = None, ckpt_every: int = 0, image: bool = False,
+                image_p: bool = False):
+    """Runs all nt forward steps from a zero state.
+
+    Returns dict(rec=[3 x (S, nt, R)] or None, energy=(3, Z, X) summed over shot groups or None,
+    ckpt=(n, 3, S, Z, pitch) or None).
+    """
+    lib = nat.load()
+    dev = geom.device
+    S, Z, X, nt, R = geom.S, geom.Z, geom.X, geom.nt, geom.R
+    L = geom.layout
+    state0 = geom.state(1)[0]
+    work = geom.state(2)
+    ckpt = None
+    if ckpt_every:
+        nseg = -(-nt // ckpt_every)
+        ckpt = geom.state(max(nseg - 1, 1))
+    rec = None
+    if record and R:
+        rec = [torch.zeros((S, nt, R), device=dev, dtype=torch.float32) for _ in range(3)]
+    energy = None
+    if energy_u_from is not None:
+        energy = torch.zeros((L.ngroups, 3, Z, geom.pitch), device=dev, dtype=torch.float32)
+    d = geom.desc(coef, src)
+    a = nat.AcousticFwdArgs()
+    a.state_in = state0.data_ptr()
+    a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
+    a.ckpt = None if ckpt is None else ckpt.data_ptr()
+    a.ckpt_every = int(ckpt_every)
+    a.t0, a.nsteps = 0, nt
+    a.energy_u_from = int(energy_u_from) if energy_u_from is not None else nt
+    if rec is not None:
+        a.rec_p, a.rec_u, a.rec_w = rec[0].data_ptr(), rec[1].data_ptr(), rec[2].data_ptr()
+    a.energy = None if energy is None else energy.data_ptr()
+    TIMELINE.call("forward", "ac_fwd_step", nt, dev, lambda: nat.check(
+        lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(a), nat.stream_handle(dev)),
+        "adfwi_acoustic_forward"))
+    out = {"rec": rec, "ckpt": ckpt, "energy": None}
+    if energy is not None:
+        out["energy"] = energy.sum(0)[:, :, :X]
+    return out
+
+
+def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, ckpt: Optional[torch.Tensor],
+                 ckpt_every: int, g_rec, need_a1: bool, need_a2: bool, need_src: bool):
+    """Hand-written adjoint over all steps with segment recomputation from the checkpoints.
+
+    g_rec: tuple of three (S, nt, R) adjoint sources (or None each).
+    Returns (grad_a1 (Z, X) or None, grad_a2 or None, grad_src (S, nt) or None).
+    """
+    lib = nat.load()
+    dev = geom.device
+    S, Z, X, nt = geom.S, geom.Z, geom.X, geom.nt
+    L = geom.layout
+    k = ckpt_every
+    nseg = -(-nt // k)
+    rw = geom.state(2)    # recompute ping-pong
+    lw = geom.state(2)    # adjoint ping-pong
+    image = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
+    image_p = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
+    ga1 = torch.zeros((L.ngroups, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
+    ga2 = torch.zeros((L.ngroups, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
+    gsrc = torch.zeros((S, nt), device=dev, dtype=torch.float32) if need_src else None
+    g = [None if x is None else x.contiguous().to(torch.float32) for x in g_rec]
+    gp = [None if x is None else x.data_ptr() for x in g]
+    d = geom.desc(coef, src)
+    stream = nat.stream_handle(dev)
+    nat.check(lib.adfwi_acoustic_adjoint_seed(ctypes.byref(d), lw[0].data_ptr(), gp[0], gp[1], gp[2],
+                                              nt - 1, stream), "adfwi_acoustic_adjoint_seed")
+    lam = lw[0]
+    need_image = need_a1 or need_a2
+    for seg in reversed(range(nseg)):
+        t_lo = seg * k
+        t_hi = min(nt, (seg + 1) * k) - 1
+        n = t_hi - t_lo + 1
+        if need_image:
+            if seg == 0:
+                rw[0].zero_()
+                sin = rw[0]
+            else:
+                sin = ckpt[seg - 1]
+            fa = nat.AcousticFwdArgs()
+            fa.state_in = sin.data_ptr()
+            fa.work[0], fa.work[1] = rw[0].data_ptr(), rw[1].data_ptr()
+            fa.t0, fa.nsteps = t_lo, n
+            fa.energy_u_from = nt
+            fa.image = None if image is None else image.data_ptr()
+            fa.image_p = None if image_p is None else image_p.data_ptr()
+            TIMELINE.call("recompute", "ac_fwd_step", n, dev, lambda: nat.check(
+                lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(fa), stream),
+                "adfwi_acoustic_forward (recompute)"))
+        aa = nat.AcousticAdjArgs()
+        aa.lam_in = lam.data_ptr()
+        aa.work[0], aa.work[1] = lw[0].data_ptr(), lw[1].data_ptr()
+        aa.t_hi, aa.nsteps, aa.image_t0 = t_hi, n, t_lo
+        aa.image = None if image is None else image.data_ptr()
+        aa.image_p = None if image_p is None else image_p.data_ptr()
+        aa.grad_rec_p, aa.grad_rec_u, aa.grad_rec_w = gp
+        aa.grad_a1 = None if ga1 is None else ga1.data_ptr()
+        aa.grad_a2 = None if ga2 is None else ga2.data_ptr()
+        aa.grad_src = None if gsrc is None else gsrc.data_ptr()
+        TIMELINE.call("adjoint", "ac_adj_step", n, dev, lambda: nat.check(
+            lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream), "adfwi_acoustic_adjoint"))
+        lam = lw[0] if aa.lam_out == lw[0].data_ptr() else lw[1]
+    out_a1 = ga1.sum(0)[:, :X] if ga1 is not None else None
+    out_a2 = ga2.sum(0)[:, :X] if ga2 is not None else None
+    return out_a1, out_a2, gsrc
+
+
+class AcousticWave(torch.autograd.Function):
+    """Records (p, u, w) and energy maps from the padded coefficient fields.
+
+    forward(a1, a2, src, omk1, omk2, omk3, geom, energy_u_from, ckpt_every)
+      a1 = rho c^2 dt/dz, a2 = dt/(rho dz), src = dt * wavelet (S, nt),
+      omk* = 1 - kappa* damping factors (not differentiated).
+    Returns rec_p, rec_u, rec_w (S, nt, R), e_p, e_u, e_w (Z, X; non-differentiable):
+      e_p = sum_t sum_s p^2, e_u = sum_{t >= energy_u_from} sum_s u^2, e_w = sum_s w^2 at t = nt-1.
+    """
+
+    @staticmethod
+    def forward(ctx, a1, a2, src, omk1, omk2, omk3, geom, energy_u_from, ckpt_every):
+        ctx.set_materialize_grads(False)
+        coef = geom.pack_coef(omk1, a1.detach(), omk2, a2.detach(), omk3)
+        src = src.detach().to(torch.float32).contiguous()
+        need_grad = any(ctx.needs_input_grad[:3])  # grad mode is off inside forward()
+        k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
+        res = run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from,
+                          ckpt_every=k if need_grad else 0)
+        rec = res["rec"]
+        if rec is None:
+            rec = [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(3)]
+        e = res["energy"]
+        ctx.geom, ctx.k = geom, k
+        ctx.ckpt = res["ckpt"] if need_grad else None
+        ctx.save_for_backward(coef, src)
+        outs = (rec[0], rec[1], rec[2], e[0].contiguous(), e[1].contiguous(), e[2].contiguous())
+        ctx.mark_non_differentiable(*outs[3:])
+        return outs
+
+    @staticmethod
+    def backward(ctx, g_p, g_u, g_w, *_):
+        coef, src = ctx.saved_tensors
+        need_a1, need_a2, need_src = ctx.needs_input_grad[:3]
+        if g_p is None and g_u is None and g_w is None:
+            return (None,) * 9
+        ga1, ga2, gsrc = run_backward(ctx.geom, coef, src, ctx.ckpt, ctx.k, (g_p, g_u, g_w),
+                                      need_a1, need_a2, need_src)
+        ctx.ckpt = None
+        return ga1, ga2, gsrc, None, None, None, None, None, None
+
+
+def coefficient_fields(v, rho, damp, nz, nx, nabc, dt, dz, free_surface, device):
+    """A1, K1, A2, K2, K3 exactly as acoustic_kernels.py:225-265 forms them (dz on both axes)."""
+    c = pad_torchSingle(v, nabc, nz, nx, 0, device)
+    den = pad_torchSingle(rho, nabc, nz, nx, 0, device)
+    Z, X = nz + 2 * nabc, nx + 2 * nabc
+    fs = nabc if free_surface else 1
+    alpha1 = den * c * c * dt / dz
+    kappa1 = damp * dt
+    alpha2 = dt / (den * dz)
+    kappa2 = torch.zeros_like(damp)
+    kappa2[:, 1:X - 2] = 0.5 * (damp[:, 1:X - 2] + damp[:, 2:X - 1]) * dt
+    kappa3 = torch.zeros_like(damp)
+    kappa3[fs:Z - 2, :] = 0.5 * (damp[fs:Z - 2, :] + damp[fs + 1:Z - 1, :]) * dt
+    return alpha1, kappa1, alpha2, kappa2, kappa3
+
+
+def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
+                   nabc: int, free_surface: bool,
+                   src_x: torch.Tensor, src_z: torch.Tensor, src_n: int, src_v: torch.Tensor,
+                   rcv_x: torch.Tensor, rcv_z: torch.Tensor, rcv_n: int,
+                   damp: torch.Tensor, v: torch.Tensor, rho: torch.Tensor,
+                   checkpoint_segments: int = 1,
+                   device: torch.device = torch.device("cpu"), dtype: torch.dtype = torch.float32,
+                   geometry: Optional[AcousticGeometry] = None, ckpt_every: int = 0) -> Dict[str, torch.Tensor]:
+    """Acoustic forward modelling with the reference's signature and result dictionary
+    (acoustic_kernels.py:179-301).
+
+    ``checkpoint_segments`` keeps the reference meaning for the energy maps
+    (the u/w maps depend on the last time chunk, acoustic_kernels.py:286-288);
+    memory is governed by the sqrt(nt) wavefield checkpoints of the HIP adjoint
+    (``ckpt_every`` overrides the interval).
+    """
+    nat.require_gpu(device)
+    if dtype != torch.float32:
+        raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
+    Z, X = nz + 2 * nabc, nx + 2 * nabc
+    a1, k1, a2, k2, k3 = coefficient_fields(v, rho, damp.to(device=device, dtype=torch.float32),
+                                            nz, nx, nabc, dt, dz, free_surface, device)
+    if src_v.dim() == 1:
+        src_v = src_v.unsqueeze(0).expand(src_n, -1)
+    src = dt * src_v
+    if geometry is None:
+        geometry = AcousticGeometry(Z, X, nt, free_surface, nabc,
+                                    (src_z + nabc).cpu().numpy(), (src_x + nabc).cpu().numpy(),
+                                    (rcv_z + nabc).cpu().numpy(), (rcv_x + nabc).cpu().numpy(), device)
+    eu_from = last_chunk_start(nt, checkpoint_segments)
+    rp, ru, rw, ep, eu, ew = AcousticWave.apply(a1, a2, src, 1.0 - k1, 1.0 - k2, 1.0 - k3, geometry,
+                                                eu_from, ckpt_every)
+    crop = (slice(nabc, nabc + nz), slice(nabc, nabc + nx))
+    f_p = ep[crop]
+    f_u = f_p + eu[crop]
+    f_w = f_p + (eu[crop] + ew[crop])
+    return {"p": rp, "u": ru, "w": rw,
+            "forward_wavefield_p": f_p, "forward_wavefield_u": f_u, "forward_wavefield_w": f_w}

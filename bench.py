@@ -1,0 +1,277 @@
+"""Benchmark of the propagator hot path on one MI355X.
+
+A step is one FWI gradient of the whole shot batch: forward modelling of all
+shots (records, energy maps, sqrt(nt) wavefield checkpoints), the waveform L2
+misfit, and the hand-written adjoint (segment recomputation + time-reversed
+adjoint + zero-lag imaging) down to vp.grad.
+
+Default workload (acoustic, one launch of 64 shots): grid 384 x 1152,
+dx 12.5 m, 40-cell PML, dt 1 ms, nt 4000, 8 Hz Ricker, 1152 receivers,
+checkpoint interval ceil(sqrt(nt)) = 64.  Synthetic seeded models; the
+observed data are modelled from the true model before timing.
+
+Prints one JSON line: metric, value (shot-gradients/s), roofline of the
+dominant kernel (HIP-event timed inside the timed region), cpu_baseline
+(the CPU oracle on the 100 x 200 / 10-shot / nt 1000 workload, bounded sample)
+and the plain-autograd path on the same GPU (bounded sample, extrapolated
+per time step).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload acoustic_large]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per padded point, per time step, per shot (fp32 state, imaging traffic)
+BYTES_FWD = 24.0          # forward: read + write p, u, w
+BYTES_RECOMPUTE = 28.0    # recompute: forward + stencil image store
+BYTES_ADJ = 28.0          # adjoint: read + write adjoint p, u, w + stencil image read
+BYTES_GRAD = 80.0         # per shot-gradient: 24 + 28 + 28
+
+
+def l2_misfit(syn, obs, dt):
+    """Waveform L2 misfit sum_{s,r} sqrt(sum_t rsd^2 dt) (ADFWI/fwi/misfit/L2.py:26-29)."""
+    rsd = obs - syn
+    return torch.sum(torch.sqrt(torch.sum(rsd * rsd * dt, dim=1)))
+
+
+def build_problem(wl, dev):
+    from adfwi_amd.model import AcousticModel
+    from adfwi_amd.propagator import AcousticPropagator
+    from adfwi_amd.survey import Receiver, Source, Survey
+    src = Source(wl.nt, wl.dt, wl.f0)
+    wav = wl.wavelets()[0]
+    src.add_sources(wl.src_x, wl.src_z, wav)
+    rcv = Receiver(wl.nt, wl.dt)
+    rcv.add_receivers(wl.rcv_x, wl.rcv_z, "pr")
+    survey = Survey(src, rcv)
+    true = AcousticModel(0, 0, wl.nx, wl.nz, wl.dx, wl.dx, vp=wl.models["vp_true"], rho=wl.models["rho_true"],
+                         nabc=wl.nabc, free_surface=wl.free_surface, device=dev)
+    init = AcousticModel(0, 0, wl.nx, wl.nz, wl.dx, wl.dx, vp=wl.models["vp_init"], rho=wl.models["rho_init"],
+                         vp_grad=True, nabc=wl.nabc, free_surface=wl.free_surface, device=dev)
+    prop = AcousticPropagator(init, survey, device=dev)
+    with torch.no_grad():
+        obs = prop.forward(model=true)["p"].detach().clone()
+    return prop, init, obs
+
+
+def grad_step(prop, model, obs, dt):
+    model.vp.grad = None
+    rec = prop.forward()
+    loss = l2_misfit(rec["p"], obs, dt)
+    loss.backward()
+    return loss
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds_budget=24.0):
+    """The CPU oracle (eager torch restatement of the reference path, fp32) on the
+    100 x 200 / 10-shot / nt 1000 workload: forward + L2 + autograd backward to vp.grad."""
+    from adfwi_amd.propagator.boundary_condition import bc_pml
+    from adfwi_amd.workloads import acoustic_small
+    from oracle import acoustic as oa
+    cores = min(16, os.cpu_count() or 1)
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    torch.set_num_threads(cores)
+    wl = acoustic_small()
+    nz, nx, nabc = wl.nz, wl.nx, wl.nabc
+    damp = torch.tensor(bc_pml(nx, nz, wl.dx, wl.dx, nabc, float(wl.models["vp_init"].max()), False),
+                        dtype=torch.float32)
+    wav = torch.tensor(wl.wavelets())
+    args = (torch.tensor(wl.src_x), torch.tensor(wl.src_z), wl.nshots, wav, torch.tensor(wl.rcv_x),
+            torch.tensor(wl.rcv_z), len(wl.rcv_x), damp)
+    rho = torch.tensor(wl.models["rho_init"])
+
+    def one(nt_run, obs=None):
+        v = torch.tensor(wl.models["vp_init"], requires_grad=True)
+        res = oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, nabc, False, *args[:3],
+                                args[3][:, :nt_run], *args[4:], v, rho, checkpoint_segments=1)
+        target = torch.zeros_like(res["p"]) if obs is None else obs
+        oa.l2_misfit(target, res["p"], wl.dt).backward()
+        return v.grad
+    # bounded sample: full shots and grid, time steps cut so the sample fits the budget
+    t0 = time.perf_counter()
+    one(50)
+    per_step = (time.perf_counter() - t0) / 50
+    nt_run = int(max(50, min(wl.nt, seconds_budget / 3 / max(per_step, 1e-9))))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        one(nt_run)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times)) * wl.nt / nt_run
+    N = (nz + 2 * nabc) * (nx + 2 * nabc)
+    return {"value": wl.nshots / t, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward+L2+backward, "
+                       f"100x200 grid, 20-cell PML, 10 shots, {nt_run} of 1000 steps, median of 3, "
+                       f"scaled to nt=1000"),
+            "gpt_updates_per_s": 2 * wl.nshots * wl.nt * N / t / 1e9,
+            "cpu": _cpu_name()}
+
+
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def plain_gpu_baseline(wl, dev, nt_sample=200):
+    """The same oracle (plain autograd path, reentrant checkpointing with ceil(sqrt(nt)) segments)
+    on the GPU for the benchmark workload, nt_sample steps, extrapolated per time step."""
+    from adfwi_amd.propagator.boundary_condition import bc_pml
+    from oracle import acoustic as oa
+    nz, nx, nabc = wl.nz, wl.nx, wl.nabc
+    damp = torch.tensor(bc_pml(nx, nz, wl.dx, wl.dx, nabc, float(wl.models["vp_init"].max()), False),
+                        dtype=torch.float32, device=dev)
+    wav = torch.tensor(wl.wavelets()[:, :nt_sample], device=dev)
+    rho = torch.tensor(wl.models["rho_init"], device=dev)
+    segs = int(math.ceil(math.sqrt(nt_sample)))
+
+    def one():
+        v = torch.tensor(wl.models["vp_init"], device=dev, requires_grad=True)
+        res = oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_sample, wl.dt, nabc, False,
+                                torch.tensor(wl.src_x, device=dev), torch.tensor(wl.src_z, device=dev),
+                                wl.nshots, wav, torch.tensor(wl.rcv_x, device=dev),
+                                torch.tensor(wl.rcv_z, device=dev), len(wl.rcv_x), damp, v, rho,
+                                checkpoint_segments=segs, device=dev)
+        oa.l2_misfit(torch.zeros_like(res["p"]), res["p"], wl.dt).backward()
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    one()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) * wl.nt / nt_sample
+    return {"value": wl.nshots / t, "unit": "shot-gradients/s",
+            "sample": f"oracle on GPU, {nt_sample} of {wl.nt} steps, {segs} checkpoint segments, "
+                      f"extrapolated per step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="acoustic_large")
+    ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs only)")
+    ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many MiB (PMC counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots shard across GPUs with "
+              "one gradient all-reduce per iteration, which this single-GPU build does not launch",
+              file=sys.stderr)
+        sys.exit(2)
+    dev = "cuda:0"
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe}))
+        return
+
+    from adfwi_amd import workloads
+    from adfwi_amd.propagator import acoustic_kernels as ak
+    kw = {"nt": args.nt} if args.nt else {}
+    wl = getattr(workloads, args.workload)(**kw)
+    log(f"workload {wl.name}: grid {wl.nz}x{wl.nx}, nt {wl.nt}, {wl.nshots} shots")
+    prop, model, obs = build_problem(wl, dev)
+    log("observed data modelled")
+    for i in range(args.warmup):
+        grad_step(prop, model, obs, wl.dt)
+        torch.cuda.synchronize()
+        log(f"warmup step {i} done")
+    torch.cuda.synchronize()
+    ak.TIMELINE.records.clear()
+    ak.TIMELINE.enabled = True
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        loss = grad_step(prop, model, obs, wl.dt)
+        log(f"timed step {i} issued")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ak.TIMELINE.enabled = False
+    tl = ak.TIMELINE.summary()
+    Z, X = wl.padded
+    N = Z * X
+    S = wl.nshots
+    value = S * args.steps / elapsed
+    # dominant kernel by total time; its algorithmic bytes per launch
+    kern = {}
+    for name, d in tl.items():
+        if name == "ac_fwd_step":
+            fwd_ms, rec_ms = d["calls"].get("forward", 0.0), d["calls"].get("recompute", 0.0)
+            launches_fwd = args.steps * wl.nt
+            launches_rec = d["launches"] - launches_fwd
+            byts = launches_fwd * BYTES_FWD * N * S + launches_rec * BYTES_RECOMPUTE * N * S
+        else:
+            byts = d["launches"] * BYTES_ADJ * N * S
+        avg_ms = d["ms"] / max(d["launches"], 1)
+        kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "avg_launch_ms": avg_ms,
+                      "achieved_gbs": byts / (d["ms"] / 1e3) / 1e9 if d["ms"] else 0.0,
+                      "bytes_per_launch": byts / max(d["launches"], 1)}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    ach = kern[dom]["achieved_gbs"]
+    out = {
+        "metric": "shot-gradients/s",
+        "value": value,
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded models; observed data modelled from the true model)",
+        "config": {"workload": wl.name, "grid": [wl.nz, wl.nx], "padded_grid": [Z, X], "nabc": wl.nabc,
+                   "nt": wl.nt, "shots_per_launch": S, "receivers": len(wl.rcv_x), "dt": wl.dt, "dx": wl.dx,
+                   "checkpoint_interval": ak.checkpoint_interval(wl.nt), "parallelism": "single GPU, shot batch"},
+        "gpt_updates_per_s": 2 * S * N * wl.nt * args.steps / elapsed / 1e9,
+        "grad_bytes_roofline_frac": BYTES_GRAD * S * N * wl.nt * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kern[dom]["bytes_per_launch"],
+                     "avg_launch_ms": kern[dom]["avg_launch_ms"]},
+        "kernels": kern,
+        "loss": float(loss),
+    }
+    log(f"timed region {elapsed:.2f} s, {value:.2f} shot-gradients/s")
+    if not args.no_baselines:
+        out["plain_autograd_gpu"] = plain_gpu_baseline(wl, dev)
+        log("plain autograd GPU baseline done")
+        out["speedup_vs_plain_autograd_gpu"] = value / out["plain_autograd_gpu"]["value"]
+        out["cpu_baseline"] = cpu_baseline()
+        log("CPU baseline done")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

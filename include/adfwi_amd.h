@@ -1,0 +1,143 @@
+/*
+ * adfwi_amd.h -- C ABI of libadfwi_amd.so, the MI355X (gfx950) propagator
+ * library for ADFWI's acoustic and elastic staggered-grid time stepping.
+ *
+ * Conventions
+ *   - Every buffer is caller-owned device memory; the library never allocates
+ *     or frees device memory and keeps no mutable global state, so calls on
+ *     distinct streams may run concurrently.
+ *   - Functions return 0 on success or a negative ADFWI_E* code;
+ *     adfwi_last_error() returns a thread-local message for the last failure
+ *     on the calling thread.  Arguments are checked before any launch.
+ *   - Grids are the padded grids of the reference (absorbing layers and, for
+ *     the elastic case, the two free-surface rows included).  Every 2-D field
+ *     is stored row-major with a row pitch of `pitch` floats (pitch >= nx);
+ *     a state set is [nfields][nshots][nz][pitch].
+ *   - Streams are passed as opaque pointers (a hipStream_t; NULL = default).
+ *
+ * The entry points replace the reference's per-step PyTorch op sequences:
+ *   acoustic : ADFWI/propagator/acoustic_kernels.py:40-176 (step_forward) and
+ *              its autograd replay under torch.utils.checkpoint (:267-278);
+ *   elastic  : ADFWI/propagator/elastic_kernels.py:222-423
+ *              (step_forward_PML_4order) and its autograd replay (:975-992).
+ */
+#ifndef ADFWI_AMD_H
+#define ADFWI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADFWI_ABI_VERSION 1
+
+#define ADFWI_OK 0
+#define ADFWI_E_VERSION -1   /* abi_version / struct_size mismatch */
+#define ADFWI_E_ARG -2       /* invalid argument (shape, index, null pointer) */
+#define ADFWI_E_LAUNCH -3    /* HIP launch or runtime failure */
+#define ADFWI_E_FAULT_INJECTED -4 /* ADFWI_FAULT_INJECT was set (testing) */
+
+typedef void *adfwi_stream_t;
+
+/* Version of this ABI (ADFWI_ABI_VERSION). */
+int adfwi_abi_version(void);
+/* Static text for an error code. */
+const char *adfwi_strerror(int code);
+/* Thread-local description of the last failure on this thread. */
+const char *adfwi_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Acoustic p-u-w propagator (acoustic_kernels.py:113-174)                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct adfwi_acoustic_desc {
+  int32_t abi_version;     /* ADFWI_ABI_VERSION */
+  int32_t struct_size;     /* sizeof(adfwi_acoustic_desc) */
+  int32_t nshots;          /* S: shots advanced together */
+  int32_t nz, nx;          /* padded grid Z = nz+2*nabc, X = nx+2*nabc */
+  int32_t pitch;           /* row pitch of every field, in floats (>= nx) */
+  int32_t fs;              /* first free-surface row: nabc with free surface, else 1 */
+  int32_t free_surface;    /* 0 / 1 */
+  int32_t nt;              /* time steps of the whole record */
+  int32_t nrec;            /* R: receivers shared by all shots */
+  int32_t shots_per_block; /* shots looped inside one workgroup; 0 = library choice */
+  int32_t reserved;
+  const float *coef;       /* [5][nz][pitch]: 1-K1, A1, 1-K2, A2, 1-K3 */
+  const float *src;        /* [nshots][nt]: dt * wavelet, injected into p */
+  const int32_t *src_z;    /* [nshots] padded source rows */
+  const int32_t *src_x;    /* [nshots] padded source columns */
+  const int32_t *rcv_z;    /* [nrec] padded receiver rows */
+  const int32_t *rcv_x;    /* [nrec] padded receiver columns */
+  const int32_t *rcv_bin_start; /* [ntiles+1], see adfwi_acoustic_bin_receivers */
+  const int32_t *rcv_bin_list;  /* [nrec] */
+} adfwi_acoustic_desc;
+
+typedef struct adfwi_layout {
+  int32_t tile_z, tile_x;  /* output tile of one workgroup */
+  int32_t zbase;           /* first grid row covered by tiles (rows above never change) */
+  int32_t ntiles_z, ntiles_x, ntiles;
+  int32_t shots_per_block; /* effective value */
+  int32_t ngroups;         /* shot groups = ceil(nshots / shots_per_block) */
+  int32_t nfields;         /* fields in one state set */
+  int32_t reserved;
+  int64_t field_floats;    /* nshots * nz * pitch */
+  int64_t state_floats;    /* nfields * field_floats */
+  int64_t accum_floats;    /* ngroups * nz * pitch: one per-group accumulator plane set */
+} adfwi_layout;
+
+/* Tiling and buffer sizes for a descriptor (host only, no device access). */
+int adfwi_acoustic_get_layout(const adfwi_acoustic_desc *d, adfwi_layout *out);
+
+/* Host-side counting sort of receivers by output tile.  rz/rx are host arrays
+ * of nrec padded indices; start must hold ntiles+1 and list nrec entries. */
+int adfwi_acoustic_bin_receivers(const adfwi_acoustic_desc *d, const int32_t *rz,
+                                 const int32_t *rx, int32_t *start, int32_t *list);
+
+typedef struct adfwi_acoustic_fwd_args {
+  const float *state_in;   /* [3][S][Z][pitch] p,u,w before step t0 */
+  float *work[2];          /* two scratch state sets */
+  float *ckpt;             /* [n][3][S][Z][pitch] or NULL: slot j <- state after step (j+1)*ckpt_every-1 */
+  int32_t ckpt_every;
+  int32_t t0, nsteps;      /* advance steps t0 .. t0+nsteps-1 */
+  int32_t energy_u_from;   /* first step whose u^2 enters the u energy map */
+  float *rec_p, *rec_u, *rec_w; /* [S][nt][R] or NULL */
+  float *energy;           /* [ngroups][3][Z][pitch] (+=) or NULL */
+  float *image;            /* [nsteps][S][Z][pitch] or NULL: p-update stencil of each step */
+  float *image_p;          /* [nsteps][S][Z][pitch] or NULL: p after each step */
+  const float *state_out;  /* OUT: buffer holding the state after the last step */
+} adfwi_acoustic_fwd_args;
+
+/* Forward time stepping (replaces step_forward, acoustic_kernels.py:40-176). */
+int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args *a,
+                           adfwi_stream_t stream);
+
+typedef struct adfwi_acoustic_adj_args {
+  const float *lam_in;     /* adjoint state after step t_hi, records of t_hi included */
+  float *work[2];
+  int32_t t_hi, nsteps;    /* adjoints of steps t_hi, t_hi-1, ..., t_hi-nsteps+1 */
+  int32_t image_t0;        /* image slot k holds step image_t0 + k */
+  int32_t reserved;
+  const float *image;      /* stencil images written by adfwi_acoustic_forward */
+  const float *image_p;    /* pressure images, or NULL when grad_a2 is NULL */
+  const float *grad_rec_p, *grad_rec_u, *grad_rec_w; /* [S][nt][R] or NULL */
+  float *grad_a1, *grad_a2; /* [ngroups][Z][pitch] (+=) or NULL */
+  float *grad_src;          /* [S][nt] or NULL: d loss / d src */
+  const float *lam_out;     /* OUT: adjoint state after step t_hi-nsteps (records included) */
+} adfwi_acoustic_adj_args;
+
+/* Discrete adjoint of adfwi_acoustic_forward (the transpose of every stage of
+ * acoustic_kernels.py:115-169), with zero-lag imaging of dL/dA1 and dL/dA2. */
+int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a,
+                           adfwi_stream_t stream);
+
+/* lam += receiver adjoint sources of step t (seeds the adjoint at t = nt-1). */
+int adfwi_acoustic_adjoint_seed(const adfwi_acoustic_desc *d, float *lam,
+                                const float *grad_rec_p, const float *grad_rec_u,
+                                const float *grad_rec_w, int32_t t, adfwi_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADFWI_AMD_H */

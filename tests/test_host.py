@@ -1,0 +1,176 @@
+"""CPU tests of the product's host side: damping profiles, model parameterisation, the C ABI
+library (loads, exports every declared entry point, host-only layout / binning / errors)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT, golden
+
+LIB = os.path.join(ROOT, "adfwi_amd", "libadfwi_amd.so")
+HEADER = os.path.join(ROOT, "include", "adfwi_amd.h")
+
+
+# ---------------------------------------------------------------- profiles
+@pytest.mark.parametrize("fs", [False, True])
+@pytest.mark.parametrize("shape", [(24, 16, 6), (37, 21, 9), (200, 100, 20)])
+def test_product_profiles_equal_oracle_loops(fs, shape):
+    from adfwi_amd.propagator import boundary_condition as bc
+    from oracle import boundary as ob
+    nx, nz, pml = shape
+    assert np.array_equal(bc.bc_pml(nx, nz, 10.0, 10.0, pml, 3512.5, fs), ob.bc_pml(nx, nz, 10.0, 10.0, pml, 3512.5, fs))
+    for a, b in zip(bc.bc_pml_xz(nx, nz, 5.0, 5.0, pml, 4100.0, fs), ob.bc_pml_xz(nx, nz, 5.0, 5.0, pml, 4100.0, fs)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(bc.bc_sincos(nx, nz, 10.0, 10.0, pml, fs), ob.bc_sincos(nx, nz, 10.0, 10.0, pml, fs))
+    assert np.array_equal(bc.bc_gerjan(nx, nz, 10.0, 10.0, pml, 0.0053, fs),
+                          ob.bc_gerjan(nx, nz, 10.0, 10.0, pml, 0.0053, fs))
+
+
+def test_product_profiles_equal_reference_vectors():
+    from adfwi_amd.propagator import boundary_condition as bc
+    g = golden("boundary")
+    assert np.array_equal(bc.bc_pml(24, 16, 10.0, 10.0, 6, 3000.0, False), g["pml_nofs"])
+    assert np.array_equal(bc.bc_pml_xz(24, 16, 5.0, 5.0, 6, 3000.0, True)[0], g["pmlxz_fs1_x"])
+
+
+# ------------------------------------------------------------------ models
+def test_wavelet_closed_form():
+    from adfwi_amd.utils import wavelet
+    t, w = wavelet(500, 1e-3, 10.0)
+    t0, tau = 0.12, (np.pi * 10.0) ** 2
+    np.testing.assert_allclose(w, (1 - 2 * tau * (t - t0) ** 2) * np.exp(-tau * (t - t0) ** 2), rtol=1e-14)
+    assert np.argmax(w) == 120 and abs(w.max() - 1.0) < 1e-12
+
+
+def test_acoustic_model_forward_gardner_and_clip():
+    from adfwi_amd.model import AcousticModel
+    rng = np.random.default_rng(0)
+    vp = rng.uniform(1700, 3800, (10, 12)).astype(np.float32)
+    m = AcousticModel(0, 0, 12, 10, 10.0, 10.0, vp=vp, rho=np.ones_like(vp), vp_bound=(1600.0, 3900.0),
+                      vp_grad=True)
+    with torch.no_grad():                      # an optimiser step leaves the bounds
+        m.vp[0, 0] = 1000.0
+        m.vp[1, 1] = 5000.0
+    m.forward()
+    v = m.vp.detach().numpy()
+    assert v.min() >= 1600.0 - 1e-3 and v.max() <= 3900.0 + 1e-3       # clamped in place
+    np.testing.assert_array_equal(m.rho.detach().numpy(), (np.power(v, 0.25) * 310).astype(np.float32))
+    assert not m.rho.requires_grad and m.vp.requires_grad
+
+
+def test_elastic_parameterisation_matches_oracle_and_closed_forms():
+    from adfwi_amd.model import AnisotropicElasticModel, IsotropicElasticModel
+    from oracle import model as om
+    rng = np.random.default_rng(1)
+    nz, nx = 9, 11
+    vp = rng.uniform(2000, 3000, (nz, nx)); vs = vp / 1.73; rho = rng.uniform(1800, 2400, (nz, nx))
+    eps = rng.uniform(0, 0.25, (nz, nx)); delta = rng.uniform(-0.05, 0.15, (nz, nx)); gamma = np.zeros((nz, nx))
+    m = AnisotropicElasticModel(0, 0, nx, nz, 5.0, 5.0, vp=vp, vs=vs, rho=rho, eps=eps, gamma=gamma, delta=delta)
+    m.forward()
+    T = lambda a: torch.tensor(a, dtype=torch.float32)
+    CC, bx, bz, lamu, lam = om.elastic_cc(T(vp), T(vs), T(rho), T(eps), T(delta), T(gamma), nx, nz)
+    for k in (0, 2, 11, 18):
+        assert torch.equal(m.CC[k], CC[k]), k
+    assert torch.equal(m.bx, bx) and torch.equal(m.bz, bz)
+    # closed forms: C33 = vp^2 rho, C11 = C33 (1 + 2 eps), C13 from delta
+    c33 = vp.astype(np.float32) ** 2 * rho.astype(np.float32)
+    np.testing.assert_allclose(m.CC[11].numpy(), c33, rtol=1e-6)
+    np.testing.assert_allclose(m.CC[0].numpy(), c33 * (1 + 2 * eps), rtol=1e-5)
+    c44 = vs ** 2 * rho
+    np.testing.assert_allclose(m.CC[2].numpy(), np.sqrt(2 * c33 * (c33 - c44) * delta + (c33 - c44) ** 2) - c44,
+                               rtol=1e-4)
+    iso = IsotropicElasticModel(0, 0, nx, nz, 5.0, 5.0, vp=vp, vs=vs, rho=rho, auto_update_rho=False)
+    iso.forward()
+    mu = (vs ** 2 * rho)
+    five = 0.2 * (mu[1:-1, 1:-1] + 2 * mu[2:, 1:-1] + mu[1:-1, 2:] + mu[2:, 2:])
+    np.testing.assert_allclose(iso.CC[18].numpy(), five, rtol=1e-5)
+    np.testing.assert_allclose(iso.CC[2].numpy(), vp ** 2 * rho - 2 * mu, rtol=1e-4)
+
+
+# --------------------------------------------------------------------- ABI
+def _declared_entry_points():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(adfwi_\w+)\s*\(", text, re.M)))
+
+
+def test_library_is_built_and_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "run python -m adfwi_amd.build first"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (adfwi_\w+)", out))
+    declared = _declared_entry_points()
+    assert declared and set(declared) <= exported, set(declared) - exported
+    from adfwi_amd import _native as nat
+    assert set(declared) == set(nat.exported_symbols())
+
+
+def test_library_loads_without_gpu_and_reports_version():
+    from adfwi_amd import _native as nat
+    lib = nat.load()
+    assert lib.adfwi_abi_version() == nat.ABI_VERSION
+    assert lib.adfwi_strerror(-2) == b"invalid argument"
+
+
+def _desc(S=3, Z=60, X=150, fs=1, free_surface=0, nt=10, nrec=5):
+    from adfwi_amd import _native as nat
+    d = nat.AcousticDesc()
+    d.abi_version, d.struct_size = nat.ABI_VERSION, ctypes.sizeof(nat.AcousticDesc)
+    d.nshots, d.nz, d.nx, d.pitch = S, Z, X, -(-X // 64) * 64
+    d.fs, d.free_surface, d.nt, d.nrec = fs, free_surface, nt, nrec
+    return d
+
+
+def test_layout_and_receiver_bins_host_side():
+    from adfwi_amd import _native as nat
+    lib = nat.load()
+    d = _desc(free_surface=1, fs=20)
+    L = nat.Layout()
+    assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == 0
+    assert L.zbase == 19 and L.ntiles_x == 3 and L.ntiles == L.ntiles_x * L.ntiles_z
+    assert L.ngroups * L.shots_per_block >= 3
+    rz = np.array([21, 21, 40, 58, 5], dtype=np.int32)   # last one above the free surface
+    rx = np.array([0, 70, 70, 148, 3], dtype=np.int32)
+    start = np.zeros(L.ntiles + 1, dtype=np.int32); lst = np.full(5, -1, dtype=np.int32)
+    rc = lib.adfwi_acoustic_bin_receivers(ctypes.byref(d), rz.ctypes.data_as(ctypes.c_void_p),
+                                          rx.ctypes.data_as(ctypes.c_void_p), start.ctypes.data_as(ctypes.c_void_p),
+                                          lst.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0 and start[-1] == 4
+    for t in range(L.ntiles):
+        for k in range(start[t], start[t + 1]):
+            r = lst[k]
+            assert (rz[r] - L.zbase) // L.tile_z * L.ntiles_x + rx[r] // L.tile_x == t
+
+
+def test_abi_rejects_bad_arguments_with_messages():
+    from adfwi_amd import _native as nat
+    lib = nat.load()
+    L = nat.Layout()
+    d = _desc()
+    d.struct_size = 8
+    assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == -1
+    assert b"size" in lib.adfwi_last_error()
+    d = _desc(S=0)
+    assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == -2
+    d = _desc()
+    a = nat.AcousticFwdArgs()   # null buffers: rejected before any device access
+    assert lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(a), None) == -2
+    with pytest.raises(ValueError):
+        nat.check(-2, "test")
+
+
+def test_fault_injection_surfaces_as_error():
+    code = ("import ctypes; from adfwi_amd import _native as nat; lib = nat.load(); "
+            "print(lib.adfwi_acoustic_forward(None, None, None))")
+    env = dict(os.environ, ADFWI_FAULT_INJECT="adfwi_acoustic_forward")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, cwd=ROOT)
+    assert out.stdout.strip() == "-4", out.stderr
+
+
+def test_product_refuses_cpu_device():
+    from adfwi_amd import _native as nat
+    with pytest.raises(nat.NativeLibraryError):
+        nat.require_gpu("cpu")
