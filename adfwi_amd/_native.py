@@ -27,9 +27,9 @@ _i64 = ctypes.c_int64
 class AcousticDesc(ctypes.Structure):
     _fields_ = [("abi_version", _i32), ("struct_size", _i32), ("nshots", _i32), ("nz", _i32),
                 ("nx", _i32), ("pitch", _i32), ("fs", _i32), ("free_surface", _i32), ("nt", _i32),
-                ("nrec", _i32), ("shots_per_block", _i32), ("reserved", _i32),
+                ("nrec", _i32), ("shots_per_block", _i32), ("row_unroll", _i32), ("tile_z", _i32), ("reserved", _i32),
                 ("coef", _vp), ("src", _vp), ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp),
-                ("rcv_x", _vp), ("rcv_bin_start", _vp), ("rcv_bin_list", _vp)]
+                ("rcv_x", _vp)]
 
 
 class Layout(ctypes.Structure):
@@ -59,13 +59,10 @@ _SIGNATURES = {
     "adfwi_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "adfwi_last_error": (ctypes.c_char_p, []),
     "adfwi_acoustic_get_layout": (ctypes.c_int, [ctypes.POINTER(AcousticDesc), ctypes.POINTER(Layout)]),
-    "adfwi_acoustic_bin_receivers": (ctypes.c_int, [ctypes.POINTER(AcousticDesc), _vp, _vp, _vp, _vp]),
     "adfwi_acoustic_forward": (ctypes.c_int, [ctypes.POINTER(AcousticDesc),
                                               ctypes.POINTER(AcousticFwdArgs), _vp]),
     "adfwi_acoustic_adjoint": (ctypes.c_int, [ctypes.POINTER(AcousticDesc),
                                               ctypes.POINTER(AcousticAdjArgs), _vp]),
-    "adfwi_acoustic_adjoint_seed": (ctypes.c_int, [ctypes.POINTER(AcousticDesc), _vp, _vp, _vp, _vp,
-                                                   _i32, _vp]),
 }
 
 _lock = threading.Lock()

@@ -1,5 +1,6 @@
 // Acoustic p-u-w staggered-grid propagator for gfx950: fused forward step,
-// fused discrete-adjoint step, and the C ABI around them.
+// fused discrete-adjoint step, receiver gather / adjoint-source injection,
+// and the C ABI around them.
 //
 // One forward step (reference: ADFWI/propagator/acoustic_kernels.py:113-174):
 //   1. p <- (1-K1) p - A1 (c1 (u_j - u_{j-1} + w_i - w_{i-1}) + c2 (u_{j+1} - u_{j-2} + w_{i+1} - w_{i-2}))
@@ -14,425 +15,415 @@
 // compiled with -ffp-contract=off), so a step is bit-identical to the
 // reference's CPU op sequence given identical coefficient fields.
 //
-// A workgroup owns a TZ x TX output tile for a group of shots.  Per shot it
-// stages u/w halos and p on the extended tile [z0-1, z0+TZ+1] x [x0-1, x0+TX+1]
-// in LDS, recomputes the new p there (source and free-surface row included),
-// then updates u and w on the tile.  State is ping-ponged between two buffer
-// sets, so halo reads never race with neighbouring tiles' writes.
+// Work decomposition ("strips"): one wavefront owns 64 consecutive columns
+// x0-3 .. x0+60 of a TZ-row strip and writes the 58 interior columns
+// x0 .. x0+57; the 3 + 3 halo lanes recompute the new pressure their
+// neighbours need.  Each lane marches down its column keeping the vertical
+// stencil windows in registers; horizontal neighbours come from cross-lane
+// shuffles, so a point costs one coalesced load/store per field.  A wave loops
+// over the shots of its shot group, accumulating the energy maps (forward) or
+// the coefficient gradients (adjoint) in registers before one read-modify-
+// write of the group's accumulator plane.  State is ping-ponged between two
+// buffer sets, so halo reads never race with neighbouring strips' writes.
 #include "common.hpp"
 
 namespace {
 
-constexpr int TX = 64;   // tile width: one wavefront per row -> 256-byte coalesced rows
-constexpr int TZ = 8;    // tile height
-constexpr int NT = 256;  // threads per workgroup (4 wavefronts), each thread owns 2 tile points
-constexpr int PTS = TZ * TX / NT;
-constexpr int EH = TZ + 3, EW = TX + 3;  // extended tile rows z0-1.., cols x0-1..
-constexpr int UH = TZ + 3, UW = TX + 6;  // u halo rows z0-1.., cols x0-3..
-constexpr int WH = TZ + 6, WW = TX + 3;  // w halo rows z0-3.., cols x0-1..
+constexpr int WAVE = 64;
+constexpr int HALO_L = 3;                     // lanes left of the first output column
+constexpr int SX = WAVE - 6;                  // output columns per strip (58)
+constexpr int TZ_MAX = 64;                    // output rows per strip: runtime, 8..64
+constexpr int WPB = 4;                        // wavefronts (strips) per workgroup
+constexpr int NT = WAVE * WPB;
 
 constexpr float C1 = 9.0f / 8.0f;
 constexpr float C2 = static_cast<float>(-1.0 / 24.0);
 
 struct AcDev {
   int S, Z, X, pitch, fs, fsurf, nt, nrec;
-  int zbase, ntx, spb;
+  int zbase, nsx, nsz, nstrips, spb, unroll, tz;
   long long plane;  // Z * pitch
   long long field;  // S * Z * pitch
   const float *coef;
   const float *src;
-  const int *src_z, *src_x, *rcv_z, *rcv_x, *bin_start, *bin_list;
+  const int *src_z, *src_x, *rcv_z, *rcv_x;
 };
 
-__device__ __forceinline__ bool win_p(const AcDev &d, int i, int j) {
-  return i >= d.fs + 1 && i <= d.Z - 3 && j >= 2 && j <= d.X - 3;
+__device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, WAVE); }
+
+// Raw buffer access: a scalar resource descriptor + a wave-uniform byte offset (SGPR) + the
+// lane's column byte offset (VGPR), so no per-lane 64-bit address arithmetic is needed.
+// Accesses past num_records read zero / are dropped by the hardware.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int RSRC_DW3 = 0x00020000;  // raw 32-bit data, gfx9 family
+__device__ __forceinline__ rsrc_t rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_DW3);
 }
-__device__ __forceinline__ bool win_u(const AcDev &d, int i, int j) {
-  return i >= d.fs && i <= d.Z - 2 && j >= 1 && j <= d.X - 3;
+// the b32 builtins move raw 32-bit words: bit-cast, never value-convert
+__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-__device__ __forceinline__ bool win_w(const AcDev &d, int i, int j) {
-  return i >= d.fs && i <= d.Z - 3 && j >= 1 && j <= d.X - 2;
-}
-// valid cells of each field: p (Z, X), u (Z, X-1), w (Z-1, X)
-__device__ __forceinline__ bool ok_p(const AcDev &d, int i, int j) {
-  return i >= 0 && i < d.Z && j >= 0 && j < d.X;
-}
-__device__ __forceinline__ bool ok_u(const AcDev &d, int i, int j) {
-  return i >= 0 && i < d.Z && j >= 0 && j < d.X - 1;
-}
-__device__ __forceinline__ bool ok_w(const AcDev &d, int i, int j) {
-  return i >= 0 && i < d.Z - 1 && j >= 0 && j < d.X;
+__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned voff, unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 
-__device__ __forceinline__ void tile_origin(const AcDev &d, int tile, int &z0, int &x0) {
-  const int tz = tile / d.ntx;
-  z0 = d.zbase + tz * TZ;
-  x0 = (tile - tz * d.ntx) * TX;
+// strip index -> (z0, x0).  Workgroups are dispatched round-robin to the 8 XCDs
+// (each with its own L2); this bijection gives every XCD a contiguous run of
+// strips so that halo rows and coefficient lines are shared inside one L2.
+__device__ __forceinline__ bool strip_of(const AcDev &d, int &z0, int &x0) {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int xcd = b & 7, per = n >> 3, rem = n & 7;
+  const int logical = xcd * per + min(xcd, rem) + (b >> 3);
+  // the wave index is uniform across the wavefront: keep it (and all row offsets) scalar
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int strip = logical * WPB + wv;
+  if (strip >= d.nstrips) return false;
+  const int sz = strip / d.nsx;
+  z0 = d.zbase + sz * d.tz;
+  x0 = (strip - sz * d.nsx) * SX;
+  return true;
 }
 
 // ---------------------------------------------------------------------------
 // forward step
 // ---------------------------------------------------------------------------
+template <int SPB, int UNR>
 __global__ __launch_bounds__(NT) void ac_fwd_step(
     AcDev d, const float *__restrict__ in, float *__restrict__ out, int t,
-    float *__restrict__ rec_p, float *__restrict__ rec_u, float *__restrict__ rec_w,
-    float *__restrict__ energy, int acc_u, int acc_w,
-    float *__restrict__ img_d, float *__restrict__ img_p) {
-  __shared__ float s_u[UH][UW];
-  __shared__ float s_w[WH][WW];
-  __shared__ float s_p[EH][EW];
-  __shared__ float s_k1[EH][EW];
-  __shared__ float s_a1[EH][EW];
-  __shared__ float s_tu[TZ][TX];
-  __shared__ float s_tw[TZ][TX];
-
-  const int tile = blockIdx.x;
+    float *__restrict__ energy, int want_u, float *__restrict__ img_d, float *__restrict__ img_p) {
   int z0, x0;
-  tile_origin(d, tile, z0, x0);
-  const int s_begin = blockIdx.y * d.spb;
-  const int s_end = min(d.S, s_begin + d.spb);
-  const int tid = threadIdx.x;
-  const int lx = tid & (TX - 1), lz = tid / TX;
-  const int pitch = d.pitch;
-  const float *k1g = d.coef;
-  const float *a1g = k1g + d.plane;
-  const float *k2g = a1g + d.plane;
-  const float *a2g = k2g + d.plane;
-  const float *k3g = a2g + d.plane;
-
-  // shot-independent coefficients: ext-tile 1-K1 and A1 in LDS, tile K2/A2/K3 in registers
-  for (int k = tid; k < EH * EW; k += NT) {
-    const int r = k / EW, c = k - r * EW;
-    const int i = z0 - 1 + r, j = x0 - 1 + c;
-    const bool ok = ok_p(d, i, j);
-    s_k1[r][c] = ok ? k1g[(long long)i * pitch + j] : 1.0f;
-    s_a1[r][c] = ok ? a1g[(long long)i * pitch + j] : 0.0f;
-  }
-  float ck2[PTS], ca2[PTS], ck3[PTS];
-  float acc_pp[PTS], acc_uu[PTS], acc_ww[PTS];
+  if (!strip_of(d, z0, x0)) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - HALO_L + lane;                 // this lane's column
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);   // clamped load column (bytes)
+  const unsigned vs = 4u * (unsigned)max(c, 0);                 // store column (output lanes only)
+  const bool out_col = lane >= HALO_L && lane < HALO_L + SX && c < d.X;
+  const bool cp = c >= 2 && c <= d.X - 3;           // column part of the windows
+  const bool cu = c >= 1 && c <= d.X - 3;
+  const bool cw = c >= 1 && c <= d.X - 2;
+  const bool u_ok = c < d.X - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch;
+  const unsigned planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  const int s0 = blockIdx.y * SPB;
+  const int ns = min(SPB, d.S - s0);
+  const int zmax = d.Z - 1;
+  const int fs = d.fs;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rP = rsrc(in, fieldb), rU = rsrc(in + d.field, fieldb), rW = rsrc(in + 2 * d.field, fieldb);
+  const rsrc_t oP = rsrc(out, fieldb), oU = rsrc(out + d.field, fieldb), oW = rsrc(out + 2 * d.field, fieldb);
+  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rI = rsrc(img_d, img_d ? fieldb : 0u), rIp = rsrc(img_p, img_p ? fieldb : 0u);
+  const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
+  const unsigned eb = 2u * planeb * blockIdx.y;
+  unsigned so[SPB];
 #pragma unroll
-  for (int q = 0; q < PTS; ++q) {
-    const int i = z0 + lz + q * (NT / TX), j = x0 + lx;
-    const bool ok = ok_p(d, i, j);
-    const long long o = (long long)i * pitch + j;
-    ck2[q] = ok ? k2g[o] : 1.0f;
-    ca2[q] = ok ? a2g[o] : 0.0f;
-    ck3[q] = ok ? k3g[o] : 1.0f;
-    acc_pp[q] = acc_uu[q] = acc_ww[q] = 0.0f;
-  }
+  for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
 
-  for (int s = s_begin; s < s_end; ++s) {
-    const long long so = (long long)s * d.plane;
-    const float *P = in + so;
-    const float *U = in + d.field + so;
-    const float *W = in + 2 * d.field + so;
-    __syncthreads();  // previous shot finished with LDS
-    for (int k = tid; k < UH * UW; k += NT) {
-      const int r = k / UW, c = k - r * UW;
-      const int i = z0 - 1 + r, j = x0 - 3 + c;
-      s_u[r][c] = ok_u(d, i, j) ? U[(long long)i * pitch + j] : 0.0f;
-    }
-    for (int k = tid; k < WH * WW; k += NT) {
-      const int r = k / WW, c = k - r * WW;
-      const int i = z0 - 3 + r, j = x0 - 1 + c;
-      s_w[r][c] = ok_w(d, i, j) ? W[(long long)i * pitch + j] : 0.0f;
-    }
-    for (int k = tid; k < EH * EW; k += NT) {
-      const int r = k / EW, c = k - r * EW;
-      const int i = z0 - 1 + r, j = x0 - 1 + c;
-      s_p[r][c] = ok_p(d, i, j) ? P[(long long)i * pitch + j] : 0.0f;
-    }
-    __syncthreads();
-
-    // new pressure on the extended tile (stage 1 + source)
-    const int zs = d.src_z[s], xs = d.src_x[s];
-    const float sv = d.src[(long long)s * d.nt + t];
-    for (int k = tid; k < EH * EW; k += NT) {
-      const int r = k / EW, c = k - r * EW;
-      const int i = z0 - 1 + r, j = x0 - 1 + c;
-      float pv = s_p[r][c];
-      float dd = 0.0f;
-      if (win_p(d, i, j)) {
-        // u(i, j-2..j+1) -> s_u[r][c .. c+3]; w(i-2..i+1, j) -> s_w[r .. r+3][c]
-        dd = C1 * (((s_u[r][c + 2] - s_u[r][c + 1]) + s_w[r + 2][c]) - s_w[r + 1][c]) +
-             C2 * (((s_u[r][c + 3] - s_u[r][c]) + s_w[r + 3][c]) - s_w[r][c]);
-        pv = s_k1[r][c] * pv - s_a1[r][c] * dd;
-      }
-      if (i == zs && j == xs) pv = pv + sv;
-      s_p[r][c] = pv;
-      if (img_d && r >= 1 && r <= TZ && c >= 1 && c <= TX && ok_p(d, i, j))
-        img_d[so + (long long)i * pitch + j] = dd;
-    }
-    __syncthreads();
-    // stage 3: free-surface mirror (only the first tile row block holds rows fs-1..fs+1)
-    if (d.fsurf) {
-      const int rf = d.fs - 1 - (z0 - 1);
-      if (rf >= 0 && rf + 2 < EH) {
-        for (int c = tid; c < EW; c += NT) s_p[rf][c] = -s_p[rf + 2][c];
-      }
-    }
-    __syncthreads();
-
-    // stages 4-6 on the tile
+  // per-shot windows: pw* = new p rows k-3..k; ww* = old w rows k-2..k+1; uw* = old u rows k-2..k
+  float pw0[SPB], pw1[SPB], pw2[SPB], pw3[SPB];
+  float ww0[SPB], ww1[SPB], ww2[SPB];
+  float uw0[SPB], uw1[SPB];
+  // values of the row being processed, loaded one iteration ahead: W(k+1), U(k), P(k)
+  float cW[SPB], cU[SPB], cP[SPB];
+  float sv[SPB];
+  int zs[SPB], xs[SPB];
 #pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int tr = lz + q * (NT / TX);
-      const int r = tr + 1, c = lx + 1;
-      const int i = z0 + tr, j = x0 + lx;
-      float un = 0.0f, wn = 0.0f;
-      if (ok_p(d, i, j)) {
-        const float pn = s_p[r][c];
-        un = s_u[r][c + 2];
-        if (win_u(d, i, j))
-          un = ck2[q] * un - ca2[q] * (C1 * (s_p[r][c + 1] - s_p[r][c]) + C2 * (s_p[r][c + 2] - s_p[r][c - 1]));
-        wn = s_w[r + 2][c];
-        if (win_w(d, i, j))
-          wn = ck3[q] * wn - ca2[q] * (C1 * (s_p[r + 1][c] - s_p[r][c]) + C2 * (s_p[r + 2][c] - s_p[r - 1][c]));
-        if (d.fsurf && i == d.fs - 1) {
-          // w[fs-1] <- new w[fs]: recompute w at row fs from the same LDS data
-          float w2 = s_w[r + 3][c];
-          if (win_w(d, d.fs, j)) {
-            const long long o2 = (long long)d.fs * pitch + j;
-            w2 = k3g[o2] * w2 - a2g[o2] * (C1 * (s_p[r + 2][c] - s_p[r + 1][c]) + C2 * (s_p[r + 3][c] - s_p[r][c]));
-          }
-          wn = w2;
+  for (int j = 0; j < SPB; ++j) {
+    pw0[j] = pw1[j] = pw2[j] = pw3[j] = 0.0f;
+    uw0[j] = uw1[j] = 0.0f;
+    const int s = s0 + min(j, ns - 1);
+    zs[j] = j < ns ? d.src_z[s] : -1000;
+    xs[j] = d.src_x[s];
+    sv[j] = d.src[(long long)s * d.nt + t];
+    ww0[j] = 0.0f;
+    ww1[j] = bld(rW, vc, so[j] + rb(z0 - 3));
+    ww2[j] = bld(rW, vc, so[j] + rb(z0 - 2));
+    cW[j] = bld(rW, vc, so[j] + rb(z0 - 1));   // becomes w(k+1) once shifted for k = z0-1 ... see below
+  }
+  // first row k = z0-1 needs w rows z0-3..z0: shift so that ww* hold rows k-2..k and cW holds k+1
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) {
+    ww0[j] = ww1[j]; ww1[j] = ww2[j]; ww2[j] = cW[j];
+    cW[j] = bld(rW, vc, so[j] + rb(z0));
+    cU[j] = bld(rU, vc, so[j] + rb(z0 - 1));
+    cP[j] = bld(rP, vc, so[j] + rb(z0 - 1));
+  }
+  float ck1 = bld(rC, vc, rb(z0 - 1)), ca1 = bld(rC, vc, planeb + rb(z0 - 1));
+  float ck2 = bld(rC, vc, 2u * planeb + rb(z0 - 3)), ca2 = bld(rC, vc, 3u * planeb + rb(z0 - 3));
+  float ck3 = bld(rC, vc, 4u * planeb + rb(z0 - 3));
+
+#pragma unroll UNR
+  for (int kk = 0; kk < d.tz + 3; ++kk) {
+    const int k = z0 - 1 + kk;
+    const unsigned rk = rb(k);
+    const int r = k - 2;
+    const unsigned rr = rb(r);
+    // issue the next row's loads (all shots) before this row's stores
+    float nW[SPB], nU[SPB], nP[SPB];
+    const unsigned rn = rb(k + 1), rn1 = rb(k + 2), rrn = rb(r + 1);
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      nW[j] = bld(rW, vc, so[j] + rn1);
+      nU[j] = bld(rU, vc, so[j] + rn);
+      nP[j] = bld(rP, vc, so[j] + rn);
+    }
+    const float nk1 = bld(rC, vc, rn), na1 = bld(rC, vc, planeb + rn);
+    const float nk2 = bld(rC, vc, 2u * planeb + rrn), na2 = bld(rC, vc, 3u * planeb + rrn);
+    const float nk3 = bld(rC, vc, 4u * planeb + rrn);
+
+    const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
+    const bool rowu = cu && r >= fs && r <= d.Z - 2;
+    const bool roww = cw && r >= fs && r <= d.Z - 3;
+    const bool outr = kk >= 3;
+    const bool own = out_col && kk >= 1 && kk <= d.tz && k <= zmax;   // row k belongs to this strip
+    const bool wout = out_col && outr && r <= zmax;
+    float ep = 0.0f, eu = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      if (j < ns) {
+        // windows: w rows k-2..k+1 = ww0, ww1, ww2, cW; u rows k-2..k = uw0, uw1, cU
+        const float w3 = cW[j];
+        float pn = cP[j];
+        // stage 1: new pressure at row k
+        const float u0 = cU[j];
+        const float um2 = shfl(u0, lane - 2), um1 = shfl(u0, lane - 1), up1 = shfl(u0, lane + 1);
+        float dd = 0.0f;
+        if (rowp) {
+          dd = C1 * (((u0 - um1) + ww2[j]) - ww1[j]) + C2 * (((up1 - um2) + w3) - ww0[j]);
+          pn = ck1 * pn - ca1 * dd;
         }
-        const long long o = so + (long long)i * pitch + j;
-        out[o] = pn;
-        if (ok_u(d, i, j)) out[d.field + o] = un; else un = 0.0f;
-        if (ok_w(d, i, j)) out[2 * d.field + o] = wn; else wn = 0.0f;
-        if (img_p) img_p[o] = pn;
-        acc_pp[q] += pn * pn;
-        if (acc_u) acc_uu[q] += un * un;
-        if (acc_w) acc_ww[q] += wn * wn;
+        // stage 2: source; stage 3: free-surface mirror of row fs-1 (two rows back)
+        if (k == zs[j] && c == xs[j]) pn = pn + sv[j];
+        pw0[j] = pw1[j]; pw1[j] = pw2[j]; pw2[j] = pw3[j]; pw3[j] = pn;
+        if (d.fsurf && k == fs + 1) pw1[j] = -pn;
+        if (img_d && own) bst(dd, rI, vs, so[j] + rk);
+        if (outr) {
+          // stages 4-6 at output row r = k-2 (u(r) = uw0, w(r) = ww0)
+          const float pr = pw1[j];
+          const float pl1 = shfl(pr, lane - 1), pr1 = shfl(pr, lane + 1), pr2 = shfl(pr, lane + 2);
+          float un = uw0[j], wn = ww0[j];
+          if (rowu) un = ck2 * un - ca2 * (C1 * (pr1 - pr) + C2 * (pr2 - pl1));
+          if (roww) wn = ck3 * wn - ca2 * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
+          if (wout) {
+            bst(pr, oP, vs, so[j] + rr);
+            if (img_p) bst(pr, rIp, vs, so[j] + rr);
+            if (u_ok) bst(un, oU, vs, so[j] + rr); else un = 0.0f;
+            // stage 6: row fs-1 takes the new row fs, written once that row exists
+            if (r < d.Z - 1 && !(d.fsurf && r == fs - 1)) bst(wn, oW, vs, so[j] + rr);
+            if (d.fsurf && r == fs) bst(wn, oW, vs, so[j] + rr - rowb);
+            ep += pr * pr;
+            eu += un * un;
+          }
+        }
+        // advance the old-state windows by one row
+        ww0[j] = ww1[j]; ww1[j] = ww2[j]; ww2[j] = w3;
+        uw0[j] = uw1[j]; uw1[j] = u0;
       }
-      s_tu[tr][lx] = un;
-      s_tw[tr][lx] = wn;
+      cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j];
     }
-    if (rec_p) {
-      __syncthreads();
-      const int b0 = d.bin_start[tile], b1 = d.bin_start[tile + 1];
-      for (int k = b0 + tid; k < b1; k += NT) {
-        const int rr = d.bin_list[k];
-        const int zi = d.rcv_z[rr] - z0, xi = d.rcv_x[rr] - x0;
-        const long long o = ((long long)s * d.nt + t) * d.nrec + rr;
-        rec_p[o] = s_p[zi + 1][xi + 1];
-        rec_u[o] = s_tu[zi][xi];
-        rec_w[o] = s_tw[zi][xi];
-      }
-    }
-  }
-
-  if (energy) {
-    const long long go = (long long)blockIdx.y * 3 * d.plane;
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int i = z0 + lz + q * (NT / TX), j = x0 + lx;
-      if (!ok_p(d, i, j)) continue;
-      const long long o = go + (long long)i * pitch + j;
-      energy[o] += acc_pp[q];
-      if (acc_u) energy[d.plane + o] += acc_uu[q];
-      if (acc_w) energy[2 * d.plane + o] += acc_ww[q];
+    ck1 = nk1; ca1 = na1; ck2 = nk2; ca2 = na2; ck3 = nk3;
+    if (energy && wout) {
+      bst(bld(rE, vs, eb + rr) + ep, rE, vs, eb + rr);
+      if (want_u) bst(bld(rE, vs, eb + planeb + rr) + eu, rE, vs, eb + planeb + rr);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// adjoint step: transpose of stages 6..1 of step t, plus zero-lag imaging
+// adjoint step: transpose of stages 6..1 of step t, plus zero-lag imaging.
+// Input: adjoint of the state after step t (receiver sources of step t added).
+// Output: adjoint of the state after step t-1 (sources of t-1 not yet added).
 // ---------------------------------------------------------------------------
+template <int SPB, int UNR>
 __global__ __launch_bounds__(NT) void ac_adj_step(
     AcDev d, const float *__restrict__ lin, float *__restrict__ lout, int t,
     const float *__restrict__ img_d, const float *__restrict__ img_p,
-    const float *__restrict__ g_p, const float *__restrict__ g_u, const float *__restrict__ g_w,
     float *__restrict__ grad_a1, float *__restrict__ grad_a2, float *__restrict__ grad_src) {
-  __shared__ float s_fu[UH][UW];   // A2 * masked lambda_u (u halo)
-  __shared__ float s_fw[WH][WW];   // A2 * masked lambda_w' (w halo)
-  __shared__ float s_l[EH][EW];    // lambda_p3 -> lambda_p2 -> A1 * mu_p
-  __shared__ float s_a2u[UH][UW];  // A2 on the u halo, zero outside the u window
-  __shared__ float s_a2w[WH][WW];  // A2 on the w halo, zero outside the w window
-  __shared__ float s_a1[EH][EW];   // A1 on the ext tile, zero outside the p window
-  __shared__ float s_pp[EH][EW];   // pressure image (for dL/dA2)
-  __shared__ float s_tp[TZ][TX], s_tu[TZ][TX], s_tw[TZ][TX];
-
-  const int tile = blockIdx.x;
   int z0, x0;
-  tile_origin(d, tile, z0, x0);
-  const int s_begin = blockIdx.y * d.spb;
-  const int s_end = min(d.S, s_begin + d.spb);
-  const int tid = threadIdx.x;
-  const int lx = tid & (TX - 1), lz = tid / TX;
-  const int pitch = d.pitch;
-  const float *k1g = d.coef;
-  const float *a1g = k1g + d.plane;
-  const float *k2g = a1g + d.plane;
-  const float *a2g = k2g + d.plane;
-  const float *k3g = a2g + d.plane;
+  if (!strip_of(d, z0, x0)) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - HALO_L + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= HALO_L && lane < HALO_L + SX && c < d.X;
+  const bool cp = c >= 2 && c <= d.X - 3;
+  const bool cu = c >= 1 && c <= d.X - 3;
+  const bool cw = c >= 1 && c <= d.X - 2;
+  const unsigned rowb = 4u * (unsigned)d.pitch;
+  const unsigned planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  const int s0 = blockIdx.y * SPB;
+  const int ns = min(SPB, d.S - s0);
+  const int zmax = d.Z - 1;
+  const int fs = d.fs;
+  const int fsr = d.fsurf ? fs : -100;   // row whose adjoint collects the free-surface copy
+  const bool want_a1 = grad_a1 != nullptr && img_d != nullptr;
   const bool want_a2 = grad_a2 != nullptr && img_p != nullptr;
-
-  for (int k = tid; k < UH * UW; k += NT) {
-    const int r = k / UW, c = k - r * UW;
-    const int i = z0 - 1 + r, j = x0 - 3 + c;
-    s_a2u[r][c] = (ok_u(d, i, j) && win_u(d, i, j)) ? a2g[(long long)i * pitch + j] : 0.0f;
-  }
-  for (int k = tid; k < WH * WW; k += NT) {
-    const int r = k / WW, c = k - r * WW;
-    const int i = z0 - 3 + r, j = x0 - 1 + c;
-    s_a2w[r][c] = (ok_w(d, i, j) && win_w(d, i, j)) ? a2g[(long long)i * pitch + j] : 0.0f;
-  }
-  for (int k = tid; k < EH * EW; k += NT) {
-    const int r = k / EW, c = k - r * EW;
-    const int i = z0 - 1 + r, j = x0 - 1 + c;
-    s_a1[r][c] = (ok_p(d, i, j) && win_p(d, i, j)) ? a1g[(long long)i * pitch + j] : 0.0f;
-  }
-  float ck1[PTS], ck2[PTS], ck3[PTS], acc1[PTS], acc2[PTS];
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rP = rsrc(lin, fieldb), rU = rsrc(lin + d.field, fieldb), rW = rsrc(lin + 2 * d.field, fieldb);
+  const rsrc_t oP = rsrc(lout, fieldb), oU = rsrc(lout + d.field, fieldb), oW = rsrc(lout + 2 * d.field, fieldb);
+  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rI = rsrc(img_d, img_d ? fieldb : 0u), rIp = rsrc(img_p, img_p ? fieldb : 0u);
+  const rsrc_t rG1 = rsrc(grad_a1, grad_a1 ? planeb * gridDim.y : 0u);
+  const rsrc_t rG2 = rsrc(grad_a2, grad_a2 ? planeb * gridDim.y : 0u);
+  const unsigned gb = planeb * blockIdx.y;
+  unsigned so[SPB];
 #pragma unroll
-  for (int q = 0; q < PTS; ++q) {
-    const int i = z0 + lz + q * (NT / TX), j = x0 + lx;
-    const bool ok = ok_p(d, i, j);
-    const long long o = (long long)i * pitch + j;
-    ck1[q] = (ok && win_p(d, i, j)) ? k1g[o] : 1.0f;
-    ck2[q] = (ok && win_u(d, i, j)) ? k2g[o] : 1.0f;
-    ck3[q] = (ok && win_w(d, i, j)) ? k3g[o] : 1.0f;
-    acc1[q] = acc2[q] = 0.0f;
-  }
-  const int fsr = d.fsurf ? d.fs : -100;  // rows fs-1 / fs get the free-surface transpose
+  for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
 
-  for (int s = s_begin; s < s_end; ++s) {
-    const long long so = (long long)s * d.plane;
-    const float *LP = lin + so;
-    const float *LU = lin + d.field + so;
-    const float *LW = lin + 2 * d.field + so;
-    __syncthreads();
-    for (int k = tid; k < UH * UW; k += NT) {
-      const int r = k / UW, c = k - r * UW;
-      const int i = z0 - 1 + r, j = x0 - 3 + c;
-      const float a = s_a2u[r][c];
-      s_fu[r][c] = (a != 0.0f) ? a * LU[(long long)i * pitch + j] : 0.0f;
+  // lambda_w' of row kr from its raw value: stage 6 transposed (row fs gathers row fs-1, then zero)
+  auto lwfix = [&](unsigned soj, int kr, float raw) -> float {
+    if (kr == fsr) return raw + bld(rW, vc, soj + rb(kr - 1));
+    if (kr == fsr - 1) return 0.0f;
+    return raw;
+  };
+  auto wmask = [&](int kr) -> bool { return cw && kr >= fs && kr <= d.Z - 3; };
+
+  // shot-independent A2 window on w rows k-2..k+1 (masked)
+  float aw0 = 0.0f;
+  float aw1 = wmask(z0 - 3) ? bld(rC, vc, 3u * planeb + rb(z0 - 3)) : 0.0f;
+  float aw2 = wmask(z0 - 2) ? bld(rC, vc, 3u * planeb + rb(z0 - 2)) : 0.0f;
+  float aw3 = wmask(z0 - 1) ? bld(rC, vc, 3u * planeb + rb(z0 - 1)) : 0.0f;
+  // per-shot windows: lw* = lambda_w' rows k-2..k+1; l2* = lambda_p2 rows k-2..k;
+  // g* = A1 mask lambda_p2 rows k-3..k; lu* = lambda_u rows k-2..k; pp* = pressure image rows k-3..k
+  float lw0[SPB], lw1[SPB], lw2[SPB], lw3[SPB];
+  float l2a[SPB], l2b[SPB], l2c[SPB];
+  float g0[SPB], g1[SPB], g2[SPB], g3[SPB];
+  float lu0[SPB], lu1[SPB], lu2[SPB];
+  float pp0[SPB], pp1[SPB], pp2[SPB], pp3[SPB];
+  // row values loaded one iteration ahead: raw lambda_w(k+1), lambda_u(k), lambda_p(k), images(k)
+  float cW[SPB], cU[SPB], cP[SPB], cI[SPB], cIp[SPB];
+  int zs[SPB], xs[SPB];
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) {
+    lw0[j] = 0.0f;
+    l2a[j] = l2b[j] = l2c[j] = 0.0f;
+    g0[j] = g1[j] = g2[j] = g3[j] = 0.0f;
+    lu0[j] = lu1[j] = lu2[j] = 0.0f;
+    pp0[j] = pp1[j] = pp2[j] = pp3[j] = 0.0f;
+    const int s = s0 + min(j, ns - 1);
+    zs[j] = j < ns ? d.src_z[s] : -1000;
+    xs[j] = d.src_x[s];
+    lw1[j] = lwfix(so[j], z0 - 3, bld(rW, vc, so[j] + rb(z0 - 3)));
+    lw2[j] = lwfix(so[j], z0 - 2, bld(rW, vc, so[j] + rb(z0 - 2)));
+    lw3[j] = lwfix(so[j], z0 - 1, bld(rW, vc, so[j] + rb(z0 - 1)));
+    cW[j] = bld(rW, vc, so[j] + rb(z0));
+    cU[j] = bld(rU, vc, so[j] + rb(z0 - 1));
+    cP[j] = bld(rP, vc, so[j] + rb(z0 - 1));
+    cI[j] = want_a1 ? bld(rI, vc, so[j] + rb(z0 - 1)) : 0.0f;
+    cIp[j] = want_a2 ? bld(rIp, vc, so[j] + rb(z0 - 1)) : 0.0f;
+  }
+  float cA2w = wmask(z0) ? bld(rC, vc, 3u * planeb + rb(z0)) : 0.0f;      // A2 at row k+1 (w window)
+  float cA2u = bld(rC, vc, 3u * planeb + rb(z0 - 1));                   // A2 at row k (u)
+  float cA1 = bld(rC, vc, planeb + rb(z0 - 1));
+  float cK1 = bld(rC, vc, rb(z0 - 3)), cK2 = bld(rC, vc, 2u * planeb + rb(z0 - 3));
+  float cK3 = bld(rC, vc, 4u * planeb + rb(z0 - 3));
+
+#pragma unroll UNR
+  for (int kk = 0; kk < d.tz + 3; ++kk) {
+    const int k = z0 - 1 + kk;
+    const unsigned rk = rb(k);
+    const int r = k - 2;
+    const unsigned rr = rb(r);
+    // issue the next row's loads (all shots) before this row's stores
+    float nW[SPB], nU[SPB], nP[SPB], nI[SPB], nIp[SPB];
+    const unsigned rn = rb(k + 1), rn1 = rb(k + 2), rrn = rb(r + 1);
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      nW[j] = bld(rW, vc, so[j] + rn1);
+      nU[j] = bld(rU, vc, so[j] + rn);
+      nP[j] = bld(rP, vc, so[j] + rn);
+      nI[j] = want_a1 ? bld(rI, vc, so[j] + rn) : 0.0f;
+      nIp[j] = want_a2 ? bld(rIp, vc, so[j] + rn) : 0.0f;
     }
-    for (int k = tid; k < WH * WW; k += NT) {
-      const int r = k / WW, c = k - r * WW;
-      const int i = z0 - 3 + r, j = x0 - 1 + c;
-      const float a = s_a2w[r][c];
-      float v = 0.0f;
-      if (a != 0.0f) {
-        // stage-6 transpose: lambda_w'[fs] = lambda_w[fs] + lambda_w[fs-1]; row fs-1 is not in the window
-        v = LW[(long long)i * pitch + j];
-        if (i == fsr) v = v + LW[(long long)(i - 1) * pitch + j];
-        v = a * v;
-      }
-      s_fw[r][c] = v;
-    }
-    for (int k = tid; k < EH * EW; k += NT) {
-      const int r = k / EW, c = k - r * EW;
-      const int i = z0 - 1 + r, j = x0 - 1 + c;
-      s_l[r][c] = ok_p(d, i, j) ? LP[(long long)i * pitch + j] : 0.0f;
-      if (want_a2) s_pp[r][c] = ok_p(d, i, j) ? img_p[so + (long long)i * pitch + j] : 0.0f;
-    }
-    __syncthreads();
-    // stages 5/4 transposed into lambda_p3 on the ext tile
-    for (int k = tid; k < EH * EW; k += NT) {
-      const int r = k / EW, c = k - r * EW;
-      const float dxt = C1 * (s_fu[r][c + 1] - s_fu[r][c + 2]) + C2 * (s_fu[r][c] - s_fu[r][c + 3]);
-      const float dzt = C1 * (s_fw[r + 1][c] - s_fw[r + 2][c]) + C2 * (s_fw[r][c] - s_fw[r + 3][c]);
-      s_l[r][c] = s_l[r][c] - (dxt + dzt);
-    }
-    __syncthreads();
-    // stage 3 transposed: lambda_p2[fs+1] -= lambda_p3[fs-1]; lambda_p2[fs-1] = 0
-    if (d.fsurf) {
-      const int rf = d.fs - 1 - (z0 - 1);
-      if (rf >= 0 && rf + 2 < EH) {
-        for (int c = tid; c < EW; c += NT) {
-          const float m = s_l[rf][c];
-          s_l[rf + 2][c] = s_l[rf + 2][c] - m;
-          s_l[rf][c] = 0.0f;
+    const float nA2w = wmask(k + 2) ? bld(rC, vc, 3u * planeb + rn1) : 0.0f;
+    const float nA2u = bld(rC, vc, 3u * planeb + rn), nA1 = bld(rC, vc, planeb + rn);
+    const float nK1 = bld(rC, vc, rrn), nK2 = bld(rC, vc, 2u * planeb + rrn), nK3 = bld(rC, vc, 4u * planeb + rrn);
+
+    const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
+    const bool rowu_k = cu && k >= fs && k <= d.Z - 2;
+    const bool pmr = cp && r >= fs + 1 && r <= d.Z - 3;
+    const bool um = cu && r >= fs && r <= d.Z - 2;
+    const bool wm = cw && r >= fs && r <= d.Z - 3;
+    const bool outr = kk >= 3;
+    const bool own = out_col && kk >= 1 && kk <= d.tz && k <= zmax;
+    const bool wout = out_col && outr && r <= zmax;
+    aw0 = aw1; aw1 = aw2; aw2 = aw3; aw3 = cA2w;
+    const float a2k = rowu_k ? cA2u : 0.0f;
+    const float a1k = rowp ? cA1 : 0.0f;
+    const float k1r = pmr ? cK1 : 1.0f;
+    const float k2r = um ? cK2 : 1.0f;
+    const float k3r = wm ? cK3 : 1.0f;
+    float ga1 = 0.0f, ga2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      if (j < ns) {
+        lw0[j] = lw1[j]; lw1[j] = lw2[j]; lw2[j] = lw3[j]; lw3[j] = lwfix(so[j], k + 1, cW[j]);
+        lu0[j] = lu1[j]; lu1[j] = lu2[j]; lu2[j] = cU[j];
+        g0[j] = g1[j]; g1[j] = g2[j]; g2[j] = g3[j];
+        l2a[j] = l2b[j]; l2b[j] = l2c[j];
+        if (want_a2) { pp0[j] = pp1[j]; pp1[j] = pp2[j]; pp2[j] = pp3[j]; pp3[j] = cIp[j]; }
+        // stages 5/4 transposed: lambda_p3(k)
+        const float fu = a2k * lu2[j];
+        const float fum1 = shfl(fu, lane - 1), fum2 = shfl(fu, lane - 2), fup1 = shfl(fu, lane + 1);
+        const float dxt = C1 * (fum1 - fu) + C2 * (fum2 - fup1);
+        const float dzt = C1 * (aw1 * lw1[j] - aw2 * lw2[j]) + C2 * (aw0 * lw0[j] - aw3 * lw3[j]);
+        const float l3 = cP[j] - (dxt + dzt);
+        // stage 3 transposed: lambda_p2[fs+1] = lambda_p3[fs+1] - lambda_p3[fs-1]; lambda_p2[fs-1] = 0
+        float l2 = l3;
+        if (d.fsurf && k == fs + 1) {
+          l2 = l3 - l2a[j];
+          l2a[j] = 0.0f;
+        }
+        l2c[j] = l2;
+        // stage 2 transposed: d loss / d src[s, t]
+        if (grad_src && own && k == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t] = l2;
+        g3[j] = a1k * l2;
+        if (want_a1 && own && rowp) ga1 -= l2 * cI[j];
+        if (outr) {
+          // stage 1 transposed and the three outputs at row r = k-2
+          const float gc = g1[j];
+          const float gl1 = shfl(gc, lane - 1), gr1 = shfl(gc, lane + 1), gr2 = shfl(gc, lane + 2);
+          const float op = k1r * l2a[j];
+          const float ou = k2r * lu0[j] - (C1 * (gc - gr1) + C2 * (gl1 - gr2));
+          const float ow = k3r * lw0[j] - (C1 * (gc - g2[j]) + C2 * (g0[j] - g3[j]));
+          if (want_a2) {
+            const float pc = pp1[j];
+            const float pl = shfl(pc, lane - 1), pr1 = shfl(pc, lane + 1), pr2 = shfl(pc, lane + 2);
+            if (um) ga2 -= lu0[j] * (C1 * (pr1 - pc) + C2 * (pr2 - pl));
+            if (wm) ga2 -= lw0[j] * (C1 * (pp2[j] - pc) + C2 * (pp3[j] - pp0[j]));
+          }
+          if (wout) {
+            bst(op, oP, vs, so[j] + rr);
+            if (c < d.X - 1) bst(ou, oU, vs, so[j] + rr);
+            if (r < d.Z - 1) bst(ow, oW, vs, so[j] + rr);
+          }
         }
       }
+      cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j]; cI[j] = nI[j]; cIp[j] = nIp[j];
     }
-    __syncthreads();
-    // stage 2 transposed: d loss / d src[s, t] = lambda_p2 at the source
-    const int zs = d.src_z[s], xs = d.src_x[s];
-    if (grad_src && tid == 0 && zs >= z0 && zs < z0 + TZ && xs >= x0 && xs < x0 + TX)
-      grad_src[(long long)s * d.nt + t] = s_l[zs - z0 + 1][xs - x0 + 1];
-    float lp1[PTS];
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) lp1[q] = s_l[lz + q * (NT / TX) + 1][lx + 1];
-    __syncthreads();
-    for (int k = tid; k < EH * EW; k += NT) {
-      const int r = k / EW, c = k - r * EW;
-      s_l[r][c] = s_a1[r][c] * s_l[r][c];
-    }
-    __syncthreads();
-    // stage 1 transposed + outputs on the tile
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int tr = lz + q * (NT / TX);
-      const int r = tr + 1, c = lx + 1;
-      const int i = z0 + tr, j = x0 + lx;
-      float op = 0.0f, ou = 0.0f, ow = 0.0f;
-      if (ok_p(d, i, j)) {
-        const long long o = (long long)i * pitch + j;
-        const bool wp = win_p(d, i, j);
-        op = wp ? ck1[q] * lp1[q] : lp1[q];
-        if (ok_u(d, i, j)) {
-          const float lu = LU[o];
-          ou = ck2[q] * lu - (C1 * (s_l[r][c] - s_l[r][c + 1]) + C2 * (s_l[r][c - 1] - s_l[r][c + 2]));
-          if (want_a2 && win_u(d, i, j))
-            acc2[q] -= lu * (C1 * (s_pp[r][c + 1] - s_pp[r][c]) + C2 * (s_pp[r][c + 2] - s_pp[r][c - 1]));
-        }
-        if (ok_w(d, i, j)) {
-          float lw = LW[o];
-          if (i == fsr) lw = lw + LW[o - pitch];
-          if (i == fsr - 1) lw = 0.0f;
-          ow = ck3[q] * lw - (C1 * (s_l[r][c] - s_l[r + 1][c]) + C2 * (s_l[r - 1][c] - s_l[r + 2][c]));
-          if (want_a2 && win_w(d, i, j))
-            acc2[q] -= lw * (C1 * (s_pp[r + 1][c] - s_pp[r][c]) + C2 * (s_pp[r + 2][c] - s_pp[r - 1][c]));
-        }
-        if (grad_a1 && wp) acc1[q] -= lp1[q] * img_d[so + o];
-      }
-      s_tp[tr][lx] = op;
-      s_tu[tr][lx] = ou;
-      s_tw[tr][lx] = ow;
-    }
-    // receiver adjoint sources of step t-1 join the state after step t-1
-    if (t >= 1 && (g_p || g_u || g_w)) {
-      __syncthreads();
-      const int b0 = d.bin_start[tile], b1 = d.bin_start[tile + 1];
-      for (int k = b0 + tid; k < b1; k += NT) {
-        const int rr = d.bin_list[k];
-        const int zi = d.rcv_z[rr] - z0, xi = d.rcv_x[rr] - x0;
-        const long long o = ((long long)s * d.nt + (t - 1)) * d.nrec + rr;
-        if (g_p) atomicAdd(&s_tp[zi][xi], g_p[o]);
-        if (g_u) atomicAdd(&s_tu[zi][xi], g_u[o]);
-        if (g_w) atomicAdd(&s_tw[zi][xi], g_w[o]);
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < PTS; ++q) {
-      const int tr = lz + q * (NT / TX);
-      const int i = z0 + tr, j = x0 + lx;
-      if (!ok_p(d, i, j)) continue;
-      const long long o = so + (long long)i * pitch + j;
-      lout[o] = s_tp[tr][lx];
-      if (ok_u(d, i, j)) lout[d.field + o] = s_tu[tr][lx];
-      if (ok_w(d, i, j)) lout[2 * d.field + o] = s_tw[tr][lx];
-    }
-  }
-
-  const long long go = (long long)blockIdx.y * d.plane;
-#pragma unroll
-  for (int q = 0; q < PTS; ++q) {
-    const int i = z0 + lz + q * (NT / TX), j = x0 + lx;
-    if (!ok_p(d, i, j)) continue;
-    const long long o = go + (long long)i * pitch + j;
-    if (grad_a1) grad_a1[o] += acc1[q];
-    if (want_a2) grad_a2[o] += acc2[q];
+    cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
+    // image terms of row k (stencil image) and row r (pressure image) into the group planes
+    if (want_a1 && own && rowp) bst(bld(rG1, vs, gb + rk) + ga1, rG1, vs, gb + rk);
+    if (want_a2 && wout) bst(bld(rG2, vs, gb + rr) + ga2, rG2, vs, gb + rr);
   }
 }
 
-// lam[field][s][zr][xr] += g[s][t][r] for the three record components
-__global__ void ac_seed(AcDev d, float *__restrict__ lam, const float *__restrict__ g_p,
-                        const float *__restrict__ g_u, const float *__restrict__ g_w, int t) {
+// rec_{p,u,w}[s][t][r] = state at the receivers after step t
+__global__ void ac_gather(AcDev d, const float *__restrict__ st, float *__restrict__ rp, float *__restrict__ ru,
+                          float *__restrict__ rw, int t) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)d.S * d.nrec) return;
+  const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
+  const long long o = (long long)s * d.plane + (long long)d.rcv_z[r] * d.pitch + d.rcv_x[r];
+  const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
+  rp[ro] = st[o];
+  ru[ro] = st[d.field + o];
+  rw[ro] = st[2 * d.field + o];
+}
+
+// lam[field][s][zr][xr] += g[s][t][r]: receiver adjoint sources of step t
+__global__ void ac_inject(AcDev d, float *__restrict__ lam, const float *__restrict__ g_p,
+                          const float *__restrict__ g_u, const float *__restrict__ g_w, int t) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long long)d.S * d.nrec) return;
   const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
@@ -463,27 +454,46 @@ int layout_of(const adfwi_acoustic_desc *d, adfwi_layout *L) {
   } else {
     ADFWI_CHECK_ARG(d->fs == 1, "fs must be 1 without a free surface (got %d)", d->fs);
   }
-  L->tile_z = TZ;
-  L->tile_x = TX;
   L->zbase = d->free_surface ? d->fs - 1 : 0;
-  L->ntiles_z = (d->nz - L->zbase + TZ - 1) / TZ;
-  L->ntiles_x = (d->nx + TX - 1) / TX;
-  L->ntiles = L->ntiles_z * L->ntiles_x;
+  L->ntiles_x = (d->nx + SX - 1) / SX;
+  L->tile_x = SX;
+  // shots per wavefront (1, 2, 4 or 8, a kernel template) and strip height (4..64 rows): the
+  // largest shot group (<= 4) with 16-row strips that still gives >= 4096 wavefronts (about four
+  // per SIMD on 256 CUs), then shorter strips for small problems.  Measured on MI355X for a
+  // 64-shot 464 x 1232 grid: 4 shots x 16 rows beats 2 shots and 32/64-row strips.
+  const int rows = d->nz - L->zbase;
+  auto waves = [&](int spb_, int tz_) {
+    return (long long)L->ntiles_x * ((rows + tz_ - 1) / tz_) * ((d->nshots + spb_ - 1) / spb_);
+  };
   int spb = d->shots_per_block;
-  if (spb <= 0) {
-    // enough workgroups to fill 256 CUs several times over, as few shot groups as that allows
-    const long long want = 4096;
-    long long groups = (want + L->ntiles - 1) / L->ntiles;
-    if (groups > d->nshots) groups = d->nshots;
-    if (groups < 1) groups = 1;
-    spb = (int)((d->nshots + groups - 1) / groups);
+  int tz = d->tile_z;
+  ADFWI_CHECK_ARG(spb == 0 || spb == 1 || spb == 2 || spb == 4 || spb == 8,
+                  "shots_per_block must be 0, 1, 2, 4 or 8 (got %d)", spb);
+  ADFWI_CHECK_ARG(tz == 0 || (tz >= 4 && tz <= TZ_MAX), "tile_z must be 0 or 4..%d (got %d)", TZ_MAX, tz);
+  if (spb == 0) {
+    spb = 1;
+    for (int cand = 4; cand >= 1; cand >>= 1)
+      if (waves(cand, tz ? tz : 16) >= 4096) { spb = cand; break; }
   }
-  if (spb > d->nshots) spb = d->nshots;
+  if (tz == 0) {
+    tz = 4;
+    for (int cand = 16; cand >= 4; cand >>= 1)
+      if (waves(spb, cand) >= 4096) { tz = cand; break; }
+  }
+  L->tile_z = tz;
+  L->ntiles_z = (rows + tz - 1) / tz;
+  L->ntiles = L->ntiles_z * L->ntiles_x;
+  ADFWI_CHECK_ARG(d->row_unroll == 0 || d->row_unroll == 1 || d->row_unroll == 2 || d->row_unroll == 4,
+                  "row_unroll must be 0, 1, 2 or 4 (got %d)", d->row_unroll);
   L->shots_per_block = spb;
   L->ngroups = (d->nshots + spb - 1) / spb;
   L->nfields = 3;
   L->reserved = 0;
   L->field_floats = (int64_t)d->nshots * d->nz * d->pitch;
+  // buffer descriptors address one field (all shots) with 32-bit byte offsets
+  ADFWI_CHECK_ARG(L->field_floats * 4 < ((int64_t)1 << 31),
+                  "one field of %d shots x %d x %d exceeds 2 GiB; advance the shots in smaller launches",
+                  d->nshots, d->nz, d->pitch);
   L->state_floats = 3 * L->field_floats;
   L->accum_floats = (int64_t)L->ngroups * d->nz * d->pitch;
   return ADFWI_OK;
@@ -491,9 +501,7 @@ int layout_of(const adfwi_acoustic_desc *d, adfwi_layout *L) {
 
 int check_device_desc(const adfwi_acoustic_desc *d, bool need_records) {
   ADFWI_CHECK_ARG(d->coef && d->src && d->src_z && d->src_x, "null coefficient/source pointer");
-  if (need_records && d->nrec > 0)
-    ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x && d->rcv_bin_start && d->rcv_bin_list,
-                    "null receiver table pointer");
+  if (need_records && d->nrec > 0) ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
   return ADFWI_OK;
 }
 
@@ -501,12 +509,56 @@ AcDev make_dev(const adfwi_acoustic_desc *d, const adfwi_layout &L) {
   AcDev v;
   v.S = d->nshots; v.Z = d->nz; v.X = d->nx; v.pitch = d->pitch;
   v.fs = d->fs; v.fsurf = d->free_surface ? 1 : 0; v.nt = d->nt; v.nrec = d->nrec;
-  v.zbase = L.zbase; v.ntx = L.ntiles_x; v.spb = L.shots_per_block;
+  v.zbase = L.zbase; v.nsx = L.ntiles_x; v.nsz = L.ntiles_z; v.nstrips = L.ntiles; v.tz = L.tile_z;
+  v.spb = L.shots_per_block;
+  v.unroll = d->row_unroll > 0 ? d->row_unroll : 2;
   v.plane = (long long)d->nz * d->pitch;
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.src = d->src; v.src_z = d->src_z; v.src_x = d->src_x;
-  v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x; v.bin_start = d->rcv_bin_start; v.bin_list = d->rcv_bin_list;
+  v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
   return v;
+}
+
+dim3 step_grid(const adfwi_layout &L) { return dim3((L.ntiles + WPB - 1) / WPB, L.ngroups); }
+
+template <int SPB>
+void launch_fwd_u(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in, float *out, int t, float *e,
+                  int acc_u, float *imd, float *imp) {
+  switch (v.unroll) {
+    case 4: hipLaunchKernelGGL((ac_fwd_step<SPB, 4>), g, b, 0, st, v, in, out, t, e, acc_u, imd, imp); break;
+    case 2: hipLaunchKernelGGL((ac_fwd_step<SPB, 2>), g, b, 0, st, v, in, out, t, e, acc_u, imd, imp); break;
+    default: hipLaunchKernelGGL((ac_fwd_step<SPB, 1>), g, b, 0, st, v, in, out, t, e, acc_u, imd, imp);
+  }
+}
+
+void launch_fwd(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in, float *out, int t, float *e,
+                int acc_u, float *imd, float *imp) {
+  switch (v.spb) {
+    case 8: launch_fwd_u<8>(v, g, b, st, in, out, t, e, acc_u, imd, imp); break;
+    case 4: launch_fwd_u<4>(v, g, b, st, in, out, t, e, acc_u, imd, imp); break;
+    case 2: launch_fwd_u<2>(v, g, b, st, in, out, t, e, acc_u, imd, imp); break;
+    default: launch_fwd_u<1>(v, g, b, st, in, out, t, e, acc_u, imd, imp);
+  }
+}
+
+template <int SPB>
+void launch_adj_u(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in, float *out, int t,
+                  const float *imd, const float *imp, float *ga1, float *ga2, float *gs) {
+  switch (v.unroll) {
+    case 4: hipLaunchKernelGGL((ac_adj_step<SPB, 4>), g, b, 0, st, v, in, out, t, imd, imp, ga1, ga2, gs); break;
+    case 2: hipLaunchKernelGGL((ac_adj_step<SPB, 2>), g, b, 0, st, v, in, out, t, imd, imp, ga1, ga2, gs); break;
+    default: hipLaunchKernelGGL((ac_adj_step<SPB, 1>), g, b, 0, st, v, in, out, t, imd, imp, ga1, ga2, gs);
+  }
+}
+
+void launch_adj(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in, float *out, int t,
+                const float *imd, const float *imp, float *ga1, float *ga2, float *gs) {
+  switch (v.spb) {
+    case 8: launch_adj_u<8>(v, g, b, st, in, out, t, imd, imp, ga1, ga2, gs); break;
+    case 4: launch_adj_u<4>(v, g, b, st, in, out, t, imd, imp, ga1, ga2, gs); break;
+    case 2: launch_adj_u<2>(v, g, b, st, in, out, t, imd, imp, ga1, ga2, gs); break;
+    default: launch_adj_u<1>(v, g, b, st, in, out, t, imd, imp, ga1, ga2, gs);
+  }
 }
 
 }  // namespace
@@ -516,35 +568,6 @@ extern "C" {
 int adfwi_acoustic_get_layout(const adfwi_acoustic_desc *d, adfwi_layout *out) {
   ADFWI_CHECK_ARG(out != nullptr, "null layout pointer");
   return layout_of(d, out);
-}
-
-int adfwi_acoustic_bin_receivers(const adfwi_acoustic_desc *d, const int32_t *rz, const int32_t *rx,
-                                 int32_t *start, int32_t *list) {
-  adfwi_layout L;
-  int rc = layout_of(d, &L);
-  if (rc) return rc;
-  ADFWI_CHECK_ARG(start && (d->nrec == 0 || (rz && rx && list)), "null receiver binning array");
-  for (int k = 0; k <= L.ntiles; ++k) start[k] = 0;
-  for (int r = 0; r < d->nrec; ++r) {
-    const int z = rz[r], x = rx[r];
-    ADFWI_CHECK_ARG(z >= 0 && z < d->nz - 1 && x >= 0 && x < d->nx - 1,
-                    "receiver %d at padded (%d, %d) outside the grid %d x %d", r, z, x, d->nz, d->nx);
-    if (z < L.zbase) continue;  // rows above the free surface never change: records stay zero
-    const int tile = ((z - L.zbase) / TZ) * L.ntiles_x + x / TX;
-    start[tile + 1]++;
-  }
-  for (int k = 0; k < L.ntiles; ++k) start[k + 1] += start[k];
-  // stable fill
-  int *fill = new int[L.ntiles];
-  for (int k = 0; k < L.ntiles; ++k) fill[k] = start[k];
-  for (int r = 0; r < d->nrec; ++r) {
-    const int z = rz[r], x = rx[r];
-    if (z < L.zbase) continue;
-    const int tile = ((z - L.zbase) / TZ) * L.ntiles_x + x / TX;
-    list[fill[tile]++] = r;
-  }
-  delete[] fill;
-  return ADFWI_OK;
 }
 
 int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args *a, adfwi_stream_t stream) {
@@ -563,7 +586,9 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   ADFWI_CHECK_ARG(!a->ckpt || a->ckpt_every >= 1, "ckpt_every must be >= 1 with a checkpoint buffer");
   AcDev v = make_dev(d, L);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const dim3 grid(L.ntiles, L.ngroups), block(NT);
+  const dim3 grid = step_grid(L), block(NT);
+  const long long ng = (long long)d->nshots * d->nrec;
+  const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
   const float *cur = a->state_in;
   for (int k = 0; k < a->nsteps; ++k) {
     const int t = a->t0 + k;
@@ -576,10 +601,12 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
     float *imd = a->image ? a->image + (long long)k * L.field_floats : nullptr;
     float *imp = a->image_p ? a->image_p + (long long)k * L.field_floats : nullptr;
     const int acc_u = (a->energy && t >= a->energy_u_from) ? 1 : 0;
-    const int acc_w = (a->energy && t == d->nt - 1) ? 1 : 0;
-    hipLaunchKernelGGL(ac_fwd_step, grid, block, 0, st, v, cur, nxt, t, a->rec_p, a->rec_u, a->rec_w,
-                       a->energy, acc_u, acc_w, imd, imp);
+    launch_fwd(v, grid, block, st, cur, nxt, t, a->energy, acc_u, imd, imp);
     if ((rc = adfwi::check_launch("ac_fwd_step"))) return rc;
+    if (recs && ng > 0) {
+      hipLaunchKernelGGL(ac_gather, ggrid, gblock, 0, st, v, nxt, a->rec_p, a->rec_u, a->rec_w, t);
+      if ((rc = adfwi::check_launch("ac_gather"))) return rc;
+    }
     cur = nxt;
   }
   a->state_out = cur;
@@ -595,6 +622,9 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   const bool recs = a->grad_rec_p || a->grad_rec_u || a->grad_rec_w;
   if ((rc = check_device_desc(d, recs))) return rc;
   ADFWI_CHECK_ARG(a->lam_in && a->work[0] && a->work[1], "null adjoint state/work buffer");
+  ADFWI_CHECK_ARG(a->work[0] != a->work[1], "work[0] and work[1] must be distinct buffers");
+  ADFWI_CHECK_ARG(a->lam_in == a->work[0] || a->lam_in == a->work[1],
+                  "lam_in must be work[0] or work[1] (receiver sources are added in place)");
   ADFWI_CHECK_ARG(a->t_hi < d->nt && a->nsteps >= 0 && a->t_hi - a->nsteps + 1 >= 0,
                   "adjoint steps %d..%d outside [0, %d)", a->t_hi - a->nsteps + 1, a->t_hi, d->nt);
   ADFWI_CHECK_ARG(!a->grad_a1 || a->image, "grad_a1 needs the stencil image");
@@ -603,38 +633,26 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
                   "image_t0 %d after the first adjoint step %d", a->image_t0, a->t_hi - a->nsteps + 1);
   AcDev v = make_dev(d, L);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const dim3 grid(L.ntiles, L.ngroups), block(NT);
-  const float *cur = a->lam_in;
+  const dim3 grid = step_grid(L), block(NT);
+  const long long ng = (long long)d->nshots * d->nrec;
+  const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  float *cur = const_cast<float *>(a->lam_in);
   for (int k = 0; k < a->nsteps; ++k) {
     const int t = a->t_hi - k;
+    if (recs && ng > 0) {
+      hipLaunchKernelGGL(ac_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec_p, a->grad_rec_u, a->grad_rec_w, t);
+      if ((rc = adfwi::check_launch("ac_inject"))) return rc;
+    }
     float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
     const long long slot = (long long)(t - a->image_t0) * L.field_floats;
     const float *imd = a->image ? a->image + slot : nullptr;
     const float *imp = a->image_p ? a->image_p + slot : nullptr;
-    hipLaunchKernelGGL(ac_adj_step, grid, block, 0, st, v, cur, nxt, t, imd, imp, a->grad_rec_p,
-                       a->grad_rec_u, a->grad_rec_w, a->grad_a1, a->grad_a2, a->grad_src);
+launch_adj(v, grid, block, st, cur, nxt, t, imd, imp, a->grad_a1, a->grad_a2, a->grad_src);
     if ((rc = adfwi::check_launch("ac_adj_step"))) return rc;
     cur = nxt;
   }
   a->lam_out = cur;
   return ADFWI_OK;
-}
-
-int adfwi_acoustic_adjoint_seed(const adfwi_acoustic_desc *d, float *lam, const float *grad_rec_p,
-                                const float *grad_rec_u, const float *grad_rec_w, int32_t t,
-                                adfwi_stream_t stream) {
-  adfwi_layout L;
-  int rc = layout_of(d, &L);
-  if (rc) return rc;
-  ADFWI_CHECK_ARG(lam != nullptr, "null adjoint state");
-  ADFWI_CHECK_ARG(t >= 0 && t < d->nt, "seed step %d outside [0, %d)", t, d->nt);
-  if (d->nrec == 0 || !(grad_rec_p || grad_rec_u || grad_rec_w)) return ADFWI_OK;
-  ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
-  AcDev v = make_dev(d, L);
-  const long long n = (long long)d->nshots * d->nrec;
-  hipLaunchKernelGGL(ac_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     reinterpret_cast<hipStream_t>(stream), v, lam, grad_rec_p, grad_rec_u, grad_rec_w, t);
-  return adfwi::check_launch("ac_seed");
 }
 
 }  // extern "C"

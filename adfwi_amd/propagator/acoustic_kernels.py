@@ -10,6 +10,7 @@ checkpoints and segment recomputation.
 """
 import ctypes
 import math
+import os
 from typing import Dict, Optional
 
 import numpy as np
@@ -103,7 +104,7 @@ class AcousticGeometry:
         rcv_z = np.asarray(rcv_z, dtype=np.int64).reshape(-1)
         rcv_x = np.asarray(rcv_x, dtype=np.int64).reshape(-1)
         self.S, self.R = len(src_z), len(rcv_z)
-        zmin = self.fs - 1 if free_surface else 0
+        zmin = self.fs if free_surface else 0
         if self.S < 1:
             raise ValueError("at least one shot is required")
         if np.any(src_z < zmin) or np.any(src_z > self.Z - 1) or np.any(src_x < 0) or np.any(src_x > self.X - 1):
@@ -112,25 +113,18 @@ class AcousticGeometry:
                        or np.any(rcv_x > self.X - 2)):
             raise ValueError(f"receiver index outside the padded grid {self.Z} x {self.X} "
                              "(u has X-1 columns and w has Z-1 rows)")
+        if not shots_per_block:
+            shots_per_block = int(os.environ.get("ADFWI_SHOTS_PER_BLOCK", "0"))
         self.shots_per_block = int(shots_per_block)
+        self.row_unroll = int(os.environ.get("ADFWI_ROW_UNROLL", "0"))
+        self.tile_z = int(os.environ.get("ADFWI_TILE_Z", "0"))
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
-        self._keep = []
         desc = self.desc(None, None)
         self.layout = nat.Layout()
         nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),
                   "adfwi_acoustic_get_layout")
-        start = np.zeros(self.layout.ntiles + 1, dtype=np.int32)
-        lst = np.zeros(max(self.R, 1), dtype=np.int32)
-        rz32 = np.ascontiguousarray(rcv_z.astype(np.int32))
-        rx32 = np.ascontiguousarray(rcv_x.astype(np.int32))
-        nat.check(nat.load().adfwi_acoustic_bin_receivers(
-            ctypes.byref(desc), rz32.ctypes.data_as(ctypes.c_void_p), rx32.ctypes.data_as(ctypes.c_void_p),
-            start.ctypes.data_as(ctypes.c_void_p), lst.ctypes.data_as(ctypes.c_void_p)),
-            "adfwi_acoustic_bin_receivers")
-        self.bin_start = torch.tensor(start, device=self.device)
-        self.bin_list = torch.tensor(lst, device=self.device)
 
     def desc(self, coef: Optional[torch.Tensor], src: Optional[torch.Tensor]) -> nat.AcousticDesc:
         d = nat.AcousticDesc()
@@ -139,12 +133,12 @@ class AcousticGeometry:
         d.nshots, d.nz, d.nx, d.pitch = self.S, self.Z, self.X, self.pitch
         d.fs, d.free_surface, d.nt, d.nrec = self.fs, int(self.free_surface), self.nt, self.R
         d.shots_per_block = self.shots_per_block
+        d.row_unroll = self.row_unroll
+        d.tile_z = self.tile_z
         d.coef = None if coef is None else coef.data_ptr()
         d.src = None if src is None else src.data_ptr()
         d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
         d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
-        if hasattr(self, "bin_start"):
-            d.rcv_bin_start, d.rcv_bin_list = self.bin_start.data_ptr(), self.bin_list.data_ptr()
         return d
 
     # -- buffers ----------------------------------------------------------
@@ -159,18 +153,17 @@ class AcousticGeometry:
 
 
 def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *, record: bool = True,
-                energy_u_from: Optional[int] = None, ckpt_every: int = 0, image: bool = False,
-                image_p: bool = False):
+                energy_u_from: Optional[int] = None, ckpt_every: int = 0):
     """Runs all nt forward steps from a zero state.
 
-    Returns dict(rec=[3 x (S, nt, R)] or None, energy=(3, Z, X) summed over shot groups or None,
+    Returns dict(rec=[3 x (S, nt, R)] or None, energy=(3, Z, X) or None: sum over steps and shots of
+    p^2, sum over steps >= energy_u_from of u^2, and sum over shots of w^2 after the last step;
     ckpt=(n, 3, S, Z, pitch) or None).
     """
     lib = nat.load()
     dev = geom.device
     S, Z, X, nt, R = geom.S, geom.Z, geom.X, geom.nt, geom.R
     L = geom.layout
-    state0 = geom.state(1)[0]
     work = geom.state(2)
     ckpt = None
     if ckpt_every:
@@ -181,10 +174,10 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
         rec = [torch.zeros((S, nt, R), device=dev, dtype=torch.float32) for _ in range(3)]
     energy = None
     if energy_u_from is not None:
-        energy = torch.zeros((L.ngroups, 3, Z, geom.pitch), device=dev, dtype=torch.float32)
+        energy = torch.zeros((L.ngroups, 2, Z, geom.pitch), device=dev, dtype=torch.float32)
     d = geom.desc(coef, src)
     a = nat.AcousticFwdArgs()
-    a.state_in = state0.data_ptr()
+    a.state_in = work[0].data_ptr()          # zero initial state
     a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
     a.ckpt = None if ckpt is None else ckpt.data_ptr()
     a.ckpt_every = int(ckpt_every)
@@ -198,7 +191,13 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
         "adfwi_acoustic_forward"))
     out = {"rec": rec, "ckpt": ckpt, "energy": None}
     if energy is not None:
-        out["energy"] = energy.sum(0)[:, :, :X]
+        final = work[0] if a.state_out == work[0].data_ptr() else (
+            work[1] if a.state_out == work[1].data_ptr() else None)
+        if final is None:   # last step wrote a checkpoint slot (only when nt is a multiple of k, never)
+            raise RuntimeError("final state not in a work buffer")
+        e = energy.sum(0)[:, :, :X]
+        e_w = final[2, :, :, :X].square().sum(0)
+        out["energy"] = torch.stack([e[0], e[1], e_w])
     return out
 
 
@@ -226,9 +225,7 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
     gp = [None if x is None else x.data_ptr() for x in g]
     d = geom.desc(coef, src)
     stream = nat.stream_handle(dev)
-    nat.check(lib.adfwi_acoustic_adjoint_seed(ctypes.byref(d), lw[0].data_ptr(), gp[0], gp[1], gp[2],
-                                              nt - 1, stream), "adfwi_acoustic_adjoint_seed")
-    lam = lw[0]
+    lam = lw[0]   # zero adjoint state after the last step; its receiver sources are added in the call
     need_image = need_a1 or need_a2
     for seg in reversed(range(nseg)):
         t_lo = seg * k

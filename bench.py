@@ -39,6 +39,17 @@ BYTES_ADJ = 28.0          # adjoint: read + write adjoint p, u, w + stencil imag
 BYTES_GRAD = 80.0         # per shot-gradient: 24 + 28 + 28
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summaries, or None."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic*.md"))):
+        for line in open(f):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def l2_misfit(syn, obs, dt):
     """Waveform L2 misfit sum_{s,r} sqrt(sum_t rsd^2 dt) (ADFWI/fwi/misfit/L2.py:26-29)."""
     rsd = obs - syn
@@ -255,11 +266,11 @@ def main():
         "gpt_updates_per_s": 2 * S * N * wl.nt * args.steps / elapsed / 1e9,
         "grad_bytes_roofline_frac": BYTES_GRAD * S * N * wl.nt * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "avg_launch_ms": kern[dom]["avg_launch_ms"]},
         "kernels": kern,
-        "loss": float(loss),
+        "loss": float(loss.detach()),
     }
     log(f"timed region {elapsed:.2f} s, {value:.2f} shot-gradients/s")
     if not args.no_baselines:

@@ -61,20 +61,20 @@ typedef struct adfwi_acoustic_desc {
   int32_t free_surface;    /* 0 / 1 */
   int32_t nt;              /* time steps of the whole record */
   int32_t nrec;            /* R: receivers shared by all shots */
-  int32_t shots_per_block; /* shots looped inside one workgroup; 0 = library choice */
+  int32_t shots_per_block; /* shots advanced by one wavefront: 1, 2, 4, 8; 0 = library choice */
+  int32_t row_unroll;      /* row-loop unroll of the step kernels: 1, 2, 4; 0 = library choice */
+  int32_t tile_z;          /* rows of one strip: 4..64; 0 = library choice */
   int32_t reserved;
   const float *coef;       /* [5][nz][pitch]: 1-K1, A1, 1-K2, A2, 1-K3 */
   const float *src;        /* [nshots][nt]: dt * wavelet, injected into p */
   const int32_t *src_z;    /* [nshots] padded source rows */
   const int32_t *src_x;    /* [nshots] padded source columns */
-  const int32_t *rcv_z;    /* [nrec] padded receiver rows */
-  const int32_t *rcv_x;    /* [nrec] padded receiver columns */
-  const int32_t *rcv_bin_start; /* [ntiles+1], see adfwi_acoustic_bin_receivers */
-  const int32_t *rcv_bin_list;  /* [nrec] */
+  const int32_t *rcv_z;    /* [nrec] padded receiver rows (< nz-1) */
+  const int32_t *rcv_x;    /* [nrec] padded receiver columns (< nx-1) */
 } adfwi_acoustic_desc;
 
 typedef struct adfwi_layout {
-  int32_t tile_z, tile_x;  /* output tile of one workgroup */
+  int32_t tile_z, tile_x;  /* output tile (strip) of one wavefront */
   int32_t zbase;           /* first grid row covered by tiles (rows above never change) */
   int32_t ntiles_z, ntiles_x, ntiles;
   int32_t shots_per_block; /* effective value */
@@ -89,11 +89,6 @@ typedef struct adfwi_layout {
 /* Tiling and buffer sizes for a descriptor (host only, no device access). */
 int adfwi_acoustic_get_layout(const adfwi_acoustic_desc *d, adfwi_layout *out);
 
-/* Host-side counting sort of receivers by output tile.  rz/rx are host arrays
- * of nrec padded indices; start must hold ntiles+1 and list nrec entries. */
-int adfwi_acoustic_bin_receivers(const adfwi_acoustic_desc *d, const int32_t *rz,
-                                 const int32_t *rx, int32_t *start, int32_t *list);
-
 typedef struct adfwi_acoustic_fwd_args {
   const float *state_in;   /* [3][S][Z][pitch] p,u,w before step t0 */
   float *work[2];          /* two scratch state sets */
@@ -102,7 +97,8 @@ typedef struct adfwi_acoustic_fwd_args {
   int32_t t0, nsteps;      /* advance steps t0 .. t0+nsteps-1 */
   int32_t energy_u_from;   /* first step whose u^2 enters the u energy map */
   float *rec_p, *rec_u, *rec_w; /* [S][nt][R] or NULL */
-  float *energy;           /* [ngroups][3][Z][pitch] (+=) or NULL */
+  float *energy;           /* [ngroups][2][Z][pitch] (+=) or NULL: sum over shots of p^2 every
+                              step and of u^2 from step energy_u_from on */
   float *image;            /* [nsteps][S][Z][pitch] or NULL: p-update stencil of each step */
   float *image_p;          /* [nsteps][S][Z][pitch] or NULL: p after each step */
   const float *state_out;  /* OUT: buffer holding the state after the last step */
@@ -113,7 +109,8 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
                            adfwi_stream_t stream);
 
 typedef struct adfwi_acoustic_adj_args {
-  const float *lam_in;     /* adjoint state after step t_hi, records of t_hi included */
+  const float *lam_in;     /* adjoint state after step t_hi, receiver sources of t_hi not yet
+                              added; must be work[0] or work[1] (sources are added in place) */
   float *work[2];
   int32_t t_hi, nsteps;    /* adjoints of steps t_hi, t_hi-1, ..., t_hi-nsteps+1 */
   int32_t image_t0;        /* image slot k holds step image_t0 + k */
@@ -123,18 +120,13 @@ typedef struct adfwi_acoustic_adj_args {
   const float *grad_rec_p, *grad_rec_u, *grad_rec_w; /* [S][nt][R] or NULL */
   float *grad_a1, *grad_a2; /* [ngroups][Z][pitch] (+=) or NULL */
   float *grad_src;          /* [S][nt] or NULL: d loss / d src */
-  const float *lam_out;     /* OUT: adjoint state after step t_hi-nsteps (records included) */
+  const float *lam_out;     /* OUT: adjoint state after step t_hi-nsteps (its sources not added) */
 } adfwi_acoustic_adj_args;
 
 /* Discrete adjoint of adfwi_acoustic_forward (the transpose of every stage of
  * acoustic_kernels.py:115-169), with zero-lag imaging of dL/dA1 and dL/dA2. */
 int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a,
                            adfwi_stream_t stream);
-
-/* lam += receiver adjoint sources of step t (seeds the adjoint at t = nt-1). */
-int adfwi_acoustic_adjoint_seed(const adfwi_acoustic_desc *d, float *lam,
-                                const float *grad_rec_p, const float *grad_rec_u,
-                                const float *grad_rec_w, int32_t t, adfwi_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -124,25 +124,18 @@ def _desc(S=3, Z=60, X=150, fs=1, free_surface=0, nt=10, nrec=5):
     return d
 
 
-def test_layout_and_receiver_bins_host_side():
+def test_layout_host_side():
     from adfwi_amd import _native as nat
     lib = nat.load()
     d = _desc(free_surface=1, fs=20)
     L = nat.Layout()
     assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == 0
-    assert L.zbase == 19 and L.ntiles_x == 3 and L.ntiles == L.ntiles_x * L.ntiles_z
-    assert L.ngroups * L.shots_per_block >= 3
-    rz = np.array([21, 21, 40, 58, 5], dtype=np.int32)   # last one above the free surface
-    rx = np.array([0, 70, 70, 148, 3], dtype=np.int32)
-    start = np.zeros(L.ntiles + 1, dtype=np.int32); lst = np.full(5, -1, dtype=np.int32)
-    rc = lib.adfwi_acoustic_bin_receivers(ctypes.byref(d), rz.ctypes.data_as(ctypes.c_void_p),
-                                          rx.ctypes.data_as(ctypes.c_void_p), start.ctypes.data_as(ctypes.c_void_p),
-                                          lst.ctypes.data_as(ctypes.c_void_p))
-    assert rc == 0 and start[-1] == 4
-    for t in range(L.ntiles):
-        for k in range(start[t], start[t + 1]):
-            r = lst[k]
-            assert (rz[r] - L.zbase) // L.tile_z * L.ntiles_x + rx[r] // L.tile_x == t
+    assert L.zbase == 19 and L.ntiles == L.ntiles_x * L.ntiles_z
+    assert L.ntiles_x * L.tile_x >= 150 and L.zbase + L.ntiles_z * L.tile_z >= 60
+    assert L.ngroups * L.shots_per_block >= 3 and (L.ngroups - 1) * L.shots_per_block < 3
+    assert L.state_floats == 3 * 3 * 60 * d.pitch
+    d.shots_per_block = 2
+    assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == 0 and L.ngroups == 2
 
 
 def test_abi_rejects_bad_arguments_with_messages():
