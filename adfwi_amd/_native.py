@@ -53,6 +53,23 @@ class AcousticAdjArgs(ctypes.Structure):
                 ("grad_a1", _vp), ("grad_a2", _vp), ("grad_src", _vp), ("lam_out", _vp)]
 
 
+_f32 = ctypes.c_float
+
+
+class ElasticDesc(ctypes.Structure):
+    _fields_ = [("abi_version", _i32), ("struct_size", _i32), ("nshots", _i32), ("nz", _i32), ("nx", _i32),
+                ("pitch", _i32), ("free_surface", _i32), ("nt", _i32), ("nrec", _i32), ("aniso", _i32),
+                ("shots_per_wave", _i32), ("tile_z", _i32), ("dt", _f32), ("dt_dx", _f32), ("dt_dz", _f32),
+                ("dx", _f32), ("dz", _f32), ("c0", _f32), ("c1", _f32), ("reserved", _i32),
+                ("coef", _vp), ("pml_x", _vp), ("pml_z", _vp), ("src", _vp), ("src_m", _vp),
+                ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp)]
+
+
+class ElasticFwdArgs(ctypes.Structure):
+    _fields_ = [("state_in", _vp), ("work", _vp * 2), ("ckpt", _vp), ("ckpt_every", _i32), ("t0", _i32),
+                ("nsteps", _i32), ("reserved", _i32), ("rec", _vp * 5), ("image", _vp), ("state_out", _vp)]
+
+
 # function name -> (restype, argtypes)
 _SIGNATURES = {
     "adfwi_abi_version": (ctypes.c_int, []),
@@ -63,6 +80,8 @@ _SIGNATURES = {
                                               ctypes.POINTER(AcousticFwdArgs), _vp]),
     "adfwi_acoustic_adjoint": (ctypes.c_int, [ctypes.POINTER(AcousticDesc),
                                               ctypes.POINTER(AcousticAdjArgs), _vp]),
+    "adfwi_elastic_get_layout": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(Layout)]),
+    "adfwi_elastic_forward": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticFwdArgs), _vp]),
 }
 
 _lock = threading.Lock()

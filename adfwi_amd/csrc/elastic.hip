@@ -1,0 +1,423 @@
+// Elastic (isotropic / VTI / general 2-D anisotropic) velocity-stress propagator for gfx950:
+// fused forward step with split-field PML, receiver gather, and the C ABI.
+//
+// One forward step (reference: ADFWI/propagator/elastic_kernels.py:339-418, 4th order, PML),
+// on the window rows [2, Z-3] x cols [2, X-3] of the padded grid:
+//   1. stress splits  s_x <- (pxn s_x + dt/dx (C.Dx v)) * (1/pxd),  s_z <- (pzn s_z + dt/dz (C.Dz v)) * (1/pzd)
+//      for txx, tzz, txz with the D operators of the reference (Dxb/Dzb for the normal
+//      stresses, Dxf/Dzf for the shear stress) applied to the velocity totals;
+//   2. moment-tensor source: -MT/2 * w(t) into both splits of each stress component;
+//   3. stress totals = split sums; free surface (rows 0..2): tzz[2]=0, tzz[1]=-tzz[3],
+//      txz[1]=-txz[2], txz[0]=-txz[3];
+//   4. velocity splits v_x <- (pxn v_x + dt b Dx(s) / dx) / pxd, v_z likewise with pzn/pzd;
+//   5. velocity totals; free surface (rows 0..1, cols [2, X-3]): vz[1]=vz[2],
+//      vx[1]=vz[1,j+1]-vz[1,j]+vz[2,j+1]-vz[2,j]+vx[3,j], vz[0]=vz[1];
+//   6. records of the five totals.
+// Every expression keeps the reference's fp32 association (compiled with
+// -ffp-contract=off, IEEE division), so a step is bit-identical to the reference's CPU
+// op sequence given identical coefficients.  C15/C35 terms are skipped for isotropic and
+// VTI media (exact: they multiply zeros).
+//
+// State: the ten split fields [10][S][Z][pitch] in the order txx_x, txx_z, tzz_x, tzz_z,
+// txz_x, txz_z, vx_x, vx_z, vz_x, vz_z.  The splits are identically zero outside the window
+// (never updated there); totals and free-surface rows are derived when needed, so the
+// reference's five total arrays need no storage.
+//
+// Work decomposition: one wavefront owns 64 columns x0-4 .. x0+59 of a TZ-row strip and
+// writes the 55 columns x0 .. x0+54: stress is valid on lanes 2..60 (it reads velocities
+// 2 columns away, plus one more column for the free-surface row), velocity on lanes 4..58.
+// Each lane marches down its column: at iteration k it loads the velocity splits of row
+// k+2, updates the stress of row k and the velocity of row k-2, keeping 5-row windows of
+// velocity and stress totals in registers.  The four wavefronts of a workgroup advance
+// four different shots of the same strip (their gradient images share one LDS reduction).
+#include "common.hpp"
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int EHL = 4;              // lanes left of the first output column
+constexpr int ESX = 55;             // output columns per strip
+constexpr int EWPB = 4;             // wavefronts per workgroup
+constexpr int ENT = WAVE * EWPB;
+constexpr int ETZ_MAX = 64;
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int RSRC_DW3 = 0x00020000;  // raw 32-bit data, gfx9 family
+__device__ __forceinline__ rsrc_t rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_DW3);
+}
+// the b32 builtins move raw 32-bit words: bit-cast, never value-convert
+__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned voff, unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, WAVE); }
+
+enum { TXX_X = 0, TXX_Z, TZZ_X, TZZ_Z, TXZ_X, TXZ_Z, VX_X, VX_Z, VZ_X, VZ_Z, NFIELD };
+enum { C11 = 0, C13, C33, C55, BX, BZ, C15, C35, NCOEF };
+
+struct ElDev {
+  int S, Z, X, pitch, fsurf, nt, nrec, spw;     // spw: shots per wavefront (runtime loop)
+  int tz, nsx, nstrips;
+  float dt, dt_dx, dt_dz, dx, dz, c0, c1;
+  long long plane, field;
+  const float *coef, *pml_x, *pml_z, *src, *src_m;
+  const int *src_z, *src_x, *rcv_z, *rcv_x;
+};
+
+__device__ __forceinline__ bool strip_of(const ElDev &d, int &z0, int &x0) {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int xcd = b & 7, per = n >> 3, rem = n & 7;
+  const int strip = xcd * per + min(xcd, rem) + (b >> 3);   // XCD-contiguous strips
+  if (strip >= d.nstrips) return false;
+  const int sz = strip / d.nsx;
+  z0 = 2 + sz * d.tz;                       // strips tile the window rows [2, Z-3]
+  x0 = 2 + (strip - sz * d.nsx) * ESX;      // and the window columns [2, X-3]
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// forward step
+// ---------------------------------------------------------------------------
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restrict__ in, float *__restrict__ out,
+                                                   int t, float *__restrict__ img) {
+  int z0, x0;
+  if (!strip_of(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - EHL + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= EHL && lane < EHL + ESX && c <= d.X - 3;   // c >= 2 by the strip origin
+  const bool cwin = c >= 2 && c <= d.X - 3;
+  const int zmax = d.Z - 1, zw1 = d.Z - 3;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rIn = rsrc(in, (unsigned)NFIELD * fieldb);
+  const rsrc_t rOut = rsrc(out, (unsigned)NFIELD * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const rsrc_t rImg = rsrc(img, img ? 5u * fieldb : 0u);
+  const float c0 = d.c0, c1 = d.c1;
+  // per-column PML factors (x); rows come from pml_z
+  const float pxn = d.pml_x[min(max(c, 0), d.X - 1)];
+  const float pxd = d.pml_x[d.pitch + min(max(c, 0), d.X - 1)];
+  const float pxi = d.pml_x[2 * d.pitch + min(max(c, 0), d.X - 1)];
+  const int end_row = min(z0 + d.tz, zw1 + 1);   // one past the last output row
+  const bool jj = c >= 2 && c <= d.X - 3;          // free-surface velocity columns J
+  const bool jj1 = c + 1 >= 2 && c + 1 <= d.X - 3;
+
+  const int s_first = (blockIdx.y * EWPB + wv) * d.spw;
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = s_first + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float wsrc = d.src[(long long)s * d.nt + t];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    // windows: velocity totals rows k-2..k+2; stress totals txz/tzz rows k-4..k, txx rows k-2..k
+    float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;
+    float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
+    float xz0 = 0.f, xz1 = 0.f, xz2 = 0.f, xz3 = 0.f, xz4 = 0.f;
+    float zz0 = 0.f, zz1 = 0.f, zz2 = 0.f, zz3 = 0.f, zz4 = 0.f;
+    float xx0 = 0.f, xx1 = 0.f, xx2 = 0.f;
+    // prologue: velocity rows z0-4 .. z0-1 (k = z0-2 needs rows z0-4 .. z0)
+    {
+      const int k = z0 - 2;
+      for (int q = -2; q <= 1; ++q) {
+        const int rr = k + q;
+        const unsigned o = rb(rr);
+        const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
+        const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
+        const bool ok = rr >= 0;
+        vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
+        vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
+      }
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < d.tz + 4; ++kk) {
+      const int k = z0 - 2 + kk;
+      // 1. velocity totals of row k+2 into the window (zero outside the grid)
+      {
+        const int rr = k + 2;
+        const unsigned o = rb(rr);
+        const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
+        const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
+        const bool ok = rr <= zmax;
+        vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
+        vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
+      }
+      // free surface of the velocities of the previous step: rows 0, 1 from rows 2, 3
+      if (d.fsurf && k == 2) {
+        // window rows 0..4 = vx0..vx4
+        const float vz2n = shfl(vz2, lane + 1);             // vz row 2, column j+1
+        const float vz1m_n = jj1 ? vz2n : 0.0f;             // modified vz row 1 at column j+1
+        const float vz1m = jj ? vz2 : vz1;
+        if (jj) {
+          vx1 = (((vz1m_n - vz1m) + vz2n) - vz2) + vx3;
+          vz0 = vz1m;
+        }
+        vz1 = vz1m;
+      }
+      if (img && kk >= 2 && kk < d.tz + 2 && out_col && k < end_row) {
+        bst(vx2, rImg, vs, 0u * fieldb + so + rb(k));
+        bst(vz2, rImg, vs, 1u * fieldb + so + rb(k));
+      }
+      if (img && d.fsurf && k == 2 && out_col) {
+        bst(vx0, rImg, vs, 0u * fieldb + so + rb(0));
+        bst(vz0, rImg, vs, 1u * fieldb + so + rb(0));
+        bst(vx1, rImg, vs, 0u * fieldb + so + rb(1));
+        bst(vz1, rImg, vs, 1u * fieldb + so + rb(1));
+      }
+      // 2. stress of row k
+      const unsigned ok_ = rb(k);
+      float sxx_x = bld(rIn, vc, F(TXX_X) + ok_), sxx_z = bld(rIn, vc, F(TXX_Z) + ok_);
+      float szz_x = bld(rIn, vc, F(TZZ_X) + ok_), szz_z = bld(rIn, vc, F(TZZ_Z) + ok_);
+      float sxz_x = bld(rIn, vc, F(TXZ_X) + ok_), sxz_z = bld(rIn, vc, F(TXZ_Z) + ok_);
+      // x derivatives of the velocity totals of row k (all lanes take part in the shuffles)
+      const float vxm1 = shfl(vx2, lane - 1), vxm2 = shfl(vx2, lane - 2), vxp1 = shfl(vx2, lane + 1);
+      const float vzm1 = shfl(vz2, lane - 1), vzp1 = shfl(vz2, lane + 1), vzp2 = shfl(vz2, lane + 2);
+      float vzm2 = 0.0f, vxp2 = 0.0f;
+      if constexpr (ANISO) {
+        vzm2 = shfl(vz2, lane - 2);
+        vxp2 = shfl(vx2, lane + 2);
+      }
+      if (k >= 2 && k <= zw1 && cwin) {
+        const float dxb_vx = c0 * (vx2 - vxm1) + c1 * (vxp1 - vxm2);
+        const float dzb_vz = c0 * (vz2 - vz1) + c1 * (vz3 - vz0);
+        const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
+        const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
+        const unsigned cr = ok_;
+        const float C11v = bld(rC, vc, C11 * planeb + cr), C13v = bld(rC, vc, C13 * planeb + cr);
+        const float C33v = bld(rC, vc, C33 * planeb + cr), C55v = bld(rC, vc, C55 * planeb + cr);
+        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        float sx_xx, sz_xx, sx_zz, sz_zz, sx_xz, sz_xz;
+        if constexpr (ANISO) {
+          const float dxb_vz = c0 * (vz2 - vzm1) + c1 * (vzp1 - vzm2);
+          const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
+          const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
+          const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
+          const float C15v = bld(rC, vc, C15 * planeb + cr), C35v = bld(rC, vc, C35 * planeb + cr);
+          sx_xx = C11v * dxb_vx + C15v * dxb_vz;
+          sz_xx = C15v * dzb_vx + C13v * dzb_vz;
+          sx_zz = C13v * dxb_vx + C35v * dxb_vz;
+          sz_zz = C35v * dzb_vx + C33v * dzb_vz;
+          sx_xz = C15v * dxf_vx + C55v * dxf_vz;
+          sz_xz = C55v * dzf_vx + C35v * dzf_vz;
+        } else {
+          sx_xx = C11v * dxb_vx;
+          sz_xx = C13v * dzb_vz;
+          sx_zz = C13v * dxb_vx;
+          sz_zz = C33v * dzb_vz;
+          sx_xz = C55v * dxf_vz;
+          sz_xz = C55v * dzf_vx;
+        }
+        sxx_x = (pxn * sxx_x + d.dt_dx * sx_xx) * pxi;
+        sxx_z = (pzn * sxx_z + d.dt_dz * sz_xx) * pzi;
+        szz_x = (pxn * szz_x + d.dt_dx * sx_zz) * pxi;
+        szz_z = (pzn * szz_z + d.dt_dz * sz_zz) * pzi;
+        sxz_x = (pxn * sxz_x + d.dt_dx * sx_xz) * pxi;
+        sxz_z = (pzn * sxz_z + d.dt_dz * sz_xz) * pzi;
+      }
+      if (k == zs && c == xs) {
+        sxx_x = sxx_x + mxx * wsrc; sxx_z = sxx_z + mxx * wsrc;
+        szz_x = szz_x + mzz * wsrc; szz_z = szz_z + mzz * wsrc;
+        sxz_x = sxz_x + mxz * wsrc; sxz_z = sxz_z + mxz * wsrc;
+      }
+      const bool own_k = out_col && k >= z0 && k < end_row;
+      if (own_k) {
+        bst(sxx_x, rOut, vs, F(TXX_X) + ok_); bst(sxx_z, rOut, vs, F(TXX_Z) + ok_);
+        bst(szz_x, rOut, vs, F(TZZ_X) + ok_); bst(szz_z, rOut, vs, F(TZZ_Z) + ok_);
+        bst(sxz_x, rOut, vs, F(TXZ_X) + ok_); bst(sxz_z, rOut, vs, F(TXZ_Z) + ok_);
+      }
+      xx0 = xx1; xx1 = xx2; xx2 = sxx_x + sxx_z;
+      xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = sxz_x + sxz_z;
+      zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = szz_x + szz_z;
+      // 3. free surface of the stresses once row 3 exists (window rows -1..3 = *0..*4)
+      if (d.fsurf && k == 3) {
+        zz3 = 0.0f;         // tzz[2] = 0
+        zz2 = -zz4;         // tzz[1] = -tzz[3]
+        xz2 = -xz3;         // txz[1] = -txz[2]
+        xz1 = -xz4;         // txz[0] = -txz[3]
+        if (img && out_col) {
+          bst(xz1, rImg, vs, 4u * fieldb + so + rb(0));
+          bst(xz2, rImg, vs, 4u * fieldb + so + rb(1));
+          bst(zz1, rImg, vs, 3u * fieldb + so + rb(0));
+          bst(zz2, rImg, vs, 3u * fieldb + so + rb(1));
+          bst(xx0, rImg, vs, 2u * fieldb + so + rb(1));
+        }
+      }
+      // 4. velocity of row r = k-2 (stress rows r-2..r+2 = *0..*4 for txz/tzz, txx row r = xx0)
+      const int r = k - 2;
+      const float txp1 = shfl(xx0, lane + 1), txp2 = shfl(xx0, lane + 2), txm1 = shfl(xx0, lane - 1);
+      const float tzm1 = shfl(xz2, lane - 1), tzm2 = shfl(xz2, lane - 2), tzp1 = shfl(xz2, lane + 1);
+      const bool own_r = out_col && r >= z0 && r < end_row;
+      if (own_r) {
+        if (img) {
+          bst(xx0, rImg, vs, 2u * fieldb + so + rb(r));
+          bst(zz2, rImg, vs, 3u * fieldb + so + rb(r));
+          bst(xz2, rImg, vs, 4u * fieldb + so + rb(r));
+        }
+        const unsigned orr = rb(r);
+        const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
+        const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
+        const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
+        const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
+        const float bxv = bld(rC, vc, BX * planeb + orr), bzv = bld(rC, vc, BZ * planeb + orr);
+        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float ux_x = bld(rIn, vc, F(VX_X) + orr), ux_z = bld(rIn, vc, F(VX_Z) + orr);
+        const float uz_x = bld(rIn, vc, F(VZ_X) + orr), uz_z = bld(rIn, vc, F(VZ_Z) + orr);
+        const float nx_x = (pxn * ux_x + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * ux_z + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        const float nz_x = (pxn * uz_x + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        const float nz_z = (pzn * uz_z + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        bst(nx_x, rOut, vs, F(VX_X) + orr); bst(nx_z, rOut, vs, F(VX_Z) + orr);
+        bst(nz_x, rOut, vs, F(VZ_X) + orr); bst(nz_z, rOut, vs, F(VZ_Z) + orr);
+      }
+    }
+  }
+}
+
+// five records of step t from the state after step t (totals; free-surface tzz row 2 is zero)
+__global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restrict__ r_txx,
+                          float *__restrict__ r_tzz, float *__restrict__ r_txz, float *__restrict__ r_vx,
+                          float *__restrict__ r_vz, int t) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)d.S * d.nrec) return;
+  const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
+  const int z = d.rcv_z[r];
+  const long long o = (long long)s * d.plane + (long long)z * d.pitch + d.rcv_x[r];
+  const long long f = d.field;
+  const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
+  r_txx[ro] = st[TXX_X * f + o] + st[TXX_Z * f + o];
+  r_tzz[ro] = (d.fsurf && z == 2) ? 0.0f : st[TZZ_X * f + o] + st[TZZ_Z * f + o];
+  r_txz[ro] = st[TXZ_X * f + o] + st[TXZ_Z * f + o];
+  r_vx[ro] = st[VX_X * f + o] + st[VX_Z * f + o];
+  r_vz[ro] = st[VZ_X * f + o] + st[VZ_Z * f + o];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
+  if (!d) { adfwi::set_error("null descriptor"); return ADFWI_E_ARG; }
+  if (d->abi_version != ADFWI_ABI_VERSION || d->struct_size != (int32_t)sizeof(adfwi_elastic_desc)) {
+    adfwi::set_error("elastic descriptor version %d size %d, library expects %d / %d", d->abi_version,
+                     d->struct_size, ADFWI_ABI_VERSION, (int)sizeof(adfwi_elastic_desc));
+    return ADFWI_E_VERSION;
+  }
+  ADFWI_CHECK_ARG(d->nshots >= 1, "nshots must be >= 1 (got %d)", d->nshots);
+  ADFWI_CHECK_ARG(d->nz >= 9 && d->nx >= 9, "padded grid %d x %d too small (need >= 9 x 9)", d->nz, d->nx);
+  ADFWI_CHECK_ARG(d->pitch >= d->nx, "pitch %d < nx %d", d->pitch, d->nx);
+  ADFWI_CHECK_ARG(d->nt >= 1 && d->nrec >= 0, "nt %d / nrec %d invalid", d->nt, d->nrec);
+  ADFWI_CHECK_ARG(d->shots_per_wave >= 0 && d->tile_z >= 0 && d->tile_z <= ETZ_MAX,
+                  "shots_per_wave %d / tile_z %d invalid", d->shots_per_wave, d->tile_z);
+  const int rows = d->nz - 4;   // window rows [2, Z-3]
+  L->zbase = 2;
+  L->tile_x = ESX;
+  L->ntiles_x = (d->nx - 4 + ESX - 1) / ESX;
+  // strips of 16 rows (shorter for small grids); shots per wavefront so that there are
+  // about 4096 wavefronts, as few shot groups as that allows (fewer gradient planes)
+  int tz = d->tile_z;
+  if (tz == 0) {
+    tz = 4;
+    for (int cand = 16; cand >= 4; cand >>= 1)
+      if ((long long)L->ntiles_x * ((rows + cand - 1) / cand) * d->nshots >= 4096) { tz = cand; break; }
+  }
+  L->tile_z = tz;
+  L->ntiles_z = (rows + tz - 1) / tz;
+  L->ntiles = L->ntiles_z * L->ntiles_x;
+  int spw = d->shots_per_wave;
+  if (spw == 0) {
+    const long long strips = L->ntiles;
+    spw = 1;
+    while (spw < d->nshots && strips * EWPB * ((d->nshots + EWPB * spw * 2 - 1) / (EWPB * spw * 2)) >= 4096)
+      spw *= 2;
+  }
+  L->shots_per_block = spw * EWPB;
+  L->ngroups = (d->nshots + L->shots_per_block - 1) / L->shots_per_block;
+  L->nfields = NFIELD;
+  L->reserved = spw;
+  L->field_floats = (int64_t)d->nshots * d->nz * d->pitch;
+  ADFWI_CHECK_ARG(L->field_floats * 4 * NFIELD < ((int64_t)1 << 32) - 1,
+                  "state of %d shots x %d x %d exceeds the 4 GiB of one buffer descriptor; "
+                  "advance the shots in smaller launches", d->nshots, d->nz, d->pitch);
+  L->state_floats = NFIELD * L->field_floats;
+  L->accum_floats = (int64_t)L->ngroups * d->nz * d->pitch;
+  return ADFWI_OK;
+}
+
+ElDev el_dev(const adfwi_elastic_desc *d, const adfwi_layout &L) {
+  ElDev v;
+  v.S = d->nshots; v.Z = d->nz; v.X = d->nx; v.pitch = d->pitch; v.fsurf = d->free_surface ? 1 : 0;
+  v.nt = d->nt; v.nrec = d->nrec; v.spw = L.reserved; v.tz = L.tile_z; v.nsx = L.ntiles_x;
+  v.nstrips = L.ntiles;
+  v.dt = d->dt; v.dt_dx = d->dt_dx; v.dt_dz = d->dt_dz; v.dx = d->dx; v.dz = d->dz; v.c0 = d->c0; v.c1 = d->c1;
+  v.plane = (long long)d->nz * d->pitch;
+  v.field = (long long)d->nshots * v.plane;
+  v.coef = d->coef; v.pml_x = d->pml_x; v.pml_z = d->pml_z; v.src = d->src; v.src_m = d->src_m;
+  v.src_z = d->src_z; v.src_x = d->src_x; v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adfwi_elastic_get_layout(const adfwi_elastic_desc *d, adfwi_layout *out) {
+  ADFWI_CHECK_ARG(out != nullptr, "null layout pointer");
+  return el_layout(d, out);
+}
+
+int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a, adfwi_stream_t stream) {
+  if (int rc = adfwi::fault_injected("adfwi_elastic_forward")) return rc;
+  adfwi_layout L;
+  int rc = el_layout(d, &L);
+  if (rc) return rc;
+  ADFWI_CHECK_ARG(a != nullptr, "null forward args");
+  ADFWI_CHECK_ARG(d->coef && d->pml_x && d->pml_z && d->src && d->src_m && d->src_z && d->src_x,
+                  "null coefficient/source pointer");
+  const bool recs = a->rec[0] || a->rec[1] || a->rec[2] || a->rec[3] || a->rec[4];
+  ADFWI_CHECK_ARG(!recs || (a->rec[0] && a->rec[1] && a->rec[2] && a->rec[3] && a->rec[4]),
+                  "records must be all set or all NULL");
+  if (recs && d->nrec > 0) ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
+  ADFWI_CHECK_ARG(a->state_in && a->work[0] && a->work[1] && a->work[0] != a->work[1],
+                  "state_in and two distinct work buffers are required");
+  ADFWI_CHECK_ARG(a->t0 >= 0 && a->nsteps >= 0 && a->t0 + a->nsteps <= d->nt,
+                  "steps [%d, %d) outside [0, %d)", a->t0, a->t0 + a->nsteps, d->nt);
+  ADFWI_CHECK_ARG(!a->ckpt || a->ckpt_every >= 1, "ckpt_every must be >= 1 with a checkpoint buffer");
+  ElDev v = el_dev(d, L);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid(L.ntiles, L.ngroups), block(ENT);
+  const long long ng = (long long)d->nshots * d->nrec;
+  const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  const float *cur = a->state_in;
+  for (int k = 0; k < a->nsteps; ++k) {
+    const int t = a->t0 + k;
+    float *nxt;
+    if (a->ckpt && (t + 1) % a->ckpt_every == 0 && t + 1 < d->nt) {
+      nxt = a->ckpt + (long long)((t + 1) / a->ckpt_every - 1) * L.state_floats;
+    } else {
+      nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
+    }
+    float *img = a->image ? a->image + (long long)k * 5 * L.field_floats : nullptr;
+    if (d->aniso)
+      hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t, img);
+    else
+      hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t, img);
+    if ((rc = adfwi::check_launch("el_fwd_step"))) return rc;
+    if (recs && ng > 0) {
+      hipLaunchKernelGGL(el_gather, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
+                         a->rec[4], t);
+      if ((rc = adfwi::check_launch("el_gather"))) return rc;
+    }
+    cur = nxt;
+  }
+  a->state_out = cur;
+  return ADFWI_OK;
+}
+
+}  // extern "C"

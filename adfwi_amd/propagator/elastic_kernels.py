@@ -1,0 +1,279 @@
+"""Elastic propagation on the MI355X: coefficient set-up in torch, time stepping in HIP.
+
+Drop-in for ADFWI/propagator/elastic_kernels.py: ``pad_torchSingle`` (:175-216),
+``DiffCoef`` (:20-58) and ``forward_kernel`` (:917-1037, 4th-order PML branch) keep
+their signatures and results.  The per-step op sequence of step_forward_PML_4order
+(:222-423) runs as one fused HIP kernel per step; its autograd replay is replaced by
+``ElasticWave``, whose backward is the hand-written discrete adjoint with sqrt(nt)
+wavefield checkpoints and segment recomputation.
+"""
+import ctypes
+import math
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from .. import _native as nat
+from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval
+
+NAMES = ("txx", "tzz", "txz", "vx", "vz")
+NFIELD = 10
+
+
+def DiffCoef(order: int, grid_scheme: str) -> torch.Tensor:
+    """Staggered-grid FD coefficients solved in float32 (elastic_kernels.py:20-58)."""
+    if grid_scheme == "r":
+        b = torch.cat((torch.tensor([1 / 2]), torch.zeros(order - 1)))
+        expo = lambda i, j: (j + 1) ** (2 * (i + 1) - 1)
+    elif grid_scheme == "s":
+        b = torch.cat((torch.tensor([1.0]), torch.zeros(order - 1)))
+        expo = lambda i, j: (2 * (j + 1) - 1) ** (2 * (i + 1) - 1)
+    else:
+        raise ValueError("Invalid grid scheme. Use 'r' for rational or 's' for symmetric.")
+    a = torch.tensor([[float(expo(i, j)) for j in range(order)] for i in range(order)], dtype=torch.float32)
+    return torch.linalg.inv(a) @ b
+
+
+def pad_torchSingle(data: torch.Tensor, pml: int, fs_offset: int, free_surface: bool = True,
+                    device: torch.device = torch.device("cpu")) -> torch.Tensor:
+    """Edge-replicating pad of an array by its own shape, float32 like the reference
+    (elastic_kernels.py:175-216): fs_offset rows on top (plus pml without a free surface),
+    pml rows below and pml columns on both sides."""
+    nz, nx = data.shape
+    top = fs_offset if free_surface else fs_offset + pml
+    out = F.pad(data.to(torch.float32).reshape(1, 1, nz, nx), (pml, pml, top, pml), mode="replicate")
+    return out.reshape(nz + top + pml, nx + 2 * pml)
+
+
+def chunk_ends(nt: int, segments: int) -> List[int]:
+    """Last step of every torch.chunk(.., segments) time chunk (elastic_kernels.py:396)."""
+    size = -(-nt // max(1, int(segments)))
+    return [min(nt, s + size) - 1 for s in range(0, nt, size)]
+
+
+class ElasticGeometry:
+    """Device-side description of one elastic problem: padded grid, sources (with their
+    moment-tensor factors), receivers, tiling."""
+
+    def __init__(self, Z: int, X: int, nt: int, free_surface: bool, src_z, src_x, src_m: torch.Tensor,
+                 rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0):
+        nat.require_gpu(device)
+        self.device = torch.device(device)
+        self.Z, self.X, self.nt = int(Z), int(X), int(nt)
+        self.free_surface = bool(free_surface)
+        self.aniso = bool(aniso)
+        self.pitch = -(-self.X // PITCH_ALIGN) * PITCH_ALIGN
+        src_z = np.asarray(src_z, dtype=np.int64).reshape(-1)
+        src_x = np.asarray(src_x, dtype=np.int64).reshape(-1)
+        rcv_z = np.asarray(rcv_z, dtype=np.int64).reshape(-1)
+        rcv_x = np.asarray(rcv_x, dtype=np.int64).reshape(-1)
+        self.S, self.R = len(src_z), len(rcv_z)
+        if self.S < 1:
+            raise ValueError("at least one shot is required")
+        for name, zz, xx in (("source", src_z, src_x), ("receiver", rcv_z, rcv_x)):
+            if len(zz) and (zz.min() < 2 or zz.max() > self.Z - 3 or xx.min() < 2 or xx.max() > self.X - 3):
+                raise ValueError(f"{name} index outside the update window [2, {self.Z - 3}] x [2, {self.X - 3}] "
+                                 f"of the padded grid")
+        i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
+        self.src_z, self.src_x = i32(src_z), i32(src_x)
+        self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
+        self.src_m = src_m.detach().to(device=self.device, dtype=torch.float32).contiguous()
+        self.shots_per_wave = int(shots_per_wave or os.environ.get("ADFWI_EL_SHOTS_PER_WAVE", "0"))
+        self.tile_z = int(tile_z or os.environ.get("ADFWI_EL_TILE_Z", "0"))
+        self.layout = nat.Layout()
+        d = self.desc(None, None, None, None, (0.0,) * 7)
+        nat.check(nat.load().adfwi_elastic_get_layout(ctypes.byref(d), ctypes.byref(self.layout)),
+                  "adfwi_elastic_get_layout")
+
+    def desc(self, coef, pml_x, pml_z, src, scalars) -> nat.ElasticDesc:
+        d = nat.ElasticDesc()
+        d.abi_version, d.struct_size = nat.ABI_VERSION, ctypes.sizeof(nat.ElasticDesc)
+        d.nshots, d.nz, d.nx, d.pitch = self.S, self.Z, self.X, self.pitch
+        d.free_surface, d.nt, d.nrec, d.aniso = int(self.free_surface), self.nt, self.R, int(self.aniso)
+        d.shots_per_wave, d.tile_z = self.shots_per_wave, self.tile_z
+        d.dt, d.dt_dx, d.dt_dz, d.dx, d.dz, d.c0, d.c1 = scalars
+        ptr = lambda t: None if t is None else t.data_ptr()
+        d.coef, d.pml_x, d.pml_z, d.src = ptr(coef), ptr(pml_x), ptr(pml_z), ptr(src)
+        d.src_m = self.src_m.data_ptr()
+        d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
+        d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
+        return d
+
+    def state(self, n: int = 1) -> torch.Tensor:
+        return torch.zeros((n, NFIELD, self.S, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+
+    def pack_coef(self, fields) -> torch.Tensor:
+        """fields: C11, C13, C33, C55, bx, bz, C15, C35 padded arrays (each from its own shape)."""
+        coef = torch.zeros((8, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+        for k, f in enumerate(fields):
+            if f is not None:
+                coef[k, :f.shape[0], :f.shape[1]] = f
+        return coef
+
+
+def pml_factors(bcx_p: torch.Tensor, bcz_p: torch.Tensor, dt: float, pitch: int):
+    """1 -/+ dt/2 BC and the reciprocal, as elastic_kernels.py:313-322 forms them, per column
+    (x) and per row (z); the reference's profiles are constant along the other axis."""
+    bx = bcx_p[2, :]
+    bz = bcz_p[:, 2]
+    if not (torch.equal(bcx_p, bx.expand_as(bcx_p)) and torch.equal(bcz_p, bz[:, None].expand_as(bcz_p))):
+        raise ValueError("the PML profiles must vary along one axis only (as bc_pml_xz builds them)")
+    px = torch.zeros((3, pitch), device=bcx_p.device, dtype=torch.float32)
+    pxd = 1 + 0.5 * dt * bx
+    px[0, :bx.shape[0]] = 1 - 0.5 * dt * bx
+    px[1, :bx.shape[0]] = pxd
+    px[2, :bx.shape[0]] = 1.0 / pxd
+    pzd = 1 + 0.5 * dt * bz
+    pz = torch.stack([1 - 0.5 * dt * bz, pzd, 1.0 / pzd]).contiguous()
+    return px, pz
+
+
+def _energy(state: torch.Tensor, X: int, free_surface: bool):
+    """sum over shots of each total squared (elastic_kernels.py:414-418), free-surface tzz row 2 = 0."""
+    out = []
+    for k in range(5):
+        tot = state[2 * k] + state[2 * k + 1]
+        if k == 1 and free_surface:
+            tot = tot.clone()
+            tot[:, 2, :] = 0.0
+        out.append(tot[:, :, :X].square().sum(0))
+    return out
+
+
+def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, record=True, energy_ends=None,
+                ckpt_every: int = 0):
+    """All nt steps from a zero state.  Returns dict(rec=[5 x (S, nt, R)] or None,
+    energy=[5 x (Z, X)] summed over the states after the steps in energy_ends, ckpt)."""
+    lib = nat.load()
+    dev = geom.device
+    S, nt, R = geom.S, geom.nt, geom.R
+    work = geom.state(2)
+    ckpt = None
+    if ckpt_every:
+        ckpt = geom.state(max(-(-nt // ckpt_every) - 1, 1))
+    rec = [torch.zeros((S, nt, R), device=dev) for _ in range(5)] if record and R else None
+    d = geom.desc(coef, pml_x, pml_z, src, scalars)
+    ends = sorted(set(energy_ends or [])) or [nt - 1]
+    energy = None
+    cur = work[0]
+    t0 = 0
+    stream = nat.stream_handle(dev)
+    for te in ends:
+        a = nat.ElasticFwdArgs()
+        a.state_in = cur.data_ptr()
+        a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
+        a.ckpt = None if ckpt is None else ckpt.data_ptr()
+        a.ckpt_every = int(ckpt_every)
+        a.t0, a.nsteps = t0, te + 1 - t0
+        if rec is not None:
+            for i in range(5):
+                a.rec[i] = rec[i].data_ptr()
+        TIMELINE.call("forward", "el_fwd_step", a.nsteps, dev, lambda: nat.check(
+            lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream), "adfwi_elastic_forward"))
+        cur = work[0] if a.state_out == work[0].data_ptr() else work[1]
+        if energy_ends is not None:
+            e = _energy(cur, geom.X, geom.free_surface)
+            energy = e if energy is None else [x + y for x, y in zip(energy, e)]
+        t0 = te + 1
+    return {"rec": rec, "energy": energy, "ckpt": ckpt, "final": cur}
+
+
+class ElasticWave(torch.autograd.Function):
+    """Five records (txx, tzz, txz, vx, vz) and energy maps from padded coefficient fields.
+
+    forward(c11, c13, c33, c55, bx, bz, c15, c35, wavelet, pml_x, pml_z, geom, scalars,
+            energy_ends, ckpt_every); c15/c35 may be None (isotropic / VTI).
+    """
+
+    @staticmethod
+    def forward(ctx, c11, c13, c33, c55, bx, bz, c15, c35, wav, pml_x, pml_z, geom, scalars, energy_ends,
+                ckpt_every):
+        ctx.set_materialize_grads(False)
+        coef = geom.pack_coef([t.detach() if t is not None else None for t in (c11, c13, c33, c55, bx, bz, c15, c35)])
+        src = wav.detach().to(torch.float32).contiguous()
+        need_grad = any(ctx.needs_input_grad[:9])
+        k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
+        res = run_forward(geom, coef, pml_x, pml_z, src, scalars, record=True, energy_ends=energy_ends,
+                          ckpt_every=k if need_grad else 0)
+        rec = res["rec"] or [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(5)]
+        ctx.geom, ctx.k, ctx.scalars = geom, k, scalars
+        ctx.ckpt = res["ckpt"] if need_grad else None
+        ctx.shapes = [None if t is None else tuple(t.shape) for t in (c11, c13, c33, c55, bx, bz, c15, c35)]
+        ctx.save_for_backward(coef, src, pml_x, pml_z)
+        en = [e.contiguous() for e in res["energy"]]
+        ctx.mark_non_differentiable(*en)
+        return tuple(rec) + tuple(en)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        if all(g is None for g in grads[:5]):
+            return (None,) * 15
+        from .elastic_adjoint import run_backward
+        coef, src, pml_x, pml_z = ctx.saved_tensors
+        gcoef, gsrc = run_backward(ctx.geom, coef, src, pml_x, pml_z, ctx.scalars, ctx.ckpt, ctx.k, grads[:5],
+                                   ctx.needs_input_grad[:9])
+        ctx.ckpt = None
+        out = []
+        for i, shp in enumerate(ctx.shapes):
+            if shp is None or not ctx.needs_input_grad[i] or gcoef[i] is None:
+                out.append(None)
+            else:
+                out.append(gcoef[i][:shp[0], :shp[1]].contiguous())
+        return tuple(out) + (gsrc if ctx.needs_input_grad[8] else None,) + (None,) * 6
+
+
+def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
+                   nabc: int, free_surface: bool,
+                   src_x: torch.Tensor, src_z: torch.Tensor, src_n: int, src_v: torch.Tensor, MT: torch.Tensor,
+                   rcv_x: torch.Tensor, rcv_z: torch.Tensor, rcv_n: int,
+                   abc_type: str, bcx: torch.Tensor, bcz: torch.Tensor, damp,
+                   lamu, lam, bx: torch.Tensor, bz: torch.Tensor, CC: List[torch.Tensor],
+                   fd_order: int = 4, n_segments: int = 1,
+                   device: torch.device = torch.device("cpu"), dtype=torch.float32,
+                   geometry: Optional[ElasticGeometry] = None, ckpt_every: int = 0) -> Dict[str, torch.Tensor]:
+    """Elastic forward modelling with the reference's signature and result dictionary
+    (elastic_kernels.py:917-1037).  ``n_segments`` keeps its meaning for the energy maps
+    (summed over the last step of every time chunk); memory is governed by the sqrt(nt)
+    wavefield checkpoints of the HIP adjoint."""
+    nat.require_gpu(device)
+    if dtype != torch.float32:
+        raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
+    if fd_order != 4 or abc_type.lower() != "pml":
+        raise NotImplementedError("the HIP elastic propagator implements the 4th-order PML branch "
+                                  f"(got fd_order={fd_order}, abc_type={abc_type!r})")
+    fso = fd_order // 2
+    X = nx + 2 * nabc
+    Z = nz + (nabc + fso if free_surface else 2 * nabc + fso)
+    pad = lambda a: pad_torchSingle(a.to(device), nabc, fso, free_surface, device)
+    c11, c13, c15, c33, c35, c55 = (pad(CC[i]) for i in (0, 2, 4, 11, 13, 18))
+    bxp, bzp = pad(bx), pad(bz)
+    aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
+    bcx_p = pad_torchSingle(bcx.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
+    bcz_p = pad_torchSingle(bcz.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
+    if geometry is None:
+        zoff = fso if free_surface else nabc + fso
+        MTd = MT.to(device=device, dtype=torch.float32)
+        half = MTd / 2
+        src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+        geometry = ElasticGeometry(Z, X, nt, free_surface, (src_z + zoff).cpu().numpy(),
+                                   (src_x + nabc).cpu().numpy(), src_m, (rcv_z + zoff).cpu().numpy(),
+                                   (rcv_x + nabc).cpu().numpy(), device, aniso=aniso)
+    elif geometry.aniso != aniso:
+        geometry.aniso = aniso
+    pml_x, pml_z = pml_factors(bcx_p, bcz_p, dt, geometry.pitch)
+    fdc = DiffCoef(fso, "s")
+    f32 = lambda v: float(np.float32(v))
+    scalars = (f32(dt), f32(dt / dx), f32(dt / dz), f32(dx), f32(dz), float(fdc[0]), float(fdc[1]))
+    if src_v.dim() == 1:
+        src_v = src_v.unsqueeze(0).expand(src_n, -1)
+    ends = chunk_ends(nt, n_segments)
+    outs = ElasticWave.apply(c11, c13, c33, c55, bxp, bzp, c15 if aniso else None, c35 if aniso else None,
+                             src_v, pml_x, pml_z, geometry, scalars, ends, ckpt_every)
+    row0 = fso if free_surface else fso + nabc
+    res = {n: outs[i] for i, n in enumerate(NAMES)}
+    for i, n in enumerate(NAMES):
+        res["forward_wavefield_" + n] = outs[5 + i][row0:row0 + nz, nabc:nabc + nx]
+    return res

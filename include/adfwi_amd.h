@@ -128,6 +128,55 @@ typedef struct adfwi_acoustic_adj_args {
 int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a,
                            adfwi_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Elastic velocity-stress propagator, 4th order, split-field PML             */
+/* (elastic_kernels.py:222-423 step, :917-1037 driver)                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct adfwi_elastic_desc {
+  int32_t abi_version;     /* ADFWI_ABI_VERSION */
+  int32_t struct_size;     /* sizeof(adfwi_elastic_desc) */
+  int32_t nshots;          /* S */
+  int32_t nz, nx;          /* padded grid (two free-surface rows on top included) */
+  int32_t pitch;           /* row pitch in floats (>= nx) */
+  int32_t free_surface;    /* 0 / 1: free-surface rows 0..2 as in the reference */
+  int32_t nt, nrec;
+  int32_t aniso;           /* 1 when C15 or C35 is non-zero (general 2-D anisotropy) */
+  int32_t shots_per_wave;  /* 0 = library choice */
+  int32_t tile_z;          /* strip rows, 0 = library choice */
+  float dt, dt_dx, dt_dz, dx, dz;  /* fp32 scalars as the reference rounds them */
+  float c0, c1;            /* staggered FD coefficients DiffCoef(2, 's') */
+  int32_t reserved;
+  const float *coef;       /* [8][nz][pitch]: C11, C13, C33, C55, bx, bz, C15, C35 (padded as in
+                              the reference, each from its own shape; row/col 0 based) */
+  const float *pml_x;      /* [3][pitch]: 1 - dt/2 BCx, 1 + dt/2 BCx, 1/(1 + dt/2 BCx) per column */
+  const float *pml_z;      /* [3][nz]: the same for BCz per row */
+  const float *src;        /* [nshots][nt] wavelets */
+  const float *src_m;      /* [nshots][3]: -MT_xx/2, -MT_zz/2, -MT_xz/2 */
+  const int32_t *src_z, *src_x;   /* [nshots] padded source indices (inside the window) */
+  const int32_t *rcv_z, *rcv_x;   /* [nrec] padded receiver indices (inside the window) */
+} adfwi_elastic_desc;
+
+/* Tiling and buffer sizes; layout.nfields = 10 (split fields), layout.reserved = shots per
+ * wavefront, layout.shots_per_block = shots per workgroup (gradient-plane group size). */
+int adfwi_elastic_get_layout(const adfwi_elastic_desc *d, adfwi_layout *out);
+
+typedef struct adfwi_elastic_fwd_args {
+  const float *state_in;   /* [10][S][Z][pitch] split fields before step t0 */
+  float *work[2];
+  float *ckpt;             /* [n][10][S][Z][pitch] or NULL */
+  int32_t ckpt_every;
+  int32_t t0, nsteps;
+  int32_t reserved;
+  float *rec[5];           /* txx, tzz, txz, vx, vz records [S][nt][R], or all NULL */
+  float *image;            /* [nsteps][5][S][Z][pitch] or NULL: vx, vz totals before and txx, tzz,
+                              txz totals after each step's stress update (free-surface rows included) */
+  const float *state_out;  /* OUT */
+} adfwi_elastic_fwd_args;
+
+/* Forward time stepping (replaces step_forward_PML_4order, elastic_kernels.py:222-423). */
+int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a, adfwi_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,137 @@
+"""Parity of the HIP elastic path (libadfwi_amd.so through the C ABI) with the oracle and with
+the reference's own vectors.  Tolerances: forward records rel-L2 <= 1e-5 (bit-exact when the
+kernel is fed the oracle's padded coefficient fields), gradients rel-L2 <= 1e-4."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+from helpers import ELASTIC_NAMES, elastic_cc_general, elastic_inputs, elastic_random_case
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _oracle_setup(g, m, *, shots_per_wave=0, tile_z=0, fdc=None):
+    """Kernel inputs built on the CPU by the oracle (padded fields, PML factors, scalars).
+    fdc: FD coefficients to use (default: the oracle's float32 solve on this host)."""
+    from adfwi_amd.propagator.elastic_kernels import ElasticGeometry, pml_factors
+    from oracle import elastic as oe
+    nz, nx, nabc, fs = m["nz"], m["nx"], m["nabc"], m["fs"]
+    cc = elastic_cc_general(g, nz, nx)
+    f = oe.padded_fields(torch.tensor(g["bx"]), torch.tensor(g["bz"]), cc,
+                         torch.tensor(g["bcx"], dtype=torch.float32), torch.tensor(g["bcz"], dtype=torch.float32),
+                         nabc, fs)
+    Z = nz + (nabc + 2 if fs else 2 * nabc + 2)
+    X = nx + 2 * nabc
+    zoff = 2 if fs else nabc + 2
+    mt = torch.tensor(g["mt"])
+    half = mt / 2
+    src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+    aniso = "c15" in g
+    geom = ElasticGeometry(Z, X, m["nt"], fs, g["src_z"] + zoff, g["src_x"] + nabc, src_m,
+                           g["rcv_z"] + zoff, g["rcv_x"] + nabc, DEV, aniso=aniso,
+                           shots_per_wave=shots_per_wave, tile_z=tile_z)
+    fields = [f[k] for k in ("c11", "c13", "c33", "c55", "bx", "bz")]
+    fields += [f["c15"], f["c35"]] if aniso else [None, None]
+    coef = geom.pack_coef([None if t is None else t.to(DEV) for t in fields])
+    px, pz = pml_factors(f["bcx"], f["bcz"], m["dt"], geom.pitch)
+    fdc = oe.diff_coef(2) if fdc is None else fdc
+    f32 = lambda v: float(np.float32(v))
+    scalars = (f32(m["dt"]), f32(m["dt"] / m["dx"]), f32(m["dt"] / m["dx"]), f32(m["dx"]), f32(m["dx"]),
+               float(fdc[0]), float(fdc[1]))
+    src = torch.tensor(g["wavelet"]).to(DEV)
+    return geom, coef, px.to(DEV), pz.to(DEV), src, scalars
+
+
+def _run_oracle(g, m, seg=1, want_grad=False):
+    from oracle import elastic as oe
+    nz, nx = m["nz"], m["nx"]
+    leaves = {k: torch.tensor(g[k], requires_grad=want_grad) for k in ("c11", "c13", "c33", "c55", "bx", "bz")}
+    cc = elastic_cc_general(g, nz, nx)
+    cc[0], cc[2], cc[11], cc[18] = leaves["c11"], leaves["c13"], leaves["c33"], leaves["c55"]
+    w = torch.tensor(g["wavelet"], requires_grad=want_grad)
+    z = torch.zeros((nz, nx))
+    res = oe.forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
+                            torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
+                            torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
+                            len(g["rcv_x"]), "PML", torch.tensor(g["bcx"], dtype=torch.float32),
+                            torch.tensor(g["bcz"], dtype=torch.float32), None, z, z,
+                            leaves["bx"], leaves["bz"], cc, n_segments=seg)
+    out = {k: res[k].detach().numpy() for k in ELASTIC_NAMES}
+    out.update({"fw_" + k: res["forward_wavefield_" + k].numpy() for k in ELASTIC_NAMES})
+    if want_grad:
+        loss = sum((res[k] * torch.tensor(g["weights"][i])).sum() for i, k in enumerate(ELASTIC_NAMES))
+        loss.backward()
+        out.update({"grad_" + k: v.grad.numpy() for k, v in leaves.items()})
+        out["grad_wavelet"] = w.grad.numpy()
+    return out
+
+
+def _dropin(g, m, seg=1, want_grad=False, **leaf_kw):
+    """The drop-in forward_kernel on the GPU with the golden case's inputs."""
+    from adfwi_amd.propagator.elastic_kernels import forward_kernel
+    nz, nx = m["nz"], m["nx"]
+    leaves = {k: torch.tensor(g[k], device=DEV, requires_grad=want_grad)
+              for k in ("c11", "c13", "c33", "c55", "bx", "bz")}
+    cc = [t.to(DEV) for t in elastic_cc_general(g, nz, nx)]
+    cc[0], cc[2], cc[11], cc[18] = leaves["c11"], leaves["c13"], leaves["c33"], leaves["c55"]
+    w = torch.tensor(g["wavelet"], device=DEV, requires_grad=want_grad)
+    z = torch.zeros((nz, nx), device=DEV)
+    res = forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
+                         torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
+                         torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
+                         len(g["rcv_x"]), "PML", torch.tensor(g["bcx"], dtype=torch.float32),
+                         torch.tensor(g["bcz"], dtype=torch.float32), None, z, z, leaves["bx"], leaves["bz"],
+                         cc, fd_order=4, n_segments=seg, device=DEV, **leaf_kw)
+    return res, leaves, w
+
+
+@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+def test_kernel_bit_exact_given_oracle_coefficients(name):
+    """Same coefficient bits in -> same record bits out: the fused step keeps the reference's
+    fp32 association, stage order, split-field PML and free-surface rows.  The FD coefficients
+    are the reference's own (a float32 matrix solve whose last bit can depend on the host's
+    LAPACK), taken from the fixture."""
+    from adfwi_amd.propagator.elastic_kernels import run_forward
+    from conftest import golden
+    g, m = elastic_inputs(name)
+    fdc = torch.tensor(golden("diffcoef")["fdc"])
+    geom, coef, px, pz, src, scalars = _oracle_setup(g, m, fdc=fdc)
+    res = run_forward(geom, coef, px, pz, src, scalars, record=True)
+    torch.cuda.synchronize()
+    for i, k in enumerate(ELASTIC_NAMES):
+        got = res["rec"][i].cpu().numpy()
+        assert np.array_equal(got, g[f"seg1_rec_{k}"]), (k, rel_l2(got, g[f"seg1_rec_{k}"]))
+
+
+@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+@pytest.mark.parametrize("seg", [1, 3])
+def test_dropin_forward_matches_reference_vectors(name, seg):
+    g, m = elastic_inputs(name)
+    res, _, _ = _dropin(g, m, seg=seg)
+    torch.cuda.synchronize()
+    for k in ELASTIC_NAMES:
+        got = res[k].detach().cpu().numpy()
+        assert got.shape == g[f"seg{seg}_rec_{k}"].shape
+        assert rel_l2(got, g[f"seg{seg}_rec_{k}"]) <= 1e-5, k
+        fw = res["forward_wavefield_" + k].cpu().numpy()
+        assert fw.shape == g[f"seg{seg}_fw_{k}"].shape
+        assert rel_l2(fw, g[f"seg{seg}_fw_{k}"]) <= 1e-5, ("energy", k)
+
+
+@pytest.mark.parametrize("fs,aniso,spw,tz", [(True, False, 1, 4), (False, False, 2, 8), (True, True, 1, 16),
+                                              (False, True, 0, 0)])
+def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz):
+    """A grid of several 55-column strips and row tiles, several shots per wavefront: strip
+    edges, tile edges, PML corners and the anisotropic C15/C35 terms against the oracle."""
+    from adfwi_amd.propagator.elastic_kernels import run_forward
+    g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso)
+    ref = _run_oracle(g, m)
+    geom, coef, px, pz, src, scalars = _oracle_setup(g, m, shots_per_wave=spw, tile_z=tz)
+    assert geom.layout.ntiles_x >= 3 and geom.layout.ntiles_z >= 2 or tz == 0
+    res = run_forward(geom, coef, px, pz, src, scalars, record=True)
+    torch.cuda.synchronize()
+    for i, k in enumerate(ELASTIC_NAMES):
+        got = res["rec"][i].cpu().numpy()
+        assert np.array_equal(got, ref[k]), (k, rel_l2(got, ref[k]))
