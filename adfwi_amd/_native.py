@@ -67,7 +67,12 @@ class ElasticDesc(ctypes.Structure):
 
 class ElasticFwdArgs(ctypes.Structure):
     _fields_ = [("state_in", _vp), ("work", _vp * 2), ("ckpt", _vp), ("ckpt_every", _i32), ("t0", _i32),
-                ("nsteps", _i32), ("reserved", _i32), ("rec", _vp * 5), ("image", _vp), ("state_out", _vp)]
+                ("nsteps", _i32), ("reserved", _i32), ("rec", _vp * 5), ("trace", _vp), ("state_out", _vp)]
+
+
+class ElasticAdjArgs(ctypes.Structure):
+    _fields_ = [("lam_in", _vp), ("work", _vp * 2), ("t_hi", _i32), ("nsteps", _i32), ("fwd0", _vp),
+                ("trace", _vp), ("grad_rec", _vp * 5), ("grad_coef", _vp), ("grad_src", _vp), ("lam_out", _vp)]
 
 
 # function name -> (restype, argtypes)
@@ -82,6 +87,7 @@ _SIGNATURES = {
                                               ctypes.POINTER(AcousticAdjArgs), _vp]),
     "adfwi_elastic_get_layout": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(Layout)]),
     "adfwi_elastic_forward": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticFwdArgs), _vp]),
+    "adfwi_elastic_adjoint": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticAdjArgs), _vp]),
 }
 
 _lock = threading.Lock()

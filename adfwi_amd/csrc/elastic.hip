@@ -83,7 +83,7 @@ __device__ __forceinline__ bool strip_of(const ElDev &d, int &z0, int &x0) {
 // ---------------------------------------------------------------------------
 template <bool ANISO>
 __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restrict__ in, float *__restrict__ out,
-                                                   int t, float *__restrict__ img) {
+                                                   int t) {
   int z0, x0;
   if (!strip_of(d, z0, x0)) return;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
@@ -99,7 +99,6 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
   const rsrc_t rIn = rsrc(in, (unsigned)NFIELD * fieldb);
   const rsrc_t rOut = rsrc(out, (unsigned)NFIELD * fieldb);
   const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
-  const rsrc_t rImg = rsrc(img, img ? 5u * fieldb : 0u);
   const float c0 = d.c0, c1 = d.c1;
   // per-column PML factors (x); rows come from pml_z
   const float pxn = d.pml_x[min(max(c, 0), d.X - 1)];
@@ -161,16 +160,6 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
           vz0 = vz1m;
         }
         vz1 = vz1m;
-      }
-      if (img && kk >= 2 && kk < d.tz + 2 && out_col && k < end_row) {
-        bst(vx2, rImg, vs, 0u * fieldb + so + rb(k));
-        bst(vz2, rImg, vs, 1u * fieldb + so + rb(k));
-      }
-      if (img && d.fsurf && k == 2 && out_col) {
-        bst(vx0, rImg, vs, 0u * fieldb + so + rb(0));
-        bst(vz0, rImg, vs, 1u * fieldb + so + rb(0));
-        bst(vx1, rImg, vs, 0u * fieldb + so + rb(1));
-        bst(vz1, rImg, vs, 1u * fieldb + so + rb(1));
       }
       // 2. stress of row k
       const unsigned ok_ = rb(k);
@@ -242,13 +231,6 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         zz2 = -zz4;         // tzz[1] = -tzz[3]
         xz2 = -xz3;         // txz[1] = -txz[2]
         xz1 = -xz4;         // txz[0] = -txz[3]
-        if (img && out_col) {
-          bst(xz1, rImg, vs, 4u * fieldb + so + rb(0));
-          bst(xz2, rImg, vs, 4u * fieldb + so + rb(1));
-          bst(zz1, rImg, vs, 3u * fieldb + so + rb(0));
-          bst(zz2, rImg, vs, 3u * fieldb + so + rb(1));
-          bst(xx0, rImg, vs, 2u * fieldb + so + rb(1));
-        }
       }
       // 4. velocity of row r = k-2 (stress rows r-2..r+2 = *0..*4 for txz/tzz, txx row r = xx0)
       const int r = k - 2;
@@ -256,11 +238,6 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       const float tzm1 = shfl(xz2, lane - 1), tzm2 = shfl(xz2, lane - 2), tzp1 = shfl(xz2, lane + 1);
       const bool own_r = out_col && r >= z0 && r < end_row;
       if (own_r) {
-        if (img) {
-          bst(xx0, rImg, vs, 2u * fieldb + so + rb(r));
-          bst(zz2, rImg, vs, 3u * fieldb + so + rb(r));
-          bst(xz2, rImg, vs, 4u * fieldb + so + rb(r));
-        }
         const unsigned orr = rb(r);
         const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
         const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
@@ -297,6 +274,318 @@ __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restri
   r_txz[ro] = st[TXZ_X * f + o] + st[TXZ_Z * f + o];
   r_vx[ro] = st[VX_X * f + o] + st[VX_Z * f + o];
   r_vz[ro] = st[VZ_X * f + o] + st[VZ_Z * f + o];
+}
+
+// ---------------------------------------------------------------------------
+// adjoint step
+// ---------------------------------------------------------------------------
+// The transpose of one forward step, for the adjoint L of the split fields after step t
+// (record adjoints of step t already added by el_inject):
+//   a. velocity update^T: e = L(v') * dt b / (d pd) feeds the stress-total adjoints
+//      Lt_xx = Dxf^T e_vxx, Lt_xz = Dzb^T e_vxz + Dxb^T e_vzx, Lt_zz = Dzf^T e_vzz;
+//      gradient of bx / bz: L(v') * dt / (d pd) * (D T) with T the forward stress totals;
+//   b. stress free surface^T (tzz[2] = 0, tzz[1] = -tzz[3], txz[1] = -txz[2], txz[0] = -txz[3]):
+//      Lt_zz[2] = 0, Lt_zz[3] -= Lt_zz[1], Lt_xz[2] -= Lt_xz[1], Lt_xz[3] -= Lt_xz[0];
+//   c. L(s') = L(s after step) + Lt for both splits; the source gradient is sum_c m_c (L(s_cx') + L(s_cz'));
+//   d. stress update^T: L(s) = pn pi L(s'), a = dt/d * pi * L(s'); the velocity-total adjoints
+//      Lv_x = Dxb^T(C11 a_xxx + C13 a_zzx) + Dzf^T(C55 a_xzz) [+ Dzb^T(C15 a_xxz + C35 a_zzz) + Dxf^T(C15 a_xzx)],
+//      Lv_z = Dzb^T(C13 a_xxz + C33 a_zzz) + Dxf^T(C55 a_xzx) [+ Dxb^T(C15 a_xxx + C35 a_zzx) + Dzf^T(C35 a_xzz)],
+//      and the C gradients a * (D V) with V the forward velocity totals before the step;
+//   e. velocity free surface^T (vz[1] = vz[2], vx[1] = 2 vz[2,j+1] - 2 vz[2,j] + vx[3,j], vz[0] = vz[2]
+//      on the columns J = [2, X-3]): Lv_z[2,j] += Lv_z[1,j] + Lv_z[0,j] - 2 Lv_x[1,j] + 2 Lv_x[1,j-1],
+//      Lv_x[3,j] += Lv_x[1,j];
+//   f. L(v) = pn / pd L(v') + Lv for both splits.
+// Inside the window only; the splits outside it never change, so their adjoints are not needed.
+// Lanes and rows march exactly as in the forward step.  Each wavefront accumulates the
+// gradients of its shots in its own LDS slab (one slot per lane, so no synchronisation) and
+// adds the slab to its group's planes once per launch.
+constexpr int ETZA = 8;   // strip rows of the adjoint (bounded by the LDS slab)
+
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
+                                                   const float *__restrict__ fv, const float *__restrict__ fsn, int t,
+                                                   float *__restrict__ gacc, float *__restrict__ gsrc) {
+  constexpr int NGF = ANISO ? 8 : 6;
+  __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  int z0, x0;
+  if (!strip_of(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - EHL + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= EHL && lane < EHL + ESX && c <= d.X - 3;
+  const bool cwin = c >= 2 && c <= d.X - 3;
+  const int zmax = d.Z - 1, zw1 = d.Z - 3;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rLi = rsrc(lin, (unsigned)NFIELD * fieldb);
+  const rsrc_t rLo = rsrc(lout, (unsigned)NFIELD * fieldb);
+  const rsrc_t rFv = rsrc(fv, (unsigned)NFIELD * fieldb);
+  const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const float c0 = d.c0, c1 = d.c1;
+  const int cc = min(max(c, 0), d.X - 1);
+  const float pxn = d.pml_x[cc], pxd = d.pml_x[d.pitch + cc], pxi = d.pml_x[2 * d.pitch + cc];
+  const float sx_keep = pxn * pxi, sx_a = d.dt_dx * pxi;    // stress split: old-value factor, source factor
+  const float vx_keep = pxn / pxd, vx_w = d.dt / (d.dx * pxd);
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const bool jj = c >= 2 && c <= d.X - 3;
+  const bool jj1 = c + 1 >= 2 && c + 1 <= d.X - 3;
+  const bool jm1 = c - 1 >= 2 && c - 1 <= d.X - 3;
+  const int grp = blockIdx.y * EWPB + wv;
+  float *sl = slab + wv * (NGF * ETZA * WAVE) + lane;
+  for (int q = 0; q < NGF * ETZA; ++q) sl[q * WAVE] = 0.0f;
+  auto acc = [&](int f, int row, float v) { sl[(f * ETZA + (row - z0)) * WAVE] += v; };
+
+  bool any = false;
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = grp * d.spw + si;
+    if (s >= d.S) break;
+    any = true;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    // forward velocity totals (rows k-2..k+2) and stress totals (txz/tzz rows k-4..k, txx k-2..k)
+    float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;
+    float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
+    float xz0 = 0.f, xz1 = 0.f, xz2 = 0.f, xz3 = 0.f, xz4 = 0.f;
+    float zz0 = 0.f, zz1 = 0.f, zz2 = 0.f, zz3 = 0.f, zz4 = 0.f;
+    float xx0 = 0.f, xx1 = 0.f, xx2 = 0.f;
+    // e windows: e1 (vx_x), e3 (vz_x) rows k..k+2; e2 (vx_z), e4 (vz_z) rows k-2..k+2
+    float e1a = 0.f, e1b = 0.f, e1c = 0.f, e3a = 0.f, e3b = 0.f, e3c = 0.f;
+    float e20 = 0.f, e21 = 0.f, e22 = 0.f, e23 = 0.f, e24 = 0.f;
+    float e40 = 0.f, e41 = 0.f, e42 = 0.f, e43 = 0.f, e44 = 0.f;
+    // stress-product windows: P1, Q3 (, P3, Q1) rows k-2..k; P4, Q2 (, P2, Q4) rows k-4..k
+    float p1a = 0.f, p1b = 0.f, p1c = 0.f, q3a = 0.f, q3b = 0.f, q3c = 0.f;
+    float p40 = 0.f, p41 = 0.f, p42 = 0.f, p43 = 0.f, p44 = 0.f;
+    float q20 = 0.f, q21 = 0.f, q22 = 0.f, q23 = 0.f, q24 = 0.f;
+    float p3a = 0.f, p3b = 0.f, p3c = 0.f, q1a = 0.f, q1b = 0.f, q1c = 0.f;
+    float p20 = 0.f, p21 = 0.f, p22 = 0.f, p23 = 0.f, p24 = 0.f;
+    float q40 = 0.f, q41 = 0.f, q42 = 0.f, q43 = 0.f, q44 = 0.f;
+    // free-surface rows kept for the transposes
+    float ltzz1 = 0.f, ltxz0 = 0.f, ltxz1 = 0.f, lvz0 = 0.f, lvz1 = 0.f, lvx1 = 0.f;
+
+    // forward velocity totals and e of row rr (zero outside the grid / window)
+    auto load_row = [&](int rr, float &fvx, float &fvz, float &ea, float &eb, float &ec, float &ed) {
+      const unsigned o = rb(rr);
+      const bool in_grid = rr >= 0 && rr <= zmax;
+      const float a = bld(rFv, vc, F(VX_X) + o) + bld(rFv, vc, F(VX_Z) + o);
+      const float b = bld(rFv, vc, F(VZ_X) + o) + bld(rFv, vc, F(VZ_Z) + o);
+      fvx = in_grid ? a : 0.f;
+      fvz = in_grid ? b : 0.f;
+      ea = eb = ec = ed = 0.f;
+      if (rr >= 2 && rr <= zw1 && cwin) {
+        const float bxv = bld(rC, vc, BX * planeb + o), bzv = bld(rC, vc, BZ * planeb + o);
+        const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
+        ea = bld(rLi, vc, F(VX_X) + o) * (vx_w * bxv);
+        eb = bld(rLi, vc, F(VX_Z) + o) * (vz_w * bxv);
+        ec = bld(rLi, vc, F(VZ_X) + o) * (vx_w * bzv);
+        ed = bld(rLi, vc, F(VZ_Z) + o) * (vz_w * bzv);
+      }
+    };
+    for (int q = -2; q <= 1; ++q) {
+      float fvx, fvz, ea, eb, ec, ed;
+      load_row(z0 - 2 + q, fvx, fvz, ea, eb, ec, ed);
+      vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = fvx;
+      vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = fvz;
+      e1a = e1b; e1b = e1c; e1c = ea;
+      e21 = e22; e22 = e23; e23 = e24; e24 = eb;
+      e3a = e3b; e3b = e3c; e3c = ec;
+      e41 = e42; e42 = e43; e43 = e44; e44 = ed;
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < d.tz + 4; ++kk) {
+      const int k = z0 - 2 + kk;
+      {
+        float fvx, fvz, ea, eb, ec, ed;
+        load_row(k + 2, fvx, fvz, ea, eb, ec, ed);
+        vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = fvx;
+        vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = fvz;
+        e1a = e1b; e1b = e1c; e1c = ea;
+        e20 = e21; e21 = e22; e22 = e23; e23 = e24; e24 = eb;
+        e3a = e3b; e3b = e3c; e3c = ec;
+        e40 = e41; e41 = e42; e42 = e43; e43 = e44; e44 = ed;
+      }
+      if (d.fsurf && k == 2) {   // forward velocity free surface (rows 0, 1 from rows 2, 3)
+        const float vz2n = shfl(vz2, lane + 1);
+        const float vz1m_n = jj1 ? vz2n : 0.0f;
+        const float vz1m = jj ? vz2 : vz1;
+        if (jj) {
+          vx1 = (((vz1m_n - vz1m) + vz2n) - vz2) + vx3;
+          vz0 = vz1m;
+        }
+        vz1 = vz1m;
+      }
+      // a. stress-total adjoints of row k
+      const float e1m1 = shfl(e1a, lane - 1), e1m2 = shfl(e1a, lane - 2), e1p1 = shfl(e1a, lane + 1);
+      const float e3p1 = shfl(e3a, lane + 1), e3m1 = shfl(e3a, lane - 1), e3p2 = shfl(e3a, lane + 2);
+      float lt_xx = c0 * (e1m1 - e1a) + c1 * (e1m2 - e1p1);
+      float lt_xz = (c0 * (e22 - e23) + c1 * (e21 - e24)) + (c0 * (e3a - e3p1) + c1 * (e3m1 - e3p2));
+      float lt_zz = c0 * (e41 - e42) + c1 * (e40 - e43);
+      // b. stress free surface transposed
+      if (d.fsurf) {
+        if (k == 0) ltxz0 = lt_xz;
+        if (k == 1) { ltzz1 = lt_zz; ltxz1 = lt_xz; }
+        if (k == 2) { lt_zz = 0.0f; lt_xz = lt_xz - ltxz1; }
+        if (k == 3) { lt_zz = lt_zz - ltzz1; lt_xz = lt_xz - ltxz0; }
+      }
+      // forward velocity derivatives of row k (for the C gradients)
+      const float vxm1 = shfl(vx2, lane - 1), vxm2 = shfl(vx2, lane - 2), vxp1 = shfl(vx2, lane + 1);
+      const float vzm1 = shfl(vz2, lane - 1), vzp1 = shfl(vz2, lane + 1), vzp2 = shfl(vz2, lane + 2);
+      float vzm2 = 0.0f, vxp2 = 0.0f;
+      if constexpr (ANISO) {
+        vzm2 = shfl(vz2, lane - 2);
+        vxp2 = shfl(vx2, lane + 2);
+      }
+      // c, d. stress splits of row k
+      float P1 = 0.f, Q2 = 0.f, Q3 = 0.f, P4 = 0.f, P2 = 0.f, P3 = 0.f, Q1 = 0.f, Q4 = 0.f;
+      if (k >= 2 && k <= zw1 && cwin) {
+        const unsigned o = rb(k);
+        const float lxx_x = bld(rLi, vc, F(TXX_X) + o) + lt_xx, lxx_z = bld(rLi, vc, F(TXX_Z) + o) + lt_xx;
+        const float lzz_x = bld(rLi, vc, F(TZZ_X) + o) + lt_zz, lzz_z = bld(rLi, vc, F(TZZ_Z) + o) + lt_zz;
+        const float lxz_x = bld(rLi, vc, F(TXZ_X) + o) + lt_xz, lxz_z = bld(rLi, vc, F(TXZ_Z) + o) + lt_xz;
+        const bool own_k = out_col && k >= z0 && k < end_row;
+        if (own_k && k == zs && c == xs && gsrc)
+          gsrc[(long long)s * d.nt + t] = (mxx * (lxx_x + lxx_z) + mzz * (lzz_x + lzz_z)) + mxz * (lxz_x + lxz_z);
+        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float sz_keep = pzn * pzi, sz_a = d.dt_dz * pzi;
+        if (own_k) {
+          bst(sx_keep * lxx_x, rLo, vs, F(TXX_X) + o); bst(sz_keep * lxx_z, rLo, vs, F(TXX_Z) + o);
+          bst(sx_keep * lzz_x, rLo, vs, F(TZZ_X) + o); bst(sz_keep * lzz_z, rLo, vs, F(TZZ_Z) + o);
+          bst(sx_keep * lxz_x, rLo, vs, F(TXZ_X) + o); bst(sz_keep * lxz_z, rLo, vs, F(TXZ_Z) + o);
+        }
+        const float axx_x = sx_a * lxx_x, axx_z = sz_a * lxx_z;
+        const float azz_x = sx_a * lzz_x, azz_z = sz_a * lzz_z;
+        const float axz_x = sx_a * lxz_x, axz_z = sz_a * lxz_z;
+        const float C11v = bld(rC, vc, C11 * planeb + o), C13v = bld(rC, vc, C13 * planeb + o);
+        const float C33v = bld(rC, vc, C33 * planeb + o), C55v = bld(rC, vc, C55 * planeb + o);
+        P1 = C11v * axx_x + C13v * azz_x;
+        Q2 = C13v * axx_z + C33v * azz_z;
+        Q3 = C55v * axz_x;
+        P4 = C55v * axz_z;
+        const float dxb_vx = c0 * (vx2 - vxm1) + c1 * (vxp1 - vxm2);
+        const float dzb_vz = c0 * (vz2 - vz1) + c1 * (vz3 - vz0);
+        const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
+        const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
+        if constexpr (ANISO) {
+          const float C15v = bld(rC, vc, C15 * planeb + o), C35v = bld(rC, vc, C35 * planeb + o);
+          P2 = C15v * axx_z + C35v * azz_z;
+          P3 = C15v * axz_x;
+          Q1 = C15v * axx_x + C35v * azz_x;
+          Q4 = C35v * axz_z;
+          if (own_k) {
+            const float dxb_vz = c0 * (vz2 - vzm1) + c1 * (vzp1 - vzm2);
+            const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
+            const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
+            const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
+            acc(6, k, axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx);
+            acc(7, k, azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz);
+          }
+        }
+        if (own_k) {
+          acc(0, k, axx_x * dxb_vx);
+          acc(1, k, axx_z * dzb_vz + azz_x * dxb_vx);
+          acc(2, k, azz_z * dzb_vz);
+          acc(3, k, axz_x * dxf_vz + axz_z * dzf_vx);
+        }
+      }
+      p1a = p1b; p1b = p1c; p1c = P1;
+      q3a = q3b; q3b = q3c; q3c = Q3;
+      p40 = p41; p41 = p42; p42 = p43; p43 = p44; p44 = P4;
+      q20 = q21; q21 = q22; q22 = q23; q23 = q24; q24 = Q2;
+      if constexpr (ANISO) {
+        p3a = p3b; p3b = p3c; p3c = P3;
+        q1a = q1b; q1b = q1c; q1c = Q1;
+        p20 = p21; p21 = p22; p22 = p23; p23 = p24; p24 = P2;
+        q40 = q41; q41 = q42; q42 = q43; q43 = q44; q44 = Q4;
+      }
+      // forward stress totals of row k (after the step), free surface once row 3 exists
+      {
+        const unsigned o = rb(k);
+        const bool in_grid = k >= 0 && k <= zmax;
+        const float a = bld(rFs, vc, F(TXX_X) + o) + bld(rFs, vc, F(TXX_Z) + o);
+        const float b = bld(rFs, vc, F(TZZ_X) + o) + bld(rFs, vc, F(TZZ_Z) + o);
+        const float e = bld(rFs, vc, F(TXZ_X) + o) + bld(rFs, vc, F(TXZ_Z) + o);
+        xx0 = xx1; xx1 = xx2; xx2 = in_grid ? a : 0.f;
+        zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = in_grid ? b : 0.f;
+        xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = in_grid ? e : 0.f;
+      }
+      if (d.fsurf && k == 3) {
+        zz3 = 0.0f; zz2 = -zz4; xz2 = -xz3; xz1 = -xz4;
+      }
+      // e. velocity-total adjoints of row r = k-2
+      const int r = k - 2;
+      const float p1m1 = shfl(p1a, lane - 1), p1p1 = shfl(p1a, lane + 1), p1p2 = shfl(p1a, lane + 2);
+      const float q3m1 = shfl(q3a, lane - 1), q3m2 = shfl(q3a, lane - 2), q3p1 = shfl(q3a, lane + 1);
+      float lvx = (c0 * (p1a - p1p1) + c1 * (p1m1 - p1p2)) + (c0 * (p41 - p42) + c1 * (p40 - p43));
+      float lvz = (c0 * (q22 - q23) + c1 * (q21 - q24)) + (c0 * (q3m1 - q3a) + c1 * (q3m2 - q3p1));
+      if constexpr (ANISO) {
+        const float p3m1 = shfl(p3a, lane - 1), p3m2 = shfl(p3a, lane - 2), p3p1 = shfl(p3a, lane + 1);
+        const float q1m1 = shfl(q1a, lane - 1), q1p1 = shfl(q1a, lane + 1), q1p2 = shfl(q1a, lane + 2);
+        lvx = lvx + ((c0 * (p22 - p23) + c1 * (p21 - p24)) + (c0 * (p3m1 - p3a) + c1 * (p3m2 - p3p1)));
+        lvz = lvz + ((c0 * (q1a - q1p1) + c1 * (q1m1 - q1p2)) + (c0 * (q41 - q42) + c1 * (q40 - q43)));
+      }
+      if (d.fsurf) {
+        if (r == 0) lvz0 = lvz;
+        if (r == 1) { lvz1 = lvz; lvx1 = lvx; }
+        if (r == 2) {
+          const float lvx1m = shfl(lvx1, lane - 1);
+          if (jj) lvz = lvz + (((lvz1 + lvz0) - 2.0f * lvx1) + (jm1 ? 2.0f * lvx1m : 0.0f));
+        }
+        if (r == 3 && jj) lvx = lvx + lvx1;
+      }
+      // forward stress derivatives of row r (for the b gradients)
+      const float txp1 = shfl(xx0, lane + 1), txp2 = shfl(xx0, lane + 2), txm1 = shfl(xx0, lane - 1);
+      const float tzm1 = shfl(xz2, lane - 1), tzm2 = shfl(xz2, lane - 2), tzp1 = shfl(xz2, lane + 1);
+      const bool own_r = out_col && r >= z0 && r < end_row;
+      if (own_r) {
+        const unsigned o = rb(r);
+        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
+        const float l1 = bld(rLi, vc, F(VX_X) + o), l2 = bld(rLi, vc, F(VX_Z) + o);
+        const float l3 = bld(rLi, vc, F(VZ_X) + o), l4 = bld(rLi, vc, F(VZ_Z) + o);
+        bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);
+        bst(vx_keep * l3 + lvz, rLo, vs, F(VZ_X) + o); bst(vz_keep * l4 + lvz, rLo, vs, F(VZ_Z) + o);
+        const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
+        const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
+        const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
+        const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
+        acc(4, r, l1 * (vx_w * dxf_txx) + l2 * (vz_w * dzb_txz));
+        acc(5, r, l3 * (vx_w * dxb_txz) + l4 * (vz_w * dzf_tzz));
+      }
+    }
+  }
+  if (!any || !out_col) return;
+  const long long planef = (long long)d.Z * d.pitch;
+  float *g = gacc + (long long)grp * NCOEF * planef + c;
+  for (int row = z0; row < end_row; ++row)
+    for (int f = 0; f < NGF; ++f) {
+      float *p = g + (long long)f * planef + (long long)row * d.pitch;
+      *p = *p + sl[(f * ETZA + (row - z0)) * WAVE];
+    }
+}
+
+// adds the record adjoints of step t to both splits of each recorded field (free-surface tzz
+// on row 2 is identically zero there, so it receives nothing)
+__global__ void el_inject(ElDev d, float *__restrict__ lam, const float *__restrict__ g_txx,
+                          const float *__restrict__ g_tzz, const float *__restrict__ g_txz,
+                          const float *__restrict__ g_vx, const float *__restrict__ g_vz, int t) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)d.S * d.nrec) return;
+  const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
+  const int z = d.rcv_z[r];
+  const long long o = (long long)s * d.plane + (long long)z * d.pitch + d.rcv_x[r];
+  const long long f = d.field;
+  const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
+  const float *gs[5] = {g_txx, g_tzz, g_txz, g_vx, g_vz};
+  for (int q = 0; q < 5; ++q) {
+    if (!gs[q] || (q == 1 && d.fsurf && z == 2)) continue;
+    const float v = gs[q][ro];
+    atomicAdd(lam + (2 * q) * f + o, v);
+    atomicAdd(lam + (2 * q + 1) * f + o, v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -346,7 +635,7 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
                   "state of %d shots x %d x %d exceeds the 4 GiB of one buffer descriptor; "
                   "advance the shots in smaller launches", d->nshots, d->nz, d->pitch);
   L->state_floats = NFIELD * L->field_floats;
-  L->accum_floats = (int64_t)L->ngroups * d->nz * d->pitch;
+  L->accum_floats = (int64_t)L->ngroups * EWPB * NCOEF * d->nz * d->pitch;
   return ADFWI_OK;
 }
 
@@ -384,8 +673,9 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
   ADFWI_CHECK_ARG(!recs || (a->rec[0] && a->rec[1] && a->rec[2] && a->rec[3] && a->rec[4]),
                   "records must be all set or all NULL");
   if (recs && d->nrec > 0) ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
-  ADFWI_CHECK_ARG(a->state_in && a->work[0] && a->work[1] && a->work[0] != a->work[1],
-                  "state_in and two distinct work buffers are required");
+  ADFWI_CHECK_ARG(a->state_in && (a->trace || (a->work[0] && a->work[1] && a->work[0] != a->work[1])),
+                  "state_in and two distinct work buffers (or a trace buffer) are required");
+  ADFWI_CHECK_ARG(!(a->trace && a->ckpt), "trace and ckpt are exclusive");
   ADFWI_CHECK_ARG(a->t0 >= 0 && a->nsteps >= 0 && a->t0 + a->nsteps <= d->nt,
                   "steps [%d, %d) outside [0, %d)", a->t0, a->t0 + a->nsteps, d->nt);
   ADFWI_CHECK_ARG(!a->ckpt || a->ckpt_every >= 1, "ckpt_every must be >= 1 with a checkpoint buffer");
@@ -398,16 +688,17 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
   for (int k = 0; k < a->nsteps; ++k) {
     const int t = a->t0 + k;
     float *nxt;
-    if (a->ckpt && (t + 1) % a->ckpt_every == 0 && t + 1 < d->nt) {
+    if (a->trace) {
+      nxt = a->trace + (long long)k * L.state_floats;
+    } else if (a->ckpt && (t + 1) % a->ckpt_every == 0 && t + 1 < d->nt) {
       nxt = a->ckpt + (long long)((t + 1) / a->ckpt_every - 1) * L.state_floats;
     } else {
       nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
     }
-    float *img = a->image ? a->image + (long long)k * 5 * L.field_floats : nullptr;
     if (d->aniso)
-      hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t, img);
+      hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
     else
-      hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t, img);
+      hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
     if ((rc = adfwi::check_launch("el_fwd_step"))) return rc;
     if (recs && ng > 0) {
       hipLaunchKernelGGL(el_gather, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
@@ -417,6 +708,53 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
     cur = nxt;
   }
   a->state_out = cur;
+  return ADFWI_OK;
+}
+
+int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a, adfwi_stream_t stream) {
+  if (int rc = adfwi::fault_injected("adfwi_elastic_adjoint")) return rc;
+  adfwi_layout L;
+  int rc = el_layout(d, &L);
+  if (rc) return rc;
+  ADFWI_CHECK_ARG(a != nullptr, "null adjoint args");
+  ADFWI_CHECK_ARG(d->coef && d->pml_x && d->pml_z && d->src_m && d->src_z && d->src_x,
+                  "null coefficient/source pointer");
+  ADFWI_CHECK_ARG(a->work[0] && a->work[1] && a->work[0] != a->work[1], "two distinct work buffers are required");
+  ADFWI_CHECK_ARG(a->lam_in == a->work[0] || a->lam_in == a->work[1], "lam_in must be work[0] or work[1]");
+  ADFWI_CHECK_ARG(a->fwd0 && a->trace && a->grad_coef, "null forward state, trace or gradient buffer");
+  ADFWI_CHECK_ARG(a->nsteps >= 0 && a->t_hi - a->nsteps >= 0 && a->t_hi <= d->nt,
+                  "steps [%d, %d) outside [0, %d)", a->t_hi - a->nsteps, a->t_hi, d->nt);
+  const bool recs = a->grad_rec[0] || a->grad_rec[1] || a->grad_rec[2] || a->grad_rec[3] || a->grad_rec[4];
+  if (recs && d->nrec > 0) ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
+  ElDev v = el_dev(d, L);
+  // the adjoint marches strips of 4..8 rows (its gradient slab lives in LDS; the free-surface
+  // transposes need rows 0..3 inside the first strip)
+  v.tz = min(max(L.tile_z, 4), ETZA);
+  v.nstrips = ((d->nz - 4 + v.tz - 1) / v.tz) * v.nsx;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid(v.nstrips, L.ngroups), block(ENT);
+  const long long ng = (long long)d->nshots * d->nrec;
+  const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  const int t_lo = a->t_hi - a->nsteps;
+  float *cur = a->lam_in;
+  for (int i = a->nsteps - 1; i >= 0; --i) {
+    const int t = t_lo + i;
+    if (recs && ng > 0) {
+      hipLaunchKernelGGL(el_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec[0], a->grad_rec[1], a->grad_rec[2],
+                         a->grad_rec[3], a->grad_rec[4], t);
+      if ((rc = adfwi::check_launch("el_inject"))) return rc;
+    }
+    const float *fv = i == 0 ? a->fwd0 : a->trace + (long long)(i - 1) * L.state_floats;
+    const float *fsn = a->trace + (long long)i * L.state_floats;
+    float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
+    if (d->aniso)
+      hipLaunchKernelGGL(el_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+    else
+      hipLaunchKernelGGL(el_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+    if ((rc = adfwi::check_launch("el_adj_step"))) return rc;
+    cur = nxt;
+  }
+  a->lam_out = cur;
   return ADFWI_OK;
 }
 

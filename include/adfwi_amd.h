@@ -158,7 +158,9 @@ typedef struct adfwi_elastic_desc {
 } adfwi_elastic_desc;
 
 /* Tiling and buffer sizes; layout.nfields = 10 (split fields), layout.reserved = shots per
- * wavefront, layout.shots_per_block = shots per workgroup (gradient-plane group size). */
+ * wavefront, layout.shots_per_block = shots per workgroup, layout.ngroups = workgroups along
+ * the shots, layout.accum_floats = size of the gradient accumulator of the adjoint,
+ * [ngroups * 4][8][nz][pitch] (one group of planes per wavefront's shot set). */
 int adfwi_elastic_get_layout(const adfwi_elastic_desc *d, adfwi_layout *out);
 
 typedef struct adfwi_elastic_fwd_args {
@@ -169,13 +171,33 @@ typedef struct adfwi_elastic_fwd_args {
   int32_t t0, nsteps;
   int32_t reserved;
   float *rec[5];           /* txx, tzz, txz, vx, vz records [S][nt][R], or all NULL */
-  float *image;            /* [nsteps][5][S][Z][pitch] or NULL: vx, vz totals before and txx, tzz,
-                              txz totals after each step's stress update (free-surface rows included) */
+  float *trace;            /* [nsteps][10][S][Z][pitch] or NULL: the state after every step of the
+                              call is written to its slot (work[] unused; ckpt must be NULL) */
   const float *state_out;  /* OUT */
 } adfwi_elastic_fwd_args;
 
 /* Forward time stepping (replaces step_forward_PML_4order, elastic_kernels.py:222-423). */
 int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a, adfwi_stream_t stream);
+
+typedef struct adfwi_elastic_adj_args {
+  float *lam_in;           /* adjoint of the state after step t_hi-1: work[0] or work[1]; the record
+                              adjoints of each reversed step are added to it in place */
+  float *work[2];          /* [10][S][Z][pitch] each, distinct */
+  int32_t t_hi;            /* steps t_hi-1 down to t_hi-nsteps are reversed */
+  int32_t nsteps;
+  const float *fwd0;       /* forward state before step t_hi-nsteps [10][S][Z][pitch] */
+  const float *trace;      /* forward states after steps t_hi-nsteps .. t_hi-1 [nsteps][10][S][Z][pitch] */
+  const float *grad_rec[5];/* dL/d(record) [S][nt][R] for txx, tzz, txz, vx, vz; NULL = zero */
+  float *grad_coef;        /* accumulated: layout.accum_floats, planes C11, C13, C33, C55, bx, bz,
+                              C15, C35 of every wavefront group (sum over the groups) */
+  float *grad_src;         /* [S][nt] or NULL: dL/d(wavelet) written for the reversed steps */
+  const float *lam_out;    /* OUT: adjoint of the state before step t_hi-nsteps */
+} adfwi_elastic_adj_args;
+
+/* Reverse time stepping: the exact transpose of adfwi_elastic_forward's steps, with the
+ * zero-lag cross-correlation gradients of the padded coefficient fields.  Replaces the
+ * autograd replay of step_forward_PML_4order (elastic_kernels.py:975-1008 checkpoints). */
+int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a, adfwi_stream_t stream);
 
 #ifdef __cplusplus
 }
