@@ -1,0 +1,152 @@
+"""Continuous-batching serving loop on one GPU.
+
+Requests arrive over time (their `arrival`, seconds after the start).  The loop submits each
+arrived request to the Scheduler, which admits it as soon as a slot and its HBM footprint are
+free; every admitted request runs on a worker thread with its own HIP stream (default
+priority), so requests that are on the GPU together overlap on the device: the library's step
+kernels of several requests execute concurrently and the host-side launch loops (which release
+the GIL inside the C library) run in parallel.  A request is done when its stream has drained;
+its latency is completion time - arrival time.  Model fields, wavefields, checkpoints and
+records stay in HBM for the request's lifetime and are freed when it completes.
+"""
+import queue
+import threading
+import time
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from .requests import Request, execute, footprint_bytes
+from .scheduler import Scheduler
+
+
+@dataclass
+class Outcome:
+    rid: int
+    arrival: float
+    start: float
+    end: float
+    digest: dict
+    result: Optional[dict] = None
+    error: Optional[str] = None
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+@dataclass
+class StreamReport:
+    outcomes: List[Outcome]
+    rejected: list
+    wall_s: float
+    peak_admitted_bytes: int
+    max_concurrent: int
+
+    def summary(self) -> dict:
+        ok = [o for o in self.outcomes if o.error is None]
+        lat = np.array([o.latency for o in ok]) if ok else np.zeros(1)
+        return {"requests": len(self.outcomes), "failed": len(self.outcomes) - len(ok),
+                "rejected": len(self.rejected), "wall_s": self.wall_s,
+                "throughput_req_per_s": len(ok) / self.wall_s if self.wall_s else 0.0,
+                "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+                "latency_mean_s": float(lat.mean()), "max_concurrent": self.max_concurrent,
+                "peak_admitted_gib": self.peak_admitted_bytes / 2**30}
+
+
+class ServingLoop:
+    """Serves a list of requests on `device` with at most `max_slots` in flight and at most
+    `budget_bytes` of admitted footprint (default: 85 % of the free device memory)."""
+
+    def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
+                 footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4):
+        import torch
+        from .. import _native as nat
+        nat.require_gpu(device)
+        nat.load()
+        self.device = torch.device(device)
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            budget_bytes = int(0.85 * free)
+        self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass)
+        self.footprint = footprint
+        self.max_slots = max_slots
+        self._local = threading.local()
+
+    def _stream(self):
+        import torch
+        s = getattr(self._local, "stream", None)
+        if s is None:
+            s = torch.cuda.Stream(device=self.device)   # default priority
+            self._local.stream = s
+        return s
+
+    def _run_one(self, req: Request, t0: float) -> Outcome:
+        import torch
+        start = time.perf_counter() - t0
+        stream = self._stream()
+        try:
+            with torch.cuda.device(self.device), torch.cuda.stream(stream):
+                out, digest = execute(req, self.device)
+                stream.synchronize()
+            end = time.perf_counter() - t0
+            return Outcome(req.rid, req.arrival, start, end, digest, out if req.keep else None)
+        except Exception as exc:  # noqa: BLE001 -- reported per request
+            stream.synchronize()
+            return Outcome(req.rid, req.arrival, start, time.perf_counter() - t0, {}, None, repr(exc))
+
+    def run(self, requests: List[Request], realtime: bool = True) -> StreamReport:
+        """Serves the requests.  realtime=False submits every request at t = 0 (arrival times
+        are then taken as 0 for the latency)."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        if not realtime:
+            for r in pending:
+                r.arrival = 0.0
+        outcomes: List[Outcome] = []
+        futures = {}
+        max_conc = 0
+        t0 = time.perf_counter()
+        i = 0
+        with ThreadPoolExecutor(max_workers=self.max_slots) as pool:
+            while i < len(pending) or futures or self.scheduler.waiting:
+                now = time.perf_counter() - t0
+                while i < len(pending) and pending[i].arrival <= now:
+                    r = pending[i]
+                    if not self.scheduler.submit(r, self.footprint(r)):
+                        outcomes.append(Outcome(r.rid, r.arrival, now, now, {}, None,
+                                                self.scheduler.rejected[-1][1]))
+                    i += 1
+                for r in self.scheduler.admit():
+                    futures[pool.submit(self._run_one, r, t0)] = r
+                max_conc = max(max_conc, len(futures))
+                timeout = None
+                if i < len(pending):
+                    timeout = max(0.0, pending[i].arrival - (time.perf_counter() - t0))
+                if futures:
+                    done, _ = wait(list(futures), timeout=timeout, return_when=FIRST_COMPLETED)
+                    for f in done:
+                        r = futures.pop(f)
+                        self.scheduler.finish(r.rid)
+                        outcomes.append(f.result())
+                elif timeout is not None:
+                    time.sleep(timeout)
+                elif self.scheduler.waiting:
+                    raise RuntimeError("requests wait but nothing runs: footprint accounting is inconsistent")
+        wall = time.perf_counter() - t0
+        outcomes.sort(key=lambda o: o.rid)
+        return StreamReport(outcomes, list(self.scheduler.rejected), wall, self.scheduler.peak_bytes, max_conc)
+
+
+def run_alone(requests: List[Request], device="cuda:0") -> List[Outcome]:
+    """The plain path: each request by itself, one after the other, on the default stream."""
+    import torch
+    out = []
+    t0 = time.perf_counter()
+    for r in requests:
+        s = time.perf_counter() - t0
+        res, digest = execute(r, device)
+        torch.cuda.synchronize()
+        out.append(Outcome(r.rid, r.arrival, s, time.perf_counter() - t0, digest, res if r.keep else None))
+    return out

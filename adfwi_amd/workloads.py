@@ -57,6 +57,12 @@ class Workload:
         """Padded acoustic grid (Z, X)."""
         return self.nz + 2 * self.nabc, self.nx + 2 * self.nabc
 
+    @property
+    def padded_elastic(self):
+        """Padded 4th-order elastic grid (Z, X): two free-surface rows on top."""
+        top = self.nabc + 2 if self.free_surface else 2 * self.nabc + 2
+        return self.nz + top, self.nx + 2 * self.nabc
+
     def wavelets(self):
         return np.tile(ricker(self.nt, self.dt, self.f0)[None, :], (self.nshots, 1)).astype(np.float32)
 

@@ -37,6 +37,10 @@ BYTES_FWD = 24.0          # forward: read + write p, u, w
 BYTES_RECOMPUTE = 28.0    # recompute: forward + stencil image store
 BYTES_ADJ = 28.0          # adjoint: read + write adjoint p, u, w + stencil image read
 BYTES_GRAD = 80.0         # per shot-gradient: 24 + 28 + 28
+# elastic: ten split fields
+EL_BYTES_FWD = 80.0       # forward / recompute: read + write the ten splits
+EL_BYTES_ADJ = 120.0      # adjoint: read + write ten adjoint splits, read the forward state before and after
+EL_BYTES_GRAD = 280.0     # per shot-gradient: 80 + 80 + 120
 
 
 def profiled_traffic(kernel):
@@ -76,10 +80,47 @@ def build_problem(wl, dev):
     return prop, init, obs
 
 
+def build_elastic_problem(wl, dev):
+    """Elastic problem of an elastic workload: the true model gives the observed vx / vz, the
+    initial model (each field smoothed, sigma 10 cells) carries the gradients; rho is a leaf."""
+    from scipy.ndimage import gaussian_filter
+    from adfwi_amd.model import AnisotropicElasticModel, IsotropicElasticModel
+    from adfwi_amd.propagator import ElasticPropagator
+    from adfwi_amd.survey import Receiver, Source, Survey
+    src = Source(wl.nt, wl.dt, wl.f0)
+    src.add_sources(wl.src_x, wl.src_z, wl.wavelets()[0])
+    rcv = Receiver(wl.nt, wl.dt)
+    rcv.add_receivers(wl.rcv_x, wl.rcv_z, "vz")
+    survey = Survey(src, rcv)
+    smooth = lambda a: gaussian_filter(a, 10, mode="reflect").astype(np.float32)
+    m = wl.models
+    common = dict(free_surface=wl.free_surface, nabc=wl.nabc, auto_update_rho=False, device=dev)
+    if "eps" in m:
+        zero = np.zeros_like(m["vp"])
+        mk = lambda f, grad: AnisotropicElasticModel(
+            0, 0, wl.nx, wl.nz, wl.dx, wl.dx, vp=f(m["vp"]), vs=f(m["vs"]), rho=f(m["rho"]), eps=f(m["eps"]),
+            gamma=zero, delta=f(m["delta"]), vp_grad=grad, vs_grad=grad, rho_grad=grad, eps_grad=grad,
+            delta_grad=grad, **common)
+    else:
+        mk = lambda f, grad: IsotropicElasticModel(
+            0, 0, wl.nx, wl.nz, wl.dx, wl.dx, vp=f(m["vp"]), vs=f(m["vs"]), rho=f(m["rho"]),
+            vp_grad=grad, vs_grad=grad, rho_grad=grad, **common)
+    true, init = mk(lambda a: a, False), mk(smooth, True)
+    prop = ElasticPropagator(init, survey, device=dev)
+    with torch.no_grad():
+        res = prop.forward(model=true)
+        obs = {k: res[k].detach().clone() for k in ("vx", "vz")}
+    return prop, init, obs
+
+
 def grad_step(prop, model, obs, dt):
-    model.vp.grad = None
+    for p in model.parameters():
+        p.grad = None
     rec = prop.forward()
-    loss = l2_misfit(rec["p"], obs, dt)
+    if isinstance(obs, dict):
+        loss = sum(l2_misfit(rec[k], obs[k], dt) for k in obs)
+    else:
+        loss = l2_misfit(rec["p"], obs, dt)
     loss.backward()
     return loss
 
@@ -133,6 +174,58 @@ def cpu_baseline(seconds_budget=24.0):
                        f"100x200 grid, 20-cell PML, 10 shots, {nt_run} of 1000 steps, median of 3, "
                        f"scaled to nt=1000"),
             "gpt_updates_per_s": 2 * wl.nshots * wl.nt * N / t / 1e9,
+            "cpu": _cpu_name()}
+
+
+def elastic_cpu_baseline(wl, seconds_budget=24.0):
+    """The CPU oracle (eager torch fp32, the reference's op order, autograd) on a bounded sample of
+    the elastic workload: all shots cut to `nb` shots and a few time steps, scaled per shot-step."""
+    from adfwi_amd.propagator.boundary_condition import bc_pml_xz
+    from oracle import elastic as oe
+    from oracle import model as om
+    cores = min(16, os.cpu_count() or 1)
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    torch.set_num_threads(cores)
+    m = {k: torch.tensor(v) for k, v in wl.models.items()}
+    nz, nx = wl.nz, wl.nx
+    zero = torch.zeros_like(m["vp"])
+    cc0, bx0, bz0, _, _ = om.elastic_cc(m["vp"], m["vs"], m["rho"], m.get("eps", zero), m.get("delta", zero),
+                                        zero, nx, nz)
+    bcx, bcz = bc_pml_xz(nx, nz, wl.dx, wl.dx, wl.nabc, float(m["vp"].max()), wl.free_surface)
+    nb = 2
+
+    def one(nt_run):
+        CC = list(cc0)
+        for i in (0, 2, 11, 18):
+            CC[i] = cc0[i].clone().requires_grad_(True)
+        bx = bx0.clone().requires_grad_(True)
+        bz = bz0.clone().requires_grad_(True)
+        z = torch.zeros((nz, nx))
+        wav = torch.tensor(wl.wavelets()[:nb, :nt_run])
+        res = oe.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, wl.nabc, wl.free_surface,
+                                torch.tensor(wl.src_x[:nb]), torch.tensor(wl.src_z[:nb]), nb, wav,
+                                torch.tensor(np.tile(np.eye(3, dtype=np.float32), (nb, 1, 1))),
+                                torch.tensor(wl.rcv_x), torch.tensor(wl.rcv_z), len(wl.rcv_x), "PML",
+                                torch.tensor(bcx, dtype=torch.float32), torch.tensor(bcz, dtype=torch.float32),
+                                None, z, z, bx, bz, CC)
+        (res["vx"].square().sum() + res["vz"].square().sum()).backward()
+    t0 = time.perf_counter()
+    one(10)
+    per_step = (time.perf_counter() - t0) / 10
+    nt_run = int(max(10, min(wl.nt, seconds_budget / 3 / max(per_step, 1e-9))))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        one(nt_run)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times)) * (wl.nt / nt_run) * (wl.nshots / nb)
+    return {"value": wl.nshots / t, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward+backward, full "
+                       f"{nz}x{nx} grid, {nb} of {wl.nshots} shots, {nt_run} of {wl.nt} steps, median of 3, "
+                       f"scaled per shot and step"),
             "cpu": _cpu_name()}
 
 
@@ -209,8 +302,9 @@ def main():
     from adfwi_amd.propagator import acoustic_kernels as ak
     kw = {"nt": args.nt} if args.nt else {}
     wl = getattr(workloads, args.workload)(**kw)
+    elastic = wl.name.startswith("elastic")
     log(f"workload {wl.name}: grid {wl.nz}x{wl.nx}, nt {wl.nt}, {wl.nshots} shots")
-    prop, model, obs = build_problem(wl, dev)
+    prop, model, obs = (build_elastic_problem if elastic else build_problem)(wl, dev)
     log("observed data modelled")
     for i in range(args.warmup):
         grad_step(prop, model, obs, wl.dt)
@@ -227,7 +321,7 @@ def main():
     elapsed = time.perf_counter() - t0
     ak.TIMELINE.enabled = False
     tl = ak.TIMELINE.summary()
-    Z, X = wl.padded
+    Z, X = wl.padded_elastic if elastic else wl.padded
     N = Z * X
     S = wl.nshots
     value = S * args.steps / elapsed
@@ -235,12 +329,15 @@ def main():
     kern = {}
     for name, d in tl.items():
         if name == "ac_fwd_step":
-            fwd_ms, rec_ms = d["calls"].get("forward", 0.0), d["calls"].get("recompute", 0.0)
             launches_fwd = args.steps * wl.nt
             launches_rec = d["launches"] - launches_fwd
             byts = launches_fwd * BYTES_FWD * N * S + launches_rec * BYTES_RECOMPUTE * N * S
-        else:
+        elif name == "ac_adj_step":
             byts = d["launches"] * BYTES_ADJ * N * S
+        elif name == "el_fwd_step":
+            byts = d["launches"] * EL_BYTES_FWD * N * S
+        else:
+            byts = d["launches"] * EL_BYTES_ADJ * N * S
         avg_ms = d["ms"] / max(d["launches"], 1)
         kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "avg_launch_ms": avg_ms,
                       "achieved_gbs": byts / (d["ms"] / 1e3) / 1e9 if d["ms"] else 0.0,
@@ -264,7 +361,8 @@ def main():
                    "nt": wl.nt, "shots_per_launch": S, "receivers": len(wl.rcv_x), "dt": wl.dt, "dx": wl.dx,
                    "checkpoint_interval": ak.checkpoint_interval(wl.nt), "parallelism": "single GPU, shot batch"},
         "gpt_updates_per_s": 2 * S * N * wl.nt * args.steps / elapsed / 1e9,
-        "grad_bytes_roofline_frac": BYTES_GRAD * S * N * wl.nt * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        "grad_bytes_roofline_frac": (EL_BYTES_GRAD if elastic else BYTES_GRAD) * S * N * wl.nt * args.steps
+                                    / elapsed / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
@@ -273,11 +371,17 @@ def main():
         "loss": float(loss.detach()),
     }
     log(f"timed region {elapsed:.2f} s, {value:.2f} shot-gradients/s")
-    if not args.no_baselines:
+    if elastic:
+        out["config"]["free_surface"] = wl.free_surface
+        out["config"]["components"] = ["vx", "vz"]
+    if not args.no_baselines and not elastic:
         out["plain_autograd_gpu"] = plain_gpu_baseline(wl, dev)
         log("plain autograd GPU baseline done")
         out["speedup_vs_plain_autograd_gpu"] = value / out["plain_autograd_gpu"]["value"]
         out["cpu_baseline"] = cpu_baseline()
+        log("CPU baseline done")
+    elif not args.no_baselines:
+        out["cpu_baseline"] = elastic_cpu_baseline(wl)
         log("CPU baseline done")
     else:
         out["cpu_baseline"] = None

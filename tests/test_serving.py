@@ -1,0 +1,100 @@
+"""Host-side tests of the serving loop: the request stream, the footprint model and the
+admission policy (no device needed)."""
+import numpy as np
+import pytest
+
+from adfwi_amd.serving import Request, Scheduler, footprint_bytes, make_stream
+
+
+def _req(rid, nbytes_tag=0, arrival=0.0):
+    r = Request(rid, "acoustic", "forward", 128, 256, 100, 1, rid, arrival)
+    r.extra["tag"] = nbytes_tag
+    return r
+
+
+def test_stream_is_seeded_and_follows_the_mix():
+    a = make_stream(2000, 2.0, seed=4)
+    b = make_stream(2000, 2.0, seed=4)
+    assert [(r.kind, r.mode, r.nz, r.nx, r.nt, r.nshots, r.seed) for r in a] == \
+           [(r.kind, r.mode, r.nz, r.nx, r.nt, r.nshots, r.seed) for r in b]
+    kinds = np.mean([r.kind == "acoustic" for r in a])
+    grads = np.mean([r.mode == "gradient" for r in a])
+    assert abs(kinds - 0.6) < 0.04 and abs(grads - 0.5) < 0.04
+    assert all(r.nz % 32 == 0 and 128 <= r.nz <= 512 for r in a)
+    assert all(r.nx % 32 == 0 and 256 <= r.nx <= 2048 for r in a)
+    assert all(1000 <= r.nt <= 6000 and 1 <= r.nshots <= 64 for r in a)
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 0.5) < 0.05          # Poisson process at 2 requests/s
+
+
+def test_footprint_grows_with_every_dimension():
+    base = Request(0, "acoustic", "forward", 128, 256, 1000, 4, 0)
+    f0 = footprint_bytes(base)
+    for kw in ({"nz": 256}, {"nx": 512}, {"nt": 4000}, {"nshots": 8}, {"mode": "gradient"}, {"kind": "elastic"}):
+        r = Request(**{**base.__dict__, **kw})
+        assert footprint_bytes(r) > f0, kw
+    # gradient: checkpoints grow with sqrt(nt), records with nt
+    g1 = Request(0, "elastic", "gradient", 256, 512, 1000, 8, 0)
+    g4 = Request(0, "elastic", "gradient", 256, 512, 4000, 8, 0)
+    assert footprint_bytes(g4) < 4 * footprint_bytes(g1)
+
+
+def test_admission_respects_budget_and_slots():
+    s = Scheduler(budget_bytes=100, max_slots=2)
+    sizes = {0: 60, 1: 30, 2: 30, 3: 10}
+    for rid, n in sizes.items():
+        s.submit(_req(rid), n)
+    got = [r.rid for r in s.admit()]
+    assert got == [0, 1]                     # 2 and 3 fit the budget but no slot is left
+    assert s.used == 90
+    s.finish(0)
+    got = [r.rid for r in s.admit()]
+    assert got == [2]                        # 2 fits in 70; one slot free
+    s.finish(1)
+    assert [r.rid for r in s.admit()] == [3]
+    s.finish(2)
+    s.finish(3)
+    assert s.idle()
+
+
+def test_large_request_is_not_starved():
+    s = Scheduler(budget_bytes=100, max_slots=8, max_bypass=2)
+    s.submit(_req(0), 50)
+    assert [r.rid for r in s.admit()] == [0]
+    s.submit(_req(1), 85)                    # does not fit while 0 runs
+    for rid in (2, 3, 4, 5):
+        s.submit(_req(rid), 10)
+    got = [r.rid for r in s.admit()]
+    assert got == [2, 3]                     # 1 is passed twice, then blocks the queue
+    s.finish(0)
+    assert [r.rid for r in s.admit()] == []  # 85 does not fit beside 2 and 3 (20 used); 4, 5 wait behind it
+    s.finish(2)
+    s.finish(3)
+    assert [r.rid for r in s.admit()][0] == 1
+
+
+def test_oversized_request_is_rejected():
+    s = Scheduler(budget_bytes=100, max_slots=2)
+    assert not s.submit(_req(0), 101)
+    assert s.rejected and "exceeds" in s.rejected[0][1]
+    assert s.idle()
+
+
+def test_full_stream_admission_never_exceeds_budget():
+    reqs = make_stream(300, 5.0, seed=1)
+    budget = 200 * 2**30
+    s = Scheduler(budget, max_slots=6)
+    rng = np.random.default_rng(0)
+    for r in reqs:
+        s.submit(r, footprint_bytes(r))
+    done = 0
+    while not s.idle():
+        for r in s.admit():
+            pass
+        assert s.used <= budget and len(s.running) <= 6
+        # finish a random running request
+        if s.running:
+            rid = list(s.running)[rng.integers(len(s.running))]
+            s.finish(rid)
+            done += 1
+    assert done + len(s.rejected) == len(reqs)

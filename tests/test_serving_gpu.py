@@ -1,0 +1,51 @@
+"""The serving loop on the GPU: every result is the same alone and in the mix, requests
+overlap on the device, and the admitted footprint bounds what a request allocates."""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _small_stream(n, seed, rate=50.0):
+    from adfwi_amd.serving import make_stream
+    reqs = make_stream(n, rate, seed, nz_range=(128, 160), nx_range=(256, 352), nt_range=(60, 140),
+                       shots_range=(1, 4))
+    for r in reqs:
+        r.keep = True
+    return reqs
+
+
+def test_results_alone_equal_results_in_the_mix():
+    from adfwi_amd.serving import ServingLoop, run_alone
+    reqs = _small_stream(8, seed=7)
+    assert {r.kind for r in reqs} == {"acoustic", "elastic"} and {r.mode for r in reqs} == {"forward", "gradient"}
+    rep = ServingLoop(DEV, max_slots=3).run(copy.deepcopy(reqs), realtime=True)
+    assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+    assert rep.max_concurrent >= 2
+    alone = run_alone(reqs, DEV)
+    for a, b in zip(rep.outcomes, alone):
+        assert a.rid == b.rid and a.result.keys() == b.result.keys()
+        for k in b.result:
+            x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+            assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
+    s = rep.summary()
+    assert s["failed"] == 0 and s["latency_p99_s"] >= s["latency_p50_s"] > 0
+
+
+def test_footprint_bounds_device_allocation():
+    from adfwi_amd.serving import execute, footprint_bytes
+    for r in _small_stream(6, seed=11):
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        out, _ = execute(r, DEV)
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - base
+        del out
+        assert peak <= footprint_bytes(r), (r.kind, r.mode, peak, footprint_bytes(r))
