@@ -124,28 +124,48 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
     float zz0 = 0.f, zz1 = 0.f, zz2 = 0.f, zz3 = 0.f, zz4 = 0.f;
     float xx0 = 0.f, xx1 = 0.f, xx2 = 0.f;
     // prologue: velocity rows z0-4 .. z0-1 (k = z0-2 needs rows z0-4 .. z0)
-    {
-      const int k = z0 - 2;
-      for (int q = -2; q <= 1; ++q) {
-        const int rr = k + q;
-        const unsigned o = rb(rr);
-        const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
-        const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
-        const bool ok = rr >= 0;
-        vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
-        vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
-      }
+    for (int rr = z0 - 4; rr <= z0 - 1; ++rr) {
+      const unsigned o = rb(rr);
+      const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
+      const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
+      const bool ok = rr >= 0;
+      vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
+      vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
     }
+    // everything iteration k reads from memory, loaded one iteration ahead:
+    // velocity splits of row k+2, stress splits and moduli of row k, old velocity splits and
+    // buoyancies of row k-2
+    struct Row {
+      float v[4], s[6], cm[ANISO ? 6 : 4], u[4], b[2];
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o2 = rb(k + 2), o0 = rb(k), om = rb(k - 2);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.v[q] = bld(rIn, vc, F(VX_X + q) + o2);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.s[q] = bld(rIn, vc, F(TXX_X + q) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+      if (k - 2 >= z0 && k - 2 < end_row) {   // only rows this strip updates
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R.u[q] = bld(rIn, vc, F(VX_X + q) + om);
+        R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
+      }
+    };
+    Row cur;
+    load(z0 - 2, cur);
 #pragma unroll 1
     for (int kk = 0; kk < d.tz + 4; ++kk) {
       const int k = z0 - 2 + kk;
+      Row nxt;
+      load(k + 1, nxt);      // next row's loads go out before this row's stores
       // 1. velocity totals of row k+2 into the window (zero outside the grid)
       {
-        const int rr = k + 2;
-        const unsigned o = rb(rr);
-        const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
-        const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
-        const bool ok = rr <= zmax;
+        const bool ok = k + 2 <= zmax;
+        const float a = cur.v[0] + cur.v[1], b = cur.v[2] + cur.v[3];
         vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
         vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
       }
@@ -163,9 +183,8 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       }
       // 2. stress of row k
       const unsigned ok_ = rb(k);
-      float sxx_x = bld(rIn, vc, F(TXX_X) + ok_), sxx_z = bld(rIn, vc, F(TXX_Z) + ok_);
-      float szz_x = bld(rIn, vc, F(TZZ_X) + ok_), szz_z = bld(rIn, vc, F(TZZ_Z) + ok_);
-      float sxz_x = bld(rIn, vc, F(TXZ_X) + ok_), sxz_z = bld(rIn, vc, F(TXZ_Z) + ok_);
+      float sxx_x = cur.s[0], sxx_z = cur.s[1], szz_x = cur.s[2], szz_z = cur.s[3];
+      float sxz_x = cur.s[4], sxz_z = cur.s[5];
       // x derivatives of the velocity totals of row k (all lanes take part in the shuffles)
       const float vxm1 = shfl(vx2, lane - 1), vxm2 = shfl(vx2, lane - 2), vxp1 = shfl(vx2, lane + 1);
       const float vzm1 = shfl(vz2, lane - 1), vzp1 = shfl(vz2, lane + 1), vzp2 = shfl(vz2, lane + 2);
@@ -179,9 +198,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float dzb_vz = c0 * (vz2 - vz1) + c1 * (vz3 - vz0);
         const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
         const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
-        const unsigned cr = ok_;
-        const float C11v = bld(rC, vc, C11 * planeb + cr), C13v = bld(rC, vc, C13 * planeb + cr);
-        const float C33v = bld(rC, vc, C33 * planeb + cr), C55v = bld(rC, vc, C55 * planeb + cr);
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
         const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
         float sx_xx, sz_xx, sx_zz, sz_zz, sx_xz, sz_xz;
         if constexpr (ANISO) {
@@ -189,7 +206,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
           const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
           const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
           const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
-          const float C15v = bld(rC, vc, C15 * planeb + cr), C35v = bld(rC, vc, C35 * planeb + cr);
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
           sx_xx = C11v * dxb_vx + C15v * dxb_vz;
           sz_xx = C15v * dzb_vx + C13v * dzb_vz;
           sx_zz = C13v * dxb_vx + C35v * dxb_vz;
@@ -243,17 +260,16 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
         const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
         const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
-        const float bxv = bld(rC, vc, BX * planeb + orr), bzv = bld(rC, vc, BZ * planeb + orr);
+        const float bxv = cur.b[0], bzv = cur.b[1];
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
-        const float ux_x = bld(rIn, vc, F(VX_X) + orr), ux_z = bld(rIn, vc, F(VX_Z) + orr);
-        const float uz_x = bld(rIn, vc, F(VZ_X) + orr), uz_z = bld(rIn, vc, F(VZ_Z) + orr);
-        const float nx_x = (pxn * ux_x + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
-        const float nx_z = (pzn * ux_z + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
-        const float nz_x = (pxn * uz_x + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
-        const float nz_z = (pzn * uz_z + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        const float nx_x = (pxn * cur.u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * cur.u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        const float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        const float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
         bst(nx_x, rOut, vs, F(VX_X) + orr); bst(nx_z, rOut, vs, F(VX_Z) + orr);
         bst(nz_x, rOut, vs, F(VZ_X) + orr); bst(nz_z, rOut, vs, F(VZ_Z) + orr);
       }
+      cur = nxt;
     }
   }
 }
@@ -395,12 +411,53 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       e3a = e3b; e3b = e3c; e3c = ec;
       e41 = e42; e42 = e43; e43 = e44; e44 = ed;
     }
+    // everything iteration k reads from memory, loaded one iteration ahead: row k+2 of the
+    // forward velocity splits, the velocity adjoints and the buoyancies; row k of the stress
+    // adjoints, the moduli and the forward stress splits; row k-2 of the velocity adjoints
+    struct Row {
+      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4], fs[6], lr[4];
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o2 = rb(k + 2), o0 = rb(k), om = rb(k - 2);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.fv[q] = bld(rFv, vc, F(VX_X + q) + o2);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.lv[q] = bld(rLi, vc, F(VX_X + q) + o2);
+      R.b[0] = bld(rC, vc, BX * planeb + o2); R.b[1] = bld(rC, vc, BZ * planeb + o2);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.ls[q] = bld(rLi, vc, F(TXX_X + q) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.fs[q] = bld(rFs, vc, F(TXX_X + q) + o0);
+      if (k - 2 >= z0 && k - 2 < end_row) {   // only rows this strip updates
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R.lr[q] = bld(rLi, vc, F(VX_X + q) + om);
+      }
+    };
+    Row cur;
+    load(z0 - 2, cur);
 #pragma unroll 1
     for (int kk = 0; kk < d.tz + 4; ++kk) {
       const int k = z0 - 2 + kk;
+      Row nxt;
+      load(k + 1, nxt);      // next row's loads go out before this row's stores
       {
-        float fvx, fvz, ea, eb, ec, ed;
-        load_row(k + 2, fvx, fvz, ea, eb, ec, ed);
+        const int rr = k + 2;
+        const bool in_grid = rr <= zmax;
+        const float fvx = in_grid ? cur.fv[0] + cur.fv[1] : 0.f;
+        const float fvz = in_grid ? cur.fv[2] + cur.fv[3] : 0.f;
+        float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
+        if (rr >= 2 && rr <= zw1 && cwin) {
+          const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
+          ea = cur.lv[0] * (vx_w * cur.b[0]);
+          eb = cur.lv[1] * (vz_w * cur.b[0]);
+          ec = cur.lv[2] * (vx_w * cur.b[1]);
+          ed = cur.lv[3] * (vz_w * cur.b[1]);
+        }
         vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = fvx;
         vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = fvz;
         e1a = e1b; e1b = e1c; e1c = ea;
@@ -443,9 +500,9 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       float P1 = 0.f, Q2 = 0.f, Q3 = 0.f, P4 = 0.f, P2 = 0.f, P3 = 0.f, Q1 = 0.f, Q4 = 0.f;
       if (k >= 2 && k <= zw1 && cwin) {
         const unsigned o = rb(k);
-        const float lxx_x = bld(rLi, vc, F(TXX_X) + o) + lt_xx, lxx_z = bld(rLi, vc, F(TXX_Z) + o) + lt_xx;
-        const float lzz_x = bld(rLi, vc, F(TZZ_X) + o) + lt_zz, lzz_z = bld(rLi, vc, F(TZZ_Z) + o) + lt_zz;
-        const float lxz_x = bld(rLi, vc, F(TXZ_X) + o) + lt_xz, lxz_z = bld(rLi, vc, F(TXZ_Z) + o) + lt_xz;
+        const float lxx_x = cur.ls[0] + lt_xx, lxx_z = cur.ls[1] + lt_xx;
+        const float lzz_x = cur.ls[2] + lt_zz, lzz_z = cur.ls[3] + lt_zz;
+        const float lxz_x = cur.ls[4] + lt_xz, lxz_z = cur.ls[5] + lt_xz;
         const bool own_k = out_col && k >= z0 && k < end_row;
         if (own_k && k == zs && c == xs && gsrc)
           gsrc[(long long)s * d.nt + t] = (mxx * (lxx_x + lxx_z) + mzz * (lzz_x + lzz_z)) + mxz * (lxz_x + lxz_z);
@@ -459,8 +516,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float axx_x = sx_a * lxx_x, axx_z = sz_a * lxx_z;
         const float azz_x = sx_a * lzz_x, azz_z = sz_a * lzz_z;
         const float axz_x = sx_a * lxz_x, axz_z = sz_a * lxz_z;
-        const float C11v = bld(rC, vc, C11 * planeb + o), C13v = bld(rC, vc, C13 * planeb + o);
-        const float C33v = bld(rC, vc, C33 * planeb + o), C55v = bld(rC, vc, C55 * planeb + o);
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
         P1 = C11v * axx_x + C13v * azz_x;
         Q2 = C13v * axx_z + C33v * azz_z;
         Q3 = C55v * axz_x;
@@ -470,7 +526,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
         const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
         if constexpr (ANISO) {
-          const float C15v = bld(rC, vc, C15 * planeb + o), C35v = bld(rC, vc, C35 * planeb + o);
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
           P2 = C15v * axx_z + C35v * azz_z;
           P3 = C15v * axz_x;
           Q1 = C15v * axx_x + C35v * azz_x;
@@ -503,11 +559,10 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       }
       // forward stress totals of row k (after the step), free surface once row 3 exists
       {
-        const unsigned o = rb(k);
         const bool in_grid = k >= 0 && k <= zmax;
-        const float a = bld(rFs, vc, F(TXX_X) + o) + bld(rFs, vc, F(TXX_Z) + o);
-        const float b = bld(rFs, vc, F(TZZ_X) + o) + bld(rFs, vc, F(TZZ_Z) + o);
-        const float e = bld(rFs, vc, F(TXZ_X) + o) + bld(rFs, vc, F(TXZ_Z) + o);
+        const float a = cur.fs[0] + cur.fs[1];
+        const float b = cur.fs[2] + cur.fs[3];
+        const float e = cur.fs[4] + cur.fs[5];
         xx0 = xx1; xx1 = xx2; xx2 = in_grid ? a : 0.f;
         zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = in_grid ? b : 0.f;
         xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = in_grid ? e : 0.f;
@@ -544,8 +599,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const unsigned o = rb(r);
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
-        const float l1 = bld(rLi, vc, F(VX_X) + o), l2 = bld(rLi, vc, F(VX_Z) + o);
-        const float l3 = bld(rLi, vc, F(VZ_X) + o), l4 = bld(rLi, vc, F(VZ_Z) + o);
+        const float l1 = cur.lr[0], l2 = cur.lr[1], l3 = cur.lr[2], l4 = cur.lr[3];
         bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);
         bst(vx_keep * l3 + lvz, rLo, vs, F(VZ_X) + o); bst(vz_keep * l4 + lvz, rLo, vs, F(VZ_Z) + o);
         const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
@@ -555,6 +609,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         acc(4, r, l1 * (vx_w * dxf_txx) + l2 * (vz_w * dzb_txz));
         acc(5, r, l3 * (vx_w * dxb_txz) + l4 * (vz_w * dzf_tzz));
       }
+      cur = nxt;
     }
   }
   if (!any || !out_col) return;

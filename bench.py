@@ -43,13 +43,15 @@ EL_BYTES_ADJ = 120.0      # adjoint: read + write ten adjoint splits, read the f
 EL_BYTES_GRAD = 280.0     # per shot-gradient: 80 + 80 + 120
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summaries, or None."""
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter
+    summaries (lines `traffic_bytes_per_launch <kernel> <bytes> <workload>`), or None."""
     import glob
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic*.md"))):
         for line in open(f):
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel
+                    and parts[3] == workload):
                 return float(parts[2])
     return None
 
@@ -364,7 +366,7 @@ def main():
         "grad_bytes_roofline_frac": (EL_BYTES_GRAD if elastic else BYTES_GRAD) * S * N * wl.nt * args.steps
                                     / elapsed / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                     "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, wl.name),
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "avg_launch_ms": kern[dom]["avg_launch_ms"]},
         "kernels": kern,
