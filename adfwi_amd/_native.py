@@ -43,7 +43,8 @@ class AcousticFwdArgs(ctypes.Structure):
     _fields_ = [("state_in", _vp), ("work", _vp * 2), ("ckpt", _vp), ("ckpt_every", _i32),
                 ("t0", _i32), ("nsteps", _i32), ("energy_u_from", _i32),
                 ("rec_p", _vp), ("rec_u", _vp), ("rec_w", _vp), ("energy", _vp),
-                ("image", _vp), ("image_p", _vp), ("state_out", _vp)]
+                ("image", _vp), ("image_p", _vp), ("state_out", _vp), ("rcv_map", _vp),
+                ("fuse_steps", _i32), ("reserved2", _i32)]
 
 
 class AcousticAdjArgs(ctypes.Structure):

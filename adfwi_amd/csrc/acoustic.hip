@@ -230,6 +230,252 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
 }
 
 // ---------------------------------------------------------------------------
+// two forward steps in one launch (temporal blocking)
+// ---------------------------------------------------------------------------
+// Step A (t) runs exactly as ac_fwd_step: at iteration k it forms p'(k) and emits the finished
+// row r = k-2 of the intermediate state (p', u', w').  Step B (t+1) consumes those rows from
+// registers, three rows behind: p''(q) at q = k-3 and u'', w'' at q' = k-5, and only B's rows are
+// stored.  The intermediate state never touches HBM; its receiver samples are written when
+// their row is emitted (through a receiver-index map), its energy terms wait three rows in
+// registers so that one read-modify-write adds both steps.  Every lane needs 6 columns of halo
+// on each side (3 per step), so a wavefront writes 52 of its 64 columns.
+constexpr int HALO2 = 6;
+constexpr int SX2 = WAVE - 2 * HALO2;   // 52
+
+struct PairIO {
+  const float *in;
+  float *out;
+  int t;
+  int want_u0, want_u1;                 // u^2 enters the energy map for step t / t+1
+  float *energy;
+  float *img_d0, *img_d1, *img_p0, *img_p1;
+  const int *rmap;                      // [Z] receivers per row, then [Z][pitch] receiver index or -1
+  float *rec_p, *rec_u, *rec_w;         // records; step t is written here
+};
+
+__device__ __forceinline__ bool strip_of2(const AcDev &d, int &z0, int &x0) {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int xcd = b & 7, per = n >> 3, rem = n & 7;
+  const int logical = xcd * per + min(xcd, rem) + (b >> 3);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int strip = logical * WPB + wv;
+  if (strip >= d.nstrips) return false;
+  const int sz = strip / d.nsx;
+  z0 = d.zbase + sz * d.tz;
+  x0 = (strip - sz * d.nsx) * SX2;
+  return true;
+}
+
+template <int SPB>
+__global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
+  int z0, x0;
+  if (!strip_of2(d, z0, x0)) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - HALO2 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= HALO2 && lane < HALO2 + SX2 && c < d.X;
+  const bool cp = c >= 2 && c <= d.X - 3;
+  const bool cu = c >= 1 && c <= d.X - 3;
+  const bool cw = c >= 1 && c <= d.X - 2;
+  const bool u_ok = c < d.X - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch;
+  const unsigned planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  const int s0 = blockIdx.y * SPB;
+  const int ns = min(SPB, d.S - s0);
+  const int zmax = d.Z - 1;
+  const int fs = d.fs;
+  const int end_row = min(z0 + d.tz, d.Z);
+  const int t = io.t;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rP = rsrc(io.in, fieldb), rU = rsrc(io.in + d.field, fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
+  const rsrc_t oP = rsrc(io.out, fieldb), oU = rsrc(io.out + d.field, fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
+  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rI0 = rsrc(io.img_d0, io.img_d0 ? fieldb : 0u), rI1 = rsrc(io.img_d1, io.img_d1 ? fieldb : 0u);
+  const rsrc_t rJ0 = rsrc(io.img_p0, io.img_p0 ? fieldb : 0u), rJ1 = rsrc(io.img_p1, io.img_p1 ? fieldb : 0u);
+  const rsrc_t rE = rsrc(io.energy, io.energy ? 2u * planeb * gridDim.y : 0u);
+  const unsigned eb = 2u * planeb * blockIdx.y;
+  const bool recs = io.rec_p != nullptr && d.nrec > 0;
+  unsigned so[SPB];
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
+
+  // step A windows (as ac_fwd_step): pw* new p rows k-3..k; ww* old w rows k-2..k, cW row k+1;
+  // uw* old u rows k-2..k-1, cU row k; cP old p row k
+  float pw0[SPB], pw1[SPB], pw2[SPB], pw3[SPB];
+  float ww0[SPB], ww1[SPB], ww2[SPB];
+  float uw0[SPB], uw1[SPB];
+  float cW[SPB], cU[SPB], cP[SPB];
+  // step B windows over A's emitted rows (r = k-2): bw* w' rows r-3..r, bu* u' rows r-3..r,
+  // bpo p' row r-1; qw* new p'' rows q-3..q
+  float bw0[SPB], bw1[SPB], bw2[SPB], bw3[SPB];
+  float bu0[SPB], bu1[SPB], bu2[SPB], bu3[SPB];
+  float bpo[SPB];
+  float qw0[SPB], qw1[SPB], qw2[SPB], qw3[SPB];
+  float sv0[SPB], sv1[SPB];
+  int zs[SPB], xs[SPB];
+  const int kst = z0 - 4;
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) {
+    pw0[j] = pw1[j] = pw2[j] = pw3[j] = 0.0f;
+    uw0[j] = uw1[j] = 0.0f;
+    bw0[j] = bw1[j] = bw2[j] = bw3[j] = 0.0f;
+    bu0[j] = bu1[j] = bu2[j] = bu3[j] = 0.0f;
+    bpo[j] = 0.0f;
+    qw0[j] = qw1[j] = qw2[j] = qw3[j] = 0.0f;
+    const int s = s0 + min(j, ns - 1);
+    zs[j] = j < ns ? d.src_z[s] : -1000;
+    xs[j] = d.src_x[s];
+    sv0[j] = d.src[(long long)s * d.nt + t];
+    sv1[j] = d.src[(long long)s * d.nt + t + 1];
+    ww0[j] = bld(rW, vc, so[j] + rb(kst - 2));
+    ww1[j] = bld(rW, vc, so[j] + rb(kst - 1));
+    ww2[j] = bld(rW, vc, so[j] + rb(kst));
+    cW[j] = bld(rW, vc, so[j] + rb(kst + 1));
+    cU[j] = bld(rU, vc, so[j] + rb(kst));
+    cP[j] = bld(rP, vc, so[j] + rb(kst));
+  }
+  // coefficients: A at rows k (1-K1, A1) and k-2 (1-K2, A2, 1-K3); B at rows k-3 and k-5
+  float ak1 = bld(rC, vc, rb(kst)), aa1 = bld(rC, vc, planeb + rb(kst));
+  float ak2 = bld(rC, vc, 2u * planeb + rb(kst - 2)), aa2 = bld(rC, vc, 3u * planeb + rb(kst - 2));
+  float ak3 = bld(rC, vc, 4u * planeb + rb(kst - 2));
+  float bk1 = bld(rC, vc, rb(kst - 3)), ba1 = bld(rC, vc, planeb + rb(kst - 3));
+  float bk2 = bld(rC, vc, 2u * planeb + rb(kst - 5)), ba2 = bld(rC, vc, 3u * planeb + rb(kst - 5));
+  float bk3 = bld(rC, vc, 4u * planeb + rb(kst - 5));
+  // energy of A's emitted rows, waiting for B's rows three iterations later
+  float epa0 = 0.f, epa1 = 0.f, epa2 = 0.f, eua0 = 0.f, eua1 = 0.f, eua2 = 0.f;
+
+  const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
+#pragma unroll 1
+  for (int kk = 0; kk < iters; ++kk) {
+    const int k = kst + kk;
+    const int r = k - 2, q = k - 3, qq = k - 5;
+    const unsigned rk = rb(k), rr = rb(r), rq = rb(q), rqq = rb(qq);
+    // next row's loads before this row's stores
+    float nW[SPB], nU[SPB], nP[SPB];
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      nW[j] = bld(rW, vc, so[j] + rb(k + 2));
+      nU[j] = bld(rU, vc, so[j] + rb(k + 1));
+      nP[j] = bld(rP, vc, so[j] + rb(k + 1));
+    }
+    const float nak1 = bld(rC, vc, rb(k + 1)), naa1 = bld(rC, vc, planeb + rb(k + 1));
+    const float nak2 = bld(rC, vc, 2u * planeb + rb(r + 1)), naa2 = bld(rC, vc, 3u * planeb + rb(r + 1));
+    const float nak3 = bld(rC, vc, 4u * planeb + rb(r + 1));
+    const float nbk1 = bld(rC, vc, rb(q + 1)), nba1 = bld(rC, vc, planeb + rb(q + 1));
+    const float nbk2 = bld(rC, vc, 2u * planeb + rb(qq + 1)), nba2 = bld(rC, vc, 3u * planeb + rb(qq + 1));
+    const float nbk3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+
+    const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
+    const bool rowu = cu && r >= fs && r <= d.Z - 2;
+    const bool roww = cw && r >= fs && r <= d.Z - 3;
+    const bool emit = kk >= 3;                                       // A's row r is complete
+    const bool own_k = out_col && k >= z0 && k < end_row;
+    const bool own_r = out_col && r >= z0 && r < end_row;
+    const bool rowp2 = cp && q >= fs + 1 && q <= d.Z - 3;
+    const bool rowu2 = cu && qq >= fs && qq <= d.Z - 2;
+    const bool roww2 = cw && qq >= fs && qq <= d.Z - 3;
+    const bool bstep = kk >= 6;                                      // B's p'' rows q-3..q are complete
+    const bool own_q = out_col && q >= z0 && q < end_row;
+    const bool own_qq = out_col && qq >= z0 && qq < end_row;
+    // receiver slots of A's emitted row (row counts first: uniform test, then the lane's index)
+    int ridx = -1;
+    if (recs && emit && r >= z0 && r < end_row && io.rmap[r] > 0 && own_r)
+      ridx = io.rmap[d.Z + r * d.pitch + c];
+    float epa = 0.0f, eua = 0.0f, epb = 0.0f, eub = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      if (j < ns) {
+        // ---- step A: new pressure at row k
+        const float w3 = cW[j];
+        float pn = cP[j];
+        const float u0 = cU[j];
+        const float um2 = shfl(u0, lane - 2), um1 = shfl(u0, lane - 1), up1 = shfl(u0, lane + 1);
+        float dd = 0.0f;
+        if (rowp) {
+          dd = C1 * (((u0 - um1) + ww2[j]) - ww1[j]) + C2 * (((up1 - um2) + w3) - ww0[j]);
+          pn = ak1 * pn - aa1 * dd;
+        }
+        if (k == zs[j] && c == xs[j]) pn = pn + sv0[j];
+        pw0[j] = pw1[j]; pw1[j] = pw2[j]; pw2[j] = pw3[j]; pw3[j] = pn;
+        if (d.fsurf && k == fs + 1) pw1[j] = -pn;
+        if (io.img_d0 && own_k) bst(dd, rI0, vs, so[j] + rk);
+        // ---- step A: u', w' at row r; emit (p', u', w') of row r
+        const float pr = pw1[j];
+        const float pl1 = shfl(pr, lane - 1), pr1 = shfl(pr, lane + 1), pr2 = shfl(pr, lane + 2);
+        float un = uw0[j], wn = ww0[j];
+        if (rowu) un = ak2 * un - aa2 * (C1 * (pr1 - pr) + C2 * (pr2 - pl1));
+        if (roww) wn = ak3 * wn - aa2 * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
+        if (!u_ok) un = 0.0f;
+        if (emit && own_r) {
+          if (io.img_p0) bst(pr, rJ0, vs, so[j] + rr);
+          epa += pr * pr;
+          eua += un * un;
+        }
+        if (ridx >= 0) {
+          const long long ro = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
+          io.rec_p[ro] = pr;
+          io.rec_u[ro] = un;
+          io.rec_w[ro] = (d.fsurf && r == fs - 1) ? 0.0f : wn;   // row fs-1 is not sampled here
+        }
+        ww0[j] = ww1[j]; ww1[j] = ww2[j]; ww2[j] = w3;
+        uw0[j] = uw1[j]; uw1[j] = u0;
+        // ---- step B: window of A's rows (r = k-2)
+        bw0[j] = bw1[j]; bw1[j] = bw2[j]; bw2[j] = bw3[j]; bw3[j] = wn;
+        if (d.fsurf && r == fs) bw2[j] = wn;                 // w'(fs-1) = w'(fs)
+        bu0[j] = bu1[j]; bu1[j] = bu2[j]; bu2[j] = bu3[j]; bu3[j] = un;
+        const float p_old = bpo[j];                          // p'(q), q = r-1
+        bpo[j] = pr;
+        // ---- step B: new pressure at row q = k-3
+        {
+          const float v0 = bu2[j];                           // u'(q)
+          const float vm2 = shfl(v0, lane - 2), vm1 = shfl(v0, lane - 1), vp1 = shfl(v0, lane + 1);
+          float pn2 = p_old;
+          float dd2 = 0.0f;
+          if (rowp2) {
+            dd2 = C1 * (((v0 - vm1) + bw2[j]) - bw1[j]) + C2 * (((vp1 - vm2) + bw3[j]) - bw0[j]);
+            pn2 = bk1 * pn2 - ba1 * dd2;
+          }
+          if (q == zs[j] && c == xs[j]) pn2 = pn2 + sv1[j];
+          qw0[j] = qw1[j]; qw1[j] = qw2[j]; qw2[j] = qw3[j]; qw3[j] = pn2;
+          if (d.fsurf && q == fs + 1) qw1[j] = -pn2;
+          if (io.img_d1 && own_q) bst(dd2, rI1, vs, so[j] + rq);
+        }
+        // ---- step B: u'', w'' at row qq = k-5; store the final rows
+        {
+          const float ps = qw1[j];
+          const float sl1 = shfl(ps, lane - 1), sr1 = shfl(ps, lane + 1), sr2 = shfl(ps, lane + 2);
+          float un2 = bu0[j], wn2 = bw0[j];
+          if (rowu2) un2 = bk2 * un2 - ba2 * (C1 * (sr1 - ps) + C2 * (sr2 - sl1));
+          if (roww2) wn2 = bk3 * wn2 - ba2 * (C1 * (qw2[j] - ps) + C2 * (qw3[j] - qw0[j]));
+          if (bstep && own_qq) {
+            bst(ps, oP, vs, so[j] + rqq);
+            if (io.img_p1) bst(ps, rJ1, vs, so[j] + rqq);
+            if (u_ok) bst(un2, oU, vs, so[j] + rqq); else un2 = 0.0f;
+            if (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) bst(wn2, oW, vs, so[j] + rqq);
+            if (d.fsurf && qq == fs) bst(wn2, oW, vs, so[j] + rqq - rowb);
+            epb += ps * ps;
+            eub += un2 * un2;
+          }
+        }
+      }
+      cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j];
+    }
+    ak1 = nak1; aa1 = naa1; ak2 = nak2; aa2 = naa2; ak3 = nak3;
+    bk1 = nbk1; ba1 = nba1; bk2 = nbk2; ba2 = nba2; bk3 = nbk3;
+    // energy: A's row r waits three iterations for B's row qq = r-3
+    if (io.energy && bstep && own_qq) {
+      const float ep = epa0 + epb;
+      bst(bld(rE, vs, eb + rqq) + ep, rE, vs, eb + rqq);
+      const float eu = (io.want_u0 ? eua0 : 0.0f) + (io.want_u1 ? eub : 0.0f);
+      if (io.want_u0 || io.want_u1) bst(bld(rE, vs, eb + planeb + rqq) + eu, rE, vs, eb + planeb + rqq);
+    }
+    epa0 = epa1; epa1 = epa2; epa2 = epa;
+    eua0 = eua1; eua1 = eua2; eua2 = eua;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // adjoint step: transpose of stages 6..1 of step t, plus zero-lag imaging.
 // Input: adjoint of the state after step t (receiver sources of step t added).
 // Output: adjoint of the state after step t-1 (sources of t-1 not yet added).
@@ -459,8 +705,9 @@ int layout_of(const adfwi_acoustic_desc *d, adfwi_layout *L) {
   L->tile_x = SX;
   // shots per wavefront (1, 2, 4 or 8, a kernel template) and strip height (4..64 rows): the
   // largest shot group (<= 4) with 16-row strips that still gives >= 4096 wavefronts (about four
-  // per SIMD on 256 CUs), then shorter strips for small problems.  Measured on MI355X for a
-  // 64-shot 464 x 1232 grid: 4 shots x 16 rows beats 2 shots and 32/64-row strips.
+  // per SIMD on 256 CUs), then the tallest strips (32 down to 4 rows) that keep that count; the
+  // two-step kernel uses strips twice as tall.  Measured on MI355X for a 64-shot 464 x 1232 grid:
+  // 4 shots x 32 rows (64 for the two-step kernel) beats 2 shots and 8/16-row strips.
   const int rows = d->nz - L->zbase;
   auto waves = [&](int spb_, int tz_) {
     return (long long)L->ntiles_x * ((rows + tz_ - 1) / tz_) * ((d->nshots + spb_ - 1) / spb_);
@@ -477,7 +724,7 @@ int layout_of(const adfwi_acoustic_desc *d, adfwi_layout *L) {
   }
   if (tz == 0) {
     tz = 4;
-    for (int cand = 16; cand >= 4; cand >>= 1)
+    for (int cand = 32; cand >= 4; cand >>= 1)
       if (waves(spb, cand) >= 4096) { tz = cand; break; }
   }
   L->tile_z = tz;
@@ -541,6 +788,25 @@ void launch_fwd(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in,
   }
 }
 
+// the pair kernel tiles the grid with 52-column strips of twice the single-step height
+AcDev pair_dev(const AcDev &v) {
+  AcDev w = v;
+  w.tz = min(TZ_MAX, 2 * v.tz);
+  w.nsx = (v.X + SX2 - 1) / SX2;
+  w.nsz = (v.Z - v.zbase + w.tz - 1) / w.tz;
+  w.nstrips = w.nsx * w.nsz;
+  return w;
+}
+
+void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
+  switch (w.spb) {
+    case 8:
+    case 4: hipLaunchKernelGGL((ac_fwd_pair<4>), g, b, 0, st, w, io); break;
+    case 2: hipLaunchKernelGGL((ac_fwd_pair<2>), g, b, 0, st, w, io); break;
+    default: hipLaunchKernelGGL((ac_fwd_pair<1>), g, b, 0, st, w, io);
+  }
+}
+
 template <int SPB>
 void launch_adj_u(const AcDev &v, dim3 g, dim3 b, hipStream_t st, const float *in, float *out, int t,
                   const float *imd, const float *imp, float *ga1, float *ga2, float *gs) {
@@ -589,25 +855,48 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   const dim3 grid = step_grid(L), block(NT);
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  // two steps per launch where the first step's state is not needed in HBM (it is not a
+  // checkpoint) and its receiver samples can be written through the receiver-index map
+  const bool fuse = a->fuse_steps != 0 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
+  const AcDev w = pair_dev(v);
+  const dim3 pgrid((w.nstrips + WPB - 1) / WPB, L.ngroups);
+  auto ckpt_slot = [&](int t) -> float * {   // checkpoint buffer receiving the state after step t
+    if (a->ckpt && (t + 1) % a->ckpt_every == 0 && t + 1 < d->nt)
+      return a->ckpt + (long long)((t + 1) / a->ckpt_every - 1) * L.state_floats;
+    return nullptr;
+  };
   const float *cur = a->state_in;
-  for (int k = 0; k < a->nsteps; ++k) {
+  for (int k = 0; k < a->nsteps;) {
     const int t = a->t0 + k;
-    float *nxt;
-    if (a->ckpt && (t + 1) % a->ckpt_every == 0 && t + 1 < d->nt) {
-      nxt = a->ckpt + (long long)((t + 1) / a->ckpt_every - 1) * L.state_floats;
-    } else {
-      nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
-    }
+    const bool pair = fuse && k + 1 < a->nsteps && ckpt_slot(t) == nullptr;
+    const int tl = pair ? t + 1 : t;         // last step of this launch
+    float *nxt = ckpt_slot(tl);
+    if (!nxt) nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
     float *imd = a->image ? a->image + (long long)k * L.field_floats : nullptr;
     float *imp = a->image_p ? a->image_p + (long long)k * L.field_floats : nullptr;
-    const int acc_u = (a->energy && t >= a->energy_u_from) ? 1 : 0;
-    launch_fwd(v, grid, block, st, cur, nxt, t, a->energy, acc_u, imd, imp);
-    if ((rc = adfwi::check_launch("ac_fwd_step"))) return rc;
+    if (pair) {
+      PairIO io;
+      io.in = cur; io.out = nxt; io.t = t;
+      io.energy = a->energy;
+      io.want_u0 = (a->energy && t >= a->energy_u_from) ? 1 : 0;
+      io.want_u1 = (a->energy && t + 1 >= a->energy_u_from) ? 1 : 0;
+      io.img_d0 = imd; io.img_d1 = imd ? imd + L.field_floats : nullptr;
+      io.img_p0 = imp; io.img_p1 = imp ? imp + L.field_floats : nullptr;
+      io.rmap = a->rcv_map;
+      io.rec_p = recs ? a->rec_p : nullptr; io.rec_u = a->rec_u; io.rec_w = a->rec_w;
+      launch_pair(w, pgrid, block, st, io);
+      if ((rc = adfwi::check_launch("ac_fwd_pair"))) return rc;
+    } else {
+      const int acc_u = (a->energy && t >= a->energy_u_from) ? 1 : 0;
+      launch_fwd(v, grid, block, st, cur, nxt, t, a->energy, acc_u, imd, imp);
+      if ((rc = adfwi::check_launch("ac_fwd_step"))) return rc;
+    }
     if (recs && ng > 0) {
-      hipLaunchKernelGGL(ac_gather, ggrid, gblock, 0, st, v, nxt, a->rec_p, a->rec_u, a->rec_w, t);
+      hipLaunchKernelGGL(ac_gather, ggrid, gblock, 0, st, v, nxt, a->rec_p, a->rec_u, a->rec_w, tl);
       if ((rc = adfwi::check_launch("ac_gather"))) return rc;
     }
     cur = nxt;
+    k += pair ? 2 : 1;
   }
   a->state_out = cur;
   return ADFWI_OK;

@@ -57,7 +57,8 @@ class Timeline:
         self.enabled = False
         self.records = []
 
-    def call(self, label, kernel, launches, device, fn):
+    def call(self, label, kernel, launches, device, fn, steps=None):
+        """launches: kernel launches of the call; steps: time steps they advance (default: launches)."""
         if not self.enabled:
             return fn()
         st = torch.cuda.current_stream(device)
@@ -65,19 +66,39 @@ class Timeline:
         a.record(st)
         out = fn()
         b.record(st)
-        self.records.append((label, kernel, launches, a, b))
+        self.records.append((label, kernel, launches, launches if steps is None else steps, a, b))
         return out
 
     def summary(self):
         torch.cuda.synchronize()
         out = {}
-        for label, kernel, n, a, b in self.records:
-            d = out.setdefault(kernel, {"ms": 0.0, "launches": 0, "calls": {}})
+        for label, kernel, n, nstep, a, b in self.records:
+            d = out.setdefault(kernel, {"ms": 0.0, "launches": 0, "steps": 0, "calls": {}, "steps_by_call": {}})
             ms = a.elapsed_time(b)
             d["ms"] += ms
             d["launches"] += n
+            d["steps"] += nstep
             d["calls"][label] = d["calls"].get(label, 0.0) + ms
+            d["steps_by_call"][label] = d["steps_by_call"].get(label, 0) + nstep
         return out
+
+
+def forward_launches(t0: int, n: int, nt: int, ckpt_every: int, fuse: bool):
+    """Launches adfwi_acoustic_forward issues for steps t0 .. t0+n-1: (pairs, single steps).
+    A step is paired with the next one unless it is the call's last step or its state is a
+    checkpoint."""
+    pairs = singles = 0
+    k = 0
+    while k < n:
+        t = t0 + k
+        ck = ckpt_every and (t + 1) % ckpt_every == 0 and t + 1 < nt
+        if fuse and k + 1 < n and not ck:
+            pairs += 1
+            k += 2
+        else:
+            singles += 1
+            k += 1
+    return pairs, singles
 
 
 TIMELINE = Timeline()
@@ -121,10 +142,27 @@ class AcousticGeometry:
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
+        # two steps per launch (the intermediate step's receivers written from inside the kernel)
+        self.fuse_steps = int(os.environ.get("ADFWI_FUSE_STEPS", "1"))
+        self.rcv_map = self._receiver_map(rcv_z, rcv_x) if self.fuse_steps else None
         desc = self.desc(None, None)
         self.layout = nat.Layout()
         nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),
                   "adfwi_acoustic_get_layout")
+
+    def _receiver_map(self, rcv_z: np.ndarray, rcv_x: np.ndarray) -> Optional[torch.Tensor]:
+        """[Z] receivers per row, then [Z][pitch] receiver index (-1 elsewhere); None when
+        receivers repeat a point or sit on the free-surface mirror row fs-1 (records are then
+        taken one step per launch)."""
+        if self.R == 0:
+            return None
+        pts = rcv_z.astype(np.int64) * self.pitch + rcv_x.astype(np.int64)
+        if len(np.unique(pts)) != len(pts) or (self.free_surface and np.any(rcv_z == self.fs - 1)):
+            return None
+        m = np.full(self.Z + self.Z * self.pitch, -1, dtype=np.int32)
+        m[:self.Z] = np.bincount(rcv_z, minlength=self.Z)[:self.Z]
+        m[self.Z + pts] = np.arange(self.R, dtype=np.int32)
+        return torch.tensor(m, device=self.device)
 
     def desc(self, coef: Optional[torch.Tensor], src: Optional[torch.Tensor]) -> nat.AcousticDesc:
         d = nat.AcousticDesc()
@@ -186,9 +224,13 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
     if rec is not None:
         a.rec_p, a.rec_u, a.rec_w = rec[0].data_ptr(), rec[1].data_ptr(), rec[2].data_ptr()
     a.energy = None if energy is None else energy.data_ptr()
-    TIMELINE.call("forward", "ac_fwd_step", nt, dev, lambda: nat.check(
+    a.rcv_map = None if geom.rcv_map is None else geom.rcv_map.data_ptr()
+    a.fuse_steps = geom.fuse_steps
+    fused = bool(geom.fuse_steps) and (rec is None or geom.rcv_map is not None)
+    pairs, singles = forward_launches(0, nt, nt, int(ckpt_every), fused)
+    TIMELINE.call("forward", "ac_fwd_pair" if fused else "ac_fwd_step", pairs + singles, dev, lambda: nat.check(
         lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(a), nat.stream_handle(dev)),
-        "adfwi_acoustic_forward"))
+        "adfwi_acoustic_forward"), steps=nt)
     out = {"rec": rec, "ckpt": ckpt, "energy": None}
     if energy is not None:
         final = work[0] if a.state_out == work[0].data_ptr() else (
@@ -244,9 +286,12 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
             fa.energy_u_from = nt
             fa.image = None if image is None else image.data_ptr()
             fa.image_p = None if image_p is None else image_p.data_ptr()
-            TIMELINE.call("recompute", "ac_fwd_step", n, dev, lambda: nat.check(
+            fa.fuse_steps = geom.fuse_steps
+            pairs, singles = forward_launches(t_lo, n, nt, 0, bool(geom.fuse_steps))
+            TIMELINE.call("recompute", "ac_fwd_pair" if geom.fuse_steps else "ac_fwd_step", pairs + singles,
+                          dev, lambda: nat.check(
                 lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(fa), stream),
-                "adfwi_acoustic_forward (recompute)"))
+                "adfwi_acoustic_forward (recompute)"), steps=n)
         aa = nat.AcousticAdjArgs()
         aa.lam_in = lam.data_ptr()
         aa.work[0], aa.work[1] = lw[0].data_ptr(), lw[1].data_ptr()

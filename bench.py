@@ -330,18 +330,17 @@ def main():
     # dominant kernel by total time; its algorithmic bytes per launch
     kern = {}
     for name, d in tl.items():
-        if name == "ac_fwd_step":
-            launches_fwd = args.steps * wl.nt
-            launches_rec = d["launches"] - launches_fwd
-            byts = launches_fwd * BYTES_FWD * N * S + launches_rec * BYTES_RECOMPUTE * N * S
+        steps_by = d["steps_by_call"]
+        if name in ("ac_fwd_step", "ac_fwd_pair"):
+            byts = (steps_by.get("forward", 0) * BYTES_FWD + steps_by.get("recompute", 0) * BYTES_RECOMPUTE) * N * S
         elif name == "ac_adj_step":
-            byts = d["launches"] * BYTES_ADJ * N * S
+            byts = d["steps"] * BYTES_ADJ * N * S
         elif name == "el_fwd_step":
-            byts = d["launches"] * EL_BYTES_FWD * N * S
+            byts = d["steps"] * EL_BYTES_FWD * N * S
         else:
-            byts = d["launches"] * EL_BYTES_ADJ * N * S
+            byts = d["steps"] * EL_BYTES_ADJ * N * S
         avg_ms = d["ms"] / max(d["launches"], 1)
-        kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "avg_launch_ms": avg_ms,
+        kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "steps": d["steps"], "avg_launch_ms": avg_ms,
                       "achieved_gbs": byts / (d["ms"] / 1e3) / 1e9 if d["ms"] else 0.0,
                       "bytes_per_launch": byts / max(d["launches"], 1)}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])

@@ -102,6 +102,12 @@ typedef struct adfwi_acoustic_fwd_args {
   float *image;            /* [nsteps][S][Z][pitch] or NULL: p-update stencil of each step */
   float *image_p;          /* [nsteps][S][Z][pitch] or NULL: p after each step */
   const float *state_out;  /* OUT: buffer holding the state after the last step */
+  const int32_t *rcv_map;  /* [nz + nz*pitch] or NULL: receivers per row, then the receiver index
+                              at each point (-1 elsewhere); lets two steps share one launch while
+                              records are taken (receivers distinct, none on row fs-1 with a free
+                              surface) */
+  int32_t fuse_steps;      /* 1: advance two steps per launch where possible; 0: one */
+  int32_t reserved2;
 } adfwi_acoustic_fwd_args;
 
 /* Forward time stepping (replaces step_forward, acoustic_kernels.py:40-176). */

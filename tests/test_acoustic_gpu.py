@@ -182,3 +182,49 @@ def test_propagator_class_end_to_end():
     assert rel_l2(out["p"].detach().cpu().numpy(), ref["p"].detach().numpy()) <= 1e-5
     assert rel_l2(g_h, v.grad.numpy()) <= 1e-4
     assert F.src_n == 3 and out["forward_wavefield_p"].shape == (nz, nx)
+
+
+@pytest.mark.parametrize("free_surface", [False, True])
+@pytest.mark.parametrize("nt,spb,tz", [(121, 4, 0), (150, 2, 8), (97, 1, 4)])
+def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surface, nt, spb, tz):
+    """The fused two-step kernel (intermediate state in registers, its receivers written in the
+    kernel) reproduces the one-step kernel: records, checkpoints, final state and gradients bit
+    for bit, energy maps to rounding (two steps' terms are added before the read-modify-write)."""
+    from adfwi_amd.propagator import acoustic_kernels as ak
+    from adfwi_amd.propagator.boundary_condition import bc_pml
+    rng = np.random.default_rng(nt + spb)
+    nz, nx, nabc, dt, dx = 70, 170, 10, 1e-3, 10.0
+    vp = rng.uniform(1500, 3200, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+    damp = torch.tensor(bc_pml(nx, nz, dx, dx, nabc, float(vp.max()), False), dtype=torch.float32, device=DEV)
+    S = 7
+    src_x = np.linspace(1, nx - 2, S).astype(int) + nabc
+    src_z = (np.arange(S) * 9 % (nz - 2) + (nabc if free_surface else 1)) + (0 if free_surface else nabc - 1)
+    rcv_x = np.arange(0, nx, 3) + nabc
+    rcv_z = np.full(len(rcv_x), nabc + 2)
+    rcv_z[::5] = nabc + 30                                  # a second receiver row
+    wav = torch.tensor(rng.standard_normal((S, nt)), dtype=torch.float32, device=DEV)
+    Z, X = nz + 2 * nabc, nx + 2 * nabc
+    a1, k1, a2, k2, k3 = ak.coefficient_fields(torch.tensor(vp, device=DEV), torch.tensor(rho, device=DEV), damp,
+                                               nz, nx, nabc, dt, dx, free_surface, DEV)
+    outs = []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("ADFWI_FUSE_STEPS", fuse)
+        if tz:
+            monkeypatch.setenv("ADFWI_TILE_Z", str(tz))
+        geom = ak.AcousticGeometry(Z, X, nt, free_surface, nabc, src_z, src_x, rcv_z, rcv_x, DEV, shots_per_block=spb)
+        assert (geom.rcv_map is not None) == (fuse == "1")
+        a1_ = a1.detach().clone().requires_grad_(True)
+        a2_ = a2.detach().clone().requires_grad_(True)
+        w_ = wav.clone().requires_grad_(True)
+        res = ak.AcousticWave.apply(a1_, a2_, dt * w_, 1.0 - k1, 1.0 - k2, 1.0 - k3, geom, nt // 3, 9)
+        g = [torch.randn(r.shape, generator=torch.Generator().manual_seed(i)).to(DEV) for i, r in enumerate(res[:3])]
+        torch.autograd.backward(res[:3], g)
+        torch.cuda.synchronize()
+        outs.append(([r.detach().cpu() for r in res], a1_.grad.cpu(), a2_.grad.cpu(), w_.grad.cpu()))
+    (r0, ga0, gb0, gw0), (r1, ga1, gb1, gw1) = outs
+    for i in range(3):
+        assert torch.equal(r0[i], r1[i]), ("record", i, rel_l2(r1[i].numpy(), r0[i].numpy()))
+    for i in range(3, 6):
+        assert rel_l2(r1[i].numpy(), r0[i].numpy()) <= 1e-6, ("energy", i)
+    assert torch.equal(ga0, ga1) and torch.equal(gb0, gb1) and torch.equal(gw0, gw1)
