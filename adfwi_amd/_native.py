@@ -27,7 +27,7 @@ _i64 = ctypes.c_int64
 class AcousticDesc(ctypes.Structure):
     _fields_ = [("abi_version", _i32), ("struct_size", _i32), ("nshots", _i32), ("nz", _i32),
                 ("nx", _i32), ("pitch", _i32), ("fs", _i32), ("free_surface", _i32), ("nt", _i32),
-                ("nrec", _i32), ("shots_per_block", _i32), ("row_unroll", _i32), ("tile_z", _i32), ("reserved", _i32),
+                ("nrec", _i32), ("shots_per_block", _i32), ("row_unroll", _i32), ("tile_z", _i32), ("adj_shots_per_block", _i32),
                 ("coef", _vp), ("src", _vp), ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp),
                 ("rcv_x", _vp)]
 
@@ -51,7 +51,8 @@ class AcousticAdjArgs(ctypes.Structure):
     _fields_ = [("lam_in", _vp), ("work", _vp * 2), ("t_hi", _i32), ("nsteps", _i32),
                 ("image_t0", _i32), ("reserved", _i32), ("image", _vp), ("image_p", _vp),
                 ("grad_rec_p", _vp), ("grad_rec_u", _vp), ("grad_rec_w", _vp),
-                ("grad_a1", _vp), ("grad_a2", _vp), ("grad_src", _vp), ("lam_out", _vp)]
+                ("grad_a1", _vp), ("grad_a2", _vp), ("grad_src", _vp), ("lam_out", _vp),
+                ("rcv_map", _vp), ("fuse_steps", _i32), ("reserved2", _i32)]
 
 
 _f32 = ctypes.c_float

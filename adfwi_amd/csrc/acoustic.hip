@@ -654,6 +654,276 @@ __global__ __launch_bounds__(NT) void ac_adj_step(
   }
 }
 
+// ---------------------------------------------------------------------------
+// two adjoint steps in one launch
+// ---------------------------------------------------------------------------
+// Step B reverses step t+1 exactly as ac_adj_step does (its input is the adjoint of the state
+// after step t+1 with the receiver sources of t+1 injected) and emits the finished row r = k-2
+// of the adjoint state after step t.  The receiver sources of step t are added to the emitted
+// rows on every lane (through the receiver-index map), and step A reverses step t from those
+// rows three rows behind: its lambda_p3 at row q = k-3 and its outputs at row k-5, the only
+// rows stored.  B's image terms wait three rows in registers so that one read-modify-write
+// adds both steps to the gradient planes.
+struct AdjPairIO {
+  const float *in;
+  float *out;
+  int t;                                 // step A; step B is t+1
+  const float *img_d0, *img_d1;          // stencil images of steps t and t+1
+  const float *img_p0, *img_p1;          // pressure images (A2 gradient)
+  float *grad_a1, *grad_a2, *grad_src;
+  const int *rmap;
+  const float *g_p, *g_u, *g_w;          // record adjoints [S][nt][R]; step t's are added here
+};
+
+template <int SPB, bool A2>
+__global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
+  int z0, x0;
+  if (!strip_of2(d, z0, x0)) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - HALO2 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= HALO2 && lane < HALO2 + SX2 && c < d.X;
+  const bool cp = c >= 2 && c <= d.X - 3;
+  const bool cu = c >= 1 && c <= d.X - 3;
+  const bool cw = c >= 1 && c <= d.X - 2;
+  const bool in_col = c >= 0 && c < d.X;
+  const unsigned rowb = 4u * (unsigned)d.pitch;
+  const unsigned planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  const int s0 = blockIdx.y * SPB;
+  const int ns = min(SPB, d.S - s0);
+  const int zmax = d.Z - 1;
+  const int fs = d.fs;
+  const int fsr = d.fsurf ? fs : -100;   // row whose adjoint collects the free-surface copy
+  const int end_row = min(z0 + d.tz, d.Z);
+  const int t = io.t;
+  const bool want_a1 = io.grad_a1 != nullptr && io.img_d0 != nullptr && io.img_d1 != nullptr;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rP = rsrc(io.in, fieldb), rU = rsrc(io.in + d.field, fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
+  const rsrc_t oP = rsrc(io.out, fieldb), oU = rsrc(io.out + d.field, fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
+  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rI0 = rsrc(io.img_d0, want_a1 ? fieldb : 0u), rI1 = rsrc(io.img_d1, want_a1 ? fieldb : 0u);
+  const rsrc_t rJ0 = rsrc(io.img_p0, A2 ? fieldb : 0u), rJ1 = rsrc(io.img_p1, A2 ? fieldb : 0u);
+  const rsrc_t rG1 = rsrc(io.grad_a1, want_a1 ? planeb * gridDim.y : 0u);
+  const rsrc_t rG2 = rsrc(io.grad_a2, A2 ? planeb * gridDim.y : 0u);
+  const unsigned gb = planeb * blockIdx.y;
+  const bool recs = io.rmap != nullptr && d.nrec > 0;
+  unsigned so[SPB];
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
+  auto wmask = [&](int kr) -> bool { return cw && kr >= fs && kr <= d.Z - 3; };
+  auto lwfix = [&](unsigned soj, int kr, float raw) -> float {   // step B: raw rows from memory
+    if (kr == fsr) return raw + bld(rW, vc, soj + rb(kr - 1));
+    if (kr == fsr - 1) return 0.0f;
+    return raw;
+  };
+  const int kst = z0 - 4;
+
+  // step B windows (rows of k): lw* lambda_w' rows k-2..k+1, l2* lambda_p2 rows k-2..k,
+  // g* A1-masked lambda_p2 rows k-3..k, lu* lambda_u rows k-2..k, pp* pressure images k-3..k;
+  // c* the values of row k (k+1 for w) loaded one iteration ahead
+  float lw0[SPB], lw1[SPB], lw2[SPB], lw3[SPB], l2a[SPB], l2b[SPB], l2c[SPB];
+  float g0[SPB], g1[SPB], g2[SPB], g3[SPB], lu0[SPB], lu1[SPB], lu2[SPB];
+  float pp0[SPB], pp1[SPB], pp2[SPB], pp3[SPB];
+  float cW[SPB], cU[SPB], cP[SPB], cI[SPB], cIp[SPB];
+  // step A windows, the same over the rows of q = k-3, fed by B's emitted rows; eP, eU, eW hold
+  // B's previous emitted row; dI, dIp the step-t images of row q
+  float mw0[SPB], mw1[SPB], mw2[SPB], mw3[SPB], m2a[SPB], m2b[SPB], m2c[SPB];
+  float h0[SPB], h1[SPB], h2[SPB], h3[SPB], mu0[SPB], mu1[SPB], mu2[SPB];
+  float qp0[SPB], qp1[SPB], qp2[SPB], qp3[SPB];
+  float eP[SPB], eU[SPB], eW[SPB], dI[SPB], dIp[SPB];
+  int zs[SPB], xs[SPB];
+#pragma unroll
+  for (int j = 0; j < SPB; ++j) {
+    lw0[j] = 0.0f; l2a[j] = l2b[j] = l2c[j] = 0.0f; g0[j] = g1[j] = g2[j] = g3[j] = 0.0f;
+    lu0[j] = lu1[j] = lu2[j] = 0.0f; pp0[j] = pp1[j] = pp2[j] = pp3[j] = 0.0f;
+    mw0[j] = mw1[j] = mw2[j] = mw3[j] = 0.0f; m2a[j] = m2b[j] = m2c[j] = 0.0f;
+    h0[j] = h1[j] = h2[j] = h3[j] = 0.0f; mu0[j] = mu1[j] = mu2[j] = 0.0f;
+    qp0[j] = qp1[j] = qp2[j] = qp3[j] = 0.0f; eP[j] = eU[j] = eW[j] = 0.0f;
+    const int s = s0 + min(j, ns - 1);
+    zs[j] = j < ns ? d.src_z[s] : -1000;
+    xs[j] = d.src_x[s];
+    lw1[j] = lwfix(so[j], kst - 2, bld(rW, vc, so[j] + rb(kst - 2)));
+    lw2[j] = lwfix(so[j], kst - 1, bld(rW, vc, so[j] + rb(kst - 1)));
+    lw3[j] = lwfix(so[j], kst, bld(rW, vc, so[j] + rb(kst)));
+    cW[j] = bld(rW, vc, so[j] + rb(kst + 1));
+    cU[j] = bld(rU, vc, so[j] + rb(kst));
+    cP[j] = bld(rP, vc, so[j] + rb(kst));
+    cI[j] = want_a1 ? bld(rI1, vc, so[j] + rb(kst)) : 0.0f;
+    cIp[j] = A2 ? bld(rJ1, vc, so[j] + rb(kst)) : 0.0f;
+    dI[j] = want_a1 ? bld(rI0, vc, so[j] + rb(kst - 3)) : 0.0f;
+    dIp[j] = A2 ? bld(rJ0, vc, so[j] + rb(kst - 3)) : 0.0f;
+  }
+  // shot-independent coefficient windows: A2 on the w rows (B: k-2..k+1, A: q-2..q+1), and per
+  // row A2 (u), A1 at k / q and 1-K1, 1-K2, 1-K3 at k-2 / q-2, loaded one iteration ahead
+  float aw0 = 0.0f;
+  float aw1 = wmask(kst - 2) ? bld(rC, vc, 3u * planeb + rb(kst - 2)) : 0.0f;
+  float aw2 = wmask(kst - 1) ? bld(rC, vc, 3u * planeb + rb(kst - 1)) : 0.0f;
+  float aw3 = wmask(kst) ? bld(rC, vc, 3u * planeb + rb(kst)) : 0.0f;
+  float cA2w = wmask(kst + 1) ? bld(rC, vc, 3u * planeb + rb(kst + 1)) : 0.0f;
+  float cA2u = bld(rC, vc, 3u * planeb + rb(kst)), cA1 = bld(rC, vc, planeb + rb(kst));
+  float cK1 = bld(rC, vc, rb(kst - 2)), cK2 = bld(rC, vc, 2u * planeb + rb(kst - 2));
+  float cK3 = bld(rC, vc, 4u * planeb + rb(kst - 2));
+  const int qst = kst - 3;
+  float bw0 = 0.0f;
+  float bw1 = wmask(qst - 2) ? bld(rC, vc, 3u * planeb + rb(qst - 2)) : 0.0f;
+  float bw2 = wmask(qst - 1) ? bld(rC, vc, 3u * planeb + rb(qst - 1)) : 0.0f;
+  float bw3 = wmask(qst) ? bld(rC, vc, 3u * planeb + rb(qst)) : 0.0f;
+  float dA2w = wmask(qst + 1) ? bld(rC, vc, 3u * planeb + rb(qst + 1)) : 0.0f;
+  float dA2u = bld(rC, vc, 3u * planeb + rb(qst)), dA1 = bld(rC, vc, planeb + rb(qst));
+  float dK1 = bld(rC, vc, rb(qst - 2)), dK2 = bld(rC, vc, 2u * planeb + rb(qst - 2));
+  float dK3 = bld(rC, vc, 4u * planeb + rb(qst - 2));
+  // B's image terms of rows k and r waiting three iterations for A's rows q and qq
+  float ga1d0 = 0.f, ga1d1 = 0.f, ga1d2 = 0.f, ga2d0 = 0.f, ga2d1 = 0.f, ga2d2 = 0.f;
+
+  const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
+#pragma unroll 1
+  for (int kk = 0; kk < iters; ++kk) {
+    const int k = kst + kk;
+    const int r = k - 2, q = k - 3, qq = k - 5;
+    const unsigned rk = rb(k), rr = rb(r), rq = rb(q), rqq = rb(qq);
+    // next row's loads before this row's stores
+    float nW[SPB], nU[SPB], nP[SPB], nI[SPB], nIp[SPB], mI[SPB], mIp[SPB];
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      nW[j] = bld(rW, vc, so[j] + rb(k + 2));
+      nU[j] = bld(rU, vc, so[j] + rb(k + 1));
+      nP[j] = bld(rP, vc, so[j] + rb(k + 1));
+      nI[j] = want_a1 ? bld(rI1, vc, so[j] + rb(k + 1)) : 0.0f;
+      nIp[j] = A2 ? bld(rJ1, vc, so[j] + rb(k + 1)) : 0.0f;
+      mI[j] = want_a1 ? bld(rI0, vc, so[j] + rb(q + 1)) : 0.0f;
+      mIp[j] = A2 ? bld(rJ0, vc, so[j] + rb(q + 1)) : 0.0f;
+    }
+    const float nA2w = wmask(k + 2) ? bld(rC, vc, 3u * planeb + rb(k + 2)) : 0.0f;
+    const float nA2u = bld(rC, vc, 3u * planeb + rb(k + 1)), nA1 = bld(rC, vc, planeb + rb(k + 1));
+    const float nK1 = bld(rC, vc, rb(r + 1)), nK2 = bld(rC, vc, 2u * planeb + rb(r + 1));
+    const float nK3 = bld(rC, vc, 4u * planeb + rb(r + 1));
+    const float mA2w = wmask(q + 2) ? bld(rC, vc, 3u * planeb + rb(q + 2)) : 0.0f;
+    const float mA2u = bld(rC, vc, 3u * planeb + rb(q + 1)), mA1 = bld(rC, vc, planeb + rb(q + 1));
+    const float mK1 = bld(rC, vc, rb(qq + 1)), mK2 = bld(rC, vc, 2u * planeb + rb(qq + 1));
+    const float mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+
+    // step B masks (rows k, r) and step A masks (rows q, qq)
+    const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
+    const bool rowu_k = cu && k >= fs && k <= d.Z - 2;
+    const bool pmr = cp && r >= fs + 1 && r <= d.Z - 3;
+    const bool um = cu && r >= fs && r <= d.Z - 2;
+    const bool wm = cw && r >= fs && r <= d.Z - 3;
+    const bool rowp_q = cp && q >= fs + 1 && q <= d.Z - 3;
+    const bool rowu_q = cu && q >= fs && q <= d.Z - 2;
+    const bool pmr_q = cp && qq >= fs + 1 && qq <= d.Z - 3;
+    const bool um_q = cu && qq >= fs && qq <= d.Z - 2;
+    const bool wm_q = cw && qq >= fs && qq <= d.Z - 3;
+    const bool own_k = out_col && k >= z0 && k < end_row;
+    const bool own_r = out_col && r >= z0 && r < end_row;
+    const bool own_q = out_col && q >= z0 && q < end_row;
+    const bool own_qq = out_col && qq >= z0 && qq < end_row;
+    aw0 = aw1; aw1 = aw2; aw2 = aw3; aw3 = cA2w;
+    bw0 = bw1; bw1 = bw2; bw2 = bw3; bw3 = dA2w;
+    const float a2k = rowu_k ? cA2u : 0.0f, a1k = rowp ? cA1 : 0.0f;
+    const float k1r = pmr ? cK1 : 1.0f, k2r = um ? cK2 : 1.0f, k3r = wm ? cK3 : 1.0f;
+    const float a2q = rowu_q ? dA2u : 0.0f, a1q = rowp_q ? dA1 : 0.0f;
+    const float k1q = pmr_q ? dK1 : 1.0f, k2q = um_q ? dK2 : 1.0f, k3q = wm_q ? dK3 : 1.0f;
+    // receiver slot of B's emitted row r (step t's sources), on every lane of the strip
+    int ridx = -1;
+    if (recs && r >= 0 && r < d.Z && io.rmap[r] > 0 && in_col) ridx = io.rmap[d.Z + r * d.pitch + c];
+    float ga1b = 0.0f, ga2b = 0.0f, ga1a = 0.0f, ga2a = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SPB; ++j) {
+      if (j < ns) {
+        // ================= step B (t+1), as ac_adj_step at row k
+        lw0[j] = lw1[j]; lw1[j] = lw2[j]; lw2[j] = lw3[j]; lw3[j] = lwfix(so[j], k + 1, cW[j]);
+        lu0[j] = lu1[j]; lu1[j] = lu2[j]; lu2[j] = cU[j];
+        g0[j] = g1[j]; g1[j] = g2[j]; g2[j] = g3[j];
+        l2a[j] = l2b[j]; l2b[j] = l2c[j];
+        if (A2) { pp0[j] = pp1[j]; pp1[j] = pp2[j]; pp2[j] = pp3[j]; pp3[j] = cIp[j]; }
+        const float fu = a2k * lu2[j];
+        const float fum1 = shfl(fu, lane - 1), fum2 = shfl(fu, lane - 2), fup1 = shfl(fu, lane + 1);
+        const float dxt = C1 * (fum1 - fu) + C2 * (fum2 - fup1);
+        const float dzt = C1 * (aw1 * lw1[j] - aw2 * lw2[j]) + C2 * (aw0 * lw0[j] - aw3 * lw3[j]);
+        const float l3 = cP[j] - (dxt + dzt);
+        float l2 = l3;
+        if (d.fsurf && k == fs + 1) { l2 = l3 - l2a[j]; l2a[j] = 0.0f; }
+        l2c[j] = l2;
+        if (io.grad_src && own_k && k == zs[j] && c == xs[j]) io.grad_src[(long long)(s0 + j) * d.nt + t + 1] = l2;
+        g3[j] = a1k * l2;
+        if (want_a1 && own_k && rowp) ga1b -= l2 * cI[j];
+        // outputs of B at row r: the adjoint state after step t (plus step t's sources)
+        const float gc = g1[j];
+        const float gl1 = shfl(gc, lane - 1), gr1 = shfl(gc, lane + 1), gr2 = shfl(gc, lane + 2);
+        float op = k1r * l2a[j];
+        float ou = k2r * lu0[j] - (C1 * (gc - gr1) + C2 * (gl1 - gr2));
+        float ow = k3r * lw0[j] - (C1 * (gc - g2[j]) + C2 * (g0[j] - g3[j]));
+        if (A2) {
+          const float pc = pp1[j];
+          const float pl = shfl(pc, lane - 1), pr1 = shfl(pc, lane + 1), pr2 = shfl(pc, lane + 2);
+          if (own_r) {
+            if (um) ga2b -= lu0[j] * (C1 * (pr1 - pc) + C2 * (pr2 - pl));
+            if (wm) ga2b -= lw0[j] * (C1 * (pp2[j] - pc) + C2 * (pp3[j] - pp0[j]));
+          }
+        }
+        if (c >= d.X - 1) ou = 0.0f;          // rows/columns the one-step kernel never writes
+        if (r >= d.Z - 1) ow = 0.0f;
+        if (ridx >= 0) {
+          const long long gi = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
+          if (io.g_p) op = op + io.g_p[gi];
+          if (io.g_u) ou = ou + io.g_u[gi];
+          if (io.g_w) ow = ow + io.g_w[gi];
+        }
+        // ================= step A (t) at row q = k-3 over B's rows
+        const float wprev = eW[j];
+        mw0[j] = mw1[j]; mw1[j] = mw2[j]; mw2[j] = mw3[j];
+        mw3[j] = (q + 1 == fsr) ? ow + wprev : ((q + 1 == fsr - 1) ? 0.0f : ow);
+        mu0[j] = mu1[j]; mu1[j] = mu2[j]; mu2[j] = eU[j];
+        h0[j] = h1[j]; h1[j] = h2[j]; h2[j] = h3[j];
+        m2a[j] = m2b[j]; m2b[j] = m2c[j];
+        if (A2) { qp0[j] = qp1[j]; qp1[j] = qp2[j]; qp2[j] = qp3[j]; qp3[j] = dIp[j]; }
+        const float fv = a2q * mu2[j];
+        const float fvm1 = shfl(fv, lane - 1), fvm2 = shfl(fv, lane - 2), fvp1 = shfl(fv, lane + 1);
+        const float ext = C1 * (fvm1 - fv) + C2 * (fvm2 - fvp1);
+        const float ezt = C1 * (bw1 * mw1[j] - bw2 * mw2[j]) + C2 * (bw0 * mw0[j] - bw3 * mw3[j]);
+        const float m3 = eP[j] - (ext + ezt);
+        float m2 = m3;
+        if (d.fsurf && q == fs + 1) { m2 = m3 - m2a[j]; m2a[j] = 0.0f; }
+        m2c[j] = m2;
+        if (io.grad_src && own_q && q == zs[j] && c == xs[j]) io.grad_src[(long long)(s0 + j) * d.nt + t] = m2;
+        h3[j] = a1q * m2;
+        if (want_a1 && own_q && rowp_q) ga1a -= m2 * dI[j];
+        {
+          const float hc = h1[j];
+          const float hl1 = shfl(hc, lane - 1), hr1 = shfl(hc, lane + 1), hr2 = shfl(hc, lane + 2);
+          const float opa = k1q * m2a[j];
+          const float oua = k2q * mu0[j] - (C1 * (hc - hr1) + C2 * (hl1 - hr2));
+          const float owa = k3q * mw0[j] - (C1 * (hc - h2[j]) + C2 * (h0[j] - h3[j]));
+          if (A2) {
+            const float pc = qp1[j];
+            const float pl = shfl(pc, lane - 1), pr1 = shfl(pc, lane + 1), pr2 = shfl(pc, lane + 2);
+            if (own_qq) {
+              if (um_q) ga2a -= mu0[j] * (C1 * (pr1 - pc) + C2 * (pr2 - pl));
+              if (wm_q) ga2a -= mw0[j] * (C1 * (qp2[j] - pc) + C2 * (qp3[j] - qp0[j]));
+            }
+          }
+          if (own_qq) {
+            bst(opa, oP, vs, so[j] + rqq);
+            if (c < d.X - 1) bst(oua, oU, vs, so[j] + rqq);
+            if (qq < d.Z - 1) bst(owa, oW, vs, so[j] + rqq);
+          }
+        }
+        eP[j] = op; eU[j] = ou; eW[j] = ow;
+      }
+      cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j]; cI[j] = nI[j]; cIp[j] = nIp[j];
+      dI[j] = mI[j]; dIp[j] = mIp[j];
+    }
+    cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
+    dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
+    // image terms: B's row k waits three iterations for A's row q, B's row r for A's row qq
+    if (want_a1 && own_q && rowp_q) bst(bld(rG1, vs, gb + rq) + (ga1d0 + ga1a), rG1, vs, gb + rq);
+    if (A2 && own_qq) bst(bld(rG2, vs, gb + rqq) + (ga2d0 + ga2a), rG2, vs, gb + rqq);
+    ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
+    ga2d0 = ga2d1; ga2d1 = ga2d2; ga2d2 = ga2b;
+  }
+}
+
 // rec_{p,u,w}[s][t][r] = state at the receivers after step t
 __global__ void ac_gather(AcDev d, const float *__restrict__ st, float *__restrict__ rp, float *__restrict__ ru,
                           float *__restrict__ rw, int t) {
@@ -735,14 +1005,21 @@ int layout_of(const adfwi_acoustic_desc *d, adfwi_layout *L) {
   L->shots_per_block = spb;
   L->ngroups = (d->nshots + spb - 1) / spb;
   L->nfields = 3;
-  L->reserved = 0;
+  // the two-step adjoint holds twice the windows of the one-step kernel: 2 shots per wavefront
+  // keep it at 3 waves per SIMD where 4 shots would leave 2
+  int aspb = d->adj_shots_per_block;
+  ADFWI_CHECK_ARG(aspb == 0 || aspb == 1 || aspb == 2 || aspb == 4,
+                  "adj_shots_per_block must be 0, 1, 2 or 4 (got %d)", aspb);
+  if (aspb == 0) aspb = spb < 2 ? spb : 2;
+  L->reserved = aspb;
   L->field_floats = (int64_t)d->nshots * d->nz * d->pitch;
   // buffer descriptors address one field (all shots) with 32-bit byte offsets
   ADFWI_CHECK_ARG(L->field_floats * 4 < ((int64_t)1 << 31),
                   "one field of %d shots x %d x %d exceeds 2 GiB; advance the shots in smaller launches",
                   d->nshots, d->nz, d->pitch);
   L->state_floats = 3 * L->field_floats;
-  L->accum_floats = (int64_t)L->ngroups * d->nz * d->pitch;
+  const int finer = spb < aspb ? spb : aspb;
+  L->accum_floats = (int64_t)((d->nshots + finer - 1) / finer) * d->nz * d->pitch;
   return ADFWI_OK;
 }
 
@@ -800,11 +1077,26 @@ AcDev pair_dev(const AcDev &v) {
 
 void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
   switch (w.spb) {
-    case 8:
     case 4: hipLaunchKernelGGL((ac_fwd_pair<4>), g, b, 0, st, w, io); break;
     case 2: hipLaunchKernelGGL((ac_fwd_pair<2>), g, b, 0, st, w, io); break;
     default: hipLaunchKernelGGL((ac_fwd_pair<1>), g, b, 0, st, w, io);
   }
+}
+
+template <bool A2>
+void launch_adj_pair_a(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
+  switch (w.spb) {
+    case 4: hipLaunchKernelGGL((ac_adj_pair<4, A2>), g, b, 0, st, w, io); break;
+    case 2: hipLaunchKernelGGL((ac_adj_pair<2, A2>), g, b, 0, st, w, io); break;
+    default: hipLaunchKernelGGL((ac_adj_pair<1, A2>), g, b, 0, st, w, io);
+  }
+}
+
+void launch_adj_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
+  if (io.grad_a2 && io.img_p0 && io.img_p1)
+    launch_adj_pair_a<true>(w, g, b, st, io);
+  else
+    launch_adj_pair_a<false>(w, g, b, st, io);
 }
 
 template <int SPB>
@@ -857,7 +1149,7 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
   // two steps per launch where the first step's state is not needed in HBM (it is not a
   // checkpoint) and its receiver samples can be written through the receiver-index map
-  const bool fuse = a->fuse_steps != 0 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
+  const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
   const AcDev w = pair_dev(v);
   const dim3 pgrid((w.nstrips + WPB - 1) / WPB, L.ngroups);
   auto ckpt_slot = [&](int t) -> float * {   // checkpoint buffer receiving the state after step t
@@ -925,20 +1217,42 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   const dim3 grid = step_grid(L), block(NT);
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  // two steps per launch: the later one (t) reversed first, the receiver sources of t-1 added
+  // inside the kernel through the receiver-index map
+  const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
+  AcDev w = pair_dev(v);
+  w.spb = L.reserved;
+  const dim3 pgrid((w.nstrips + WPB - 1) / WPB, (d->nshots + w.spb - 1) / w.spb);
+  auto slot = [&](int t) -> long long { return (long long)(t - a->image_t0) * L.field_floats; };
   float *cur = const_cast<float *>(a->lam_in);
-  for (int k = 0; k < a->nsteps; ++k) {
+  for (int k = 0; k < a->nsteps;) {
     const int t = a->t_hi - k;
+    const bool pair = fuse && k + 1 < a->nsteps;
     if (recs && ng > 0) {
       hipLaunchKernelGGL(ac_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec_p, a->grad_rec_u, a->grad_rec_w, t);
       if ((rc = adfwi::check_launch("ac_inject"))) return rc;
     }
     float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
-    const long long slot = (long long)(t - a->image_t0) * L.field_floats;
-    const float *imd = a->image ? a->image + slot : nullptr;
-    const float *imp = a->image_p ? a->image_p + slot : nullptr;
-launch_adj(v, grid, block, st, cur, nxt, t, imd, imp, a->grad_a1, a->grad_a2, a->grad_src);
-    if ((rc = adfwi::check_launch("ac_adj_step"))) return rc;
+    if (pair) {
+      AdjPairIO io;
+      io.in = cur; io.out = nxt; io.t = t - 1;
+      io.img_d0 = a->image ? a->image + slot(t - 1) : nullptr;
+      io.img_d1 = a->image ? a->image + slot(t) : nullptr;
+      io.img_p0 = a->image_p ? a->image_p + slot(t - 1) : nullptr;
+      io.img_p1 = a->image_p ? a->image_p + slot(t) : nullptr;
+      io.grad_a1 = a->grad_a1; io.grad_a2 = a->grad_a2; io.grad_src = a->grad_src;
+      io.rmap = recs ? a->rcv_map : nullptr;
+      io.g_p = a->grad_rec_p; io.g_u = a->grad_rec_u; io.g_w = a->grad_rec_w;
+      launch_adj_pair(w, pgrid, block, st, io);
+      if ((rc = adfwi::check_launch("ac_adj_pair"))) return rc;
+    } else {
+      const float *imd = a->image ? a->image + slot(t) : nullptr;
+      const float *imp = a->image_p ? a->image_p + slot(t) : nullptr;
+      launch_adj(v, grid, block, st, cur, nxt, t, imd, imp, a->grad_a1, a->grad_a2, a->grad_src);
+      if ((rc = adfwi::check_launch("ac_adj_step"))) return rc;
+    }
     cur = nxt;
+    k += pair ? 2 : 1;
   }
   a->lam_out = cur;
   return ADFWI_OK;

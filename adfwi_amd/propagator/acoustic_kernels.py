@@ -137,6 +137,7 @@ class AcousticGeometry:
         if not shots_per_block:
             shots_per_block = int(os.environ.get("ADFWI_SHOTS_PER_BLOCK", "0"))
         self.shots_per_block = int(shots_per_block)
+        self.adj_shots_per_block = int(os.environ.get("ADFWI_ADJ_SHOTS_PER_BLOCK", "0"))
         self.row_unroll = int(os.environ.get("ADFWI_ROW_UNROLL", "0"))
         self.tile_z = int(os.environ.get("ADFWI_TILE_Z", "0"))
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
@@ -171,6 +172,7 @@ class AcousticGeometry:
         d.nshots, d.nz, d.nx, d.pitch = self.S, self.Z, self.X, self.pitch
         d.fs, d.free_surface, d.nt, d.nrec = self.fs, int(self.free_surface), self.nt, self.R
         d.shots_per_block = self.shots_per_block
+        d.adj_shots_per_block = self.adj_shots_per_block
         d.row_unroll = self.row_unroll
         d.tile_z = self.tile_z
         d.coef = None if coef is None else coef.data_ptr()
@@ -260,8 +262,9 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
     lw = geom.state(2)    # adjoint ping-pong
     image = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
     image_p = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
-    ga1 = torch.zeros((L.ngroups, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
-    ga2 = torch.zeros((L.ngroups, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
+    nacc = int(L.accum_floats) // (Z * geom.pitch)   # accumulator planes (finest shot grouping)
+    ga1 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
+    ga2 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
     gsrc = torch.zeros((S, nt), device=dev, dtype=torch.float32) if need_src else None
     g = [None if x is None else x.contiguous().to(torch.float32) for x in g_rec]
     gp = [None if x is None else x.data_ptr() for x in g]
@@ -302,8 +305,12 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
         aa.grad_a1 = None if ga1 is None else ga1.data_ptr()
         aa.grad_a2 = None if ga2 is None else ga2.data_ptr()
         aa.grad_src = None if gsrc is None else gsrc.data_ptr()
-        TIMELINE.call("adjoint", "ac_adj_step", n, dev, lambda: nat.check(
-            lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream), "adfwi_acoustic_adjoint"))
+        aa.rcv_map = None if geom.rcv_map is None else geom.rcv_map.data_ptr()
+        aa.fuse_steps = geom.fuse_steps
+        fused = bool(geom.fuse_steps) and (all(x is None for x in g) or geom.rcv_map is not None)
+        TIMELINE.call("adjoint", "ac_adj_pair" if fused else "ac_adj_step", (n + 1) // 2 if fused else n, dev,
+                      lambda: nat.check(lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream),
+                                        "adfwi_acoustic_adjoint"), steps=n)
         lam = lw[0] if aa.lam_out == lw[0].data_ptr() else lw[1]
     out_a1 = ga1.sum(0)[:, :X] if ga1 is not None else None
     out_a2 = ga2.sum(0)[:, :X] if ga2 is not None else None

@@ -333,7 +333,7 @@ def main():
         steps_by = d["steps_by_call"]
         if name in ("ac_fwd_step", "ac_fwd_pair"):
             byts = (steps_by.get("forward", 0) * BYTES_FWD + steps_by.get("recompute", 0) * BYTES_RECOMPUTE) * N * S
-        elif name == "ac_adj_step":
+        elif name in ("ac_adj_step", "ac_adj_pair"):
             byts = d["steps"] * BYTES_ADJ * N * S
         elif name == "el_fwd_step":
             byts = d["steps"] * EL_BYTES_FWD * N * S

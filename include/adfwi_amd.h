@@ -64,7 +64,8 @@ typedef struct adfwi_acoustic_desc {
   int32_t shots_per_block; /* shots advanced by one wavefront: 1, 2, 4, 8; 0 = library choice */
   int32_t row_unroll;      /* row-loop unroll of the step kernels: 1, 2, 4; 0 = library choice */
   int32_t tile_z;          /* rows of one strip: 4..64; 0 = library choice */
-  int32_t reserved;
+  int32_t adj_shots_per_block; /* shots per wavefront of the two-step adjoint kernel: 1, 2, 4;
+                                  0 = library choice */
   const float *coef;       /* [5][nz][pitch]: 1-K1, A1, 1-K2, A2, 1-K3 */
   const float *src;        /* [nshots][nt]: dt * wavelet, injected into p */
   const int32_t *src_z;    /* [nshots] padded source rows */
@@ -80,10 +81,11 @@ typedef struct adfwi_layout {
   int32_t shots_per_block; /* effective value */
   int32_t ngroups;         /* shot groups = ceil(nshots / shots_per_block) */
   int32_t nfields;         /* fields in one state set */
-  int32_t reserved;
+  int32_t reserved;        /* acoustic: shots per wavefront of the two-step adjoint kernel */
   int64_t field_floats;    /* nshots * nz * pitch */
   int64_t state_floats;    /* nfields * field_floats */
-  int64_t accum_floats;    /* ngroups * nz * pitch: one per-group accumulator plane set */
+  int64_t accum_floats;    /* acoustic: one accumulator plane (nz * pitch) per shot group of the
+                              finer of the two groupings (shots_per_block, reserved) */
 } adfwi_layout;
 
 /* Tiling and buffer sizes for a descriptor (host only, no device access). */
@@ -127,6 +129,10 @@ typedef struct adfwi_acoustic_adj_args {
   float *grad_a1, *grad_a2; /* [ngroups][Z][pitch] (+=) or NULL */
   float *grad_src;          /* [S][nt] or NULL: d loss / d src */
   const float *lam_out;     /* OUT: adjoint state after step t_hi-nsteps (its sources not added) */
+  const int32_t *rcv_map;   /* as in adfwi_acoustic_fwd_args: lets two steps share one launch
+                               while record adjoints are injected */
+  int32_t fuse_steps;       /* 1: reverse two steps per launch where possible; 0: one */
+  int32_t reserved2;
 } adfwi_acoustic_adj_args;
 
 /* Discrete adjoint of adfwi_acoustic_forward (the transpose of every stage of

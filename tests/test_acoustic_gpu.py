@@ -187,9 +187,10 @@ def test_propagator_class_end_to_end():
 @pytest.mark.parametrize("free_surface", [False, True])
 @pytest.mark.parametrize("nt,spb,tz", [(121, 4, 0), (150, 2, 8), (97, 1, 4)])
 def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surface, nt, spb, tz):
-    """The fused two-step kernel (intermediate state in registers, its receivers written in the
-    kernel) reproduces the one-step kernel: records, checkpoints, final state and gradients bit
-    for bit, energy maps to rounding (two steps' terms are added before the read-modify-write)."""
+    """The fused two-step kernels (intermediate state in registers, its receivers written /
+    its receiver sources injected in the kernel) reproduce the one-step kernels: records and the
+    wavelet gradient bit for bit; energy maps and the A1/A2 gradients to rounding (two steps'
+    terms are added before one read-modify-write of the accumulator planes)."""
     from adfwi_amd.propagator import acoustic_kernels as ak
     from adfwi_amd.propagator.boundary_condition import bc_pml
     rng = np.random.default_rng(nt + spb)
@@ -227,4 +228,5 @@ def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surfac
         assert torch.equal(r0[i], r1[i]), ("record", i, rel_l2(r1[i].numpy(), r0[i].numpy()))
     for i in range(3, 6):
         assert rel_l2(r1[i].numpy(), r0[i].numpy()) <= 1e-6, ("energy", i)
-    assert torch.equal(ga0, ga1) and torch.equal(gb0, gb1) and torch.equal(gw0, gw1)
+    assert torch.equal(gw0, gw1)
+    assert rel_l2(ga1.numpy(), ga0.numpy()) <= 1e-6 and rel_l2(gb1.numpy(), gb0.numpy()) <= 1e-6
