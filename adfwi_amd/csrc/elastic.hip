@@ -612,13 +612,21 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       cur = nxt;
     }
   }
-  if (!any || !out_col) return;
+  // the four wavefronts of the workgroup (four shot sets of the same strip) add their slabs in
+  // LDS, then one read-modify-write of the workgroup's plane set: wavefront w takes the fields
+  // f = w, w+4
+  (void)any;
+  __syncthreads();
+  if (!out_col) return;
   const long long planef = (long long)d.Z * d.pitch;
-  float *g = gacc + (long long)grp * NCOEF * planef + c;
-  for (int row = z0; row < end_row; ++row)
-    for (int f = 0; f < NGF; ++f) {
+  float *g = gacc + (long long)blockIdx.y * NCOEF * planef + c;
+  const float *sb = slab + lane;
+  for (int f = wv; f < NGF; f += EWPB)
+    for (int row = z0; row < end_row; ++row) {
+      const int o = (f * ETZA + (row - z0)) * WAVE;
+      const float v = (sb[o] + sb[NGF * ETZA * WAVE + o]) + (sb[2 * NGF * ETZA * WAVE + o] + sb[3 * NGF * ETZA * WAVE + o]);
       float *p = g + (long long)f * planef + (long long)row * d.pitch;
-      *p = *p + sl[(f * ETZA + (row - z0)) * WAVE];
+      *p = *p + v;
     }
 }
 
@@ -690,7 +698,7 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
                   "state of %d shots x %d x %d exceeds the 4 GiB of one buffer descriptor; "
                   "advance the shots in smaller launches", d->nshots, d->nz, d->pitch);
   L->state_floats = NFIELD * L->field_floats;
-  L->accum_floats = (int64_t)L->ngroups * EWPB * NCOEF * d->nz * d->pitch;
+  L->accum_floats = (int64_t)L->ngroups * NCOEF * d->nz * d->pitch;
   return ADFWI_OK;
 }
 

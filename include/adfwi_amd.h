@@ -172,7 +172,7 @@ typedef struct adfwi_elastic_desc {
 /* Tiling and buffer sizes; layout.nfields = 10 (split fields), layout.reserved = shots per
  * wavefront, layout.shots_per_block = shots per workgroup, layout.ngroups = workgroups along
  * the shots, layout.accum_floats = size of the gradient accumulator of the adjoint,
- * [ngroups * 4][8][nz][pitch] (one group of planes per wavefront's shot set). */
+ * [ngroups][8][nz][pitch] (one set of planes per workgroup's shots). */
 int adfwi_elastic_get_layout(const adfwi_elastic_desc *d, adfwi_layout *out);
 
 typedef struct adfwi_elastic_fwd_args {
@@ -201,7 +201,7 @@ typedef struct adfwi_elastic_adj_args {
   const float *trace;      /* forward states after steps t_hi-nsteps .. t_hi-1 [nsteps][10][S][Z][pitch] */
   const float *grad_rec[5];/* dL/d(record) [S][nt][R] for txx, tzz, txz, vx, vz; NULL = zero */
   float *grad_coef;        /* accumulated: layout.accum_floats, planes C11, C13, C33, C55, bx, bz,
-                              C15, C35 of every wavefront group (sum over the groups) */
+                              C15, C35 of every workgroup's shots (sum over the groups) */
   float *grad_src;         /* [S][nt] or NULL: dL/d(wavelet) written for the reversed steps */
   const float *lam_out;    /* OUT: adjoint of the state before step t_hi-nsteps */
 } adfwi_elastic_adj_args;
