@@ -62,9 +62,9 @@ class ElasticDesc(ctypes.Structure):
     _fields_ = [("abi_version", _i32), ("struct_size", _i32), ("nshots", _i32), ("nz", _i32), ("nx", _i32),
                 ("pitch", _i32), ("free_surface", _i32), ("nt", _i32), ("nrec", _i32), ("aniso", _i32),
                 ("shots_per_wave", _i32), ("tile_z", _i32), ("dt", _f32), ("dt_dx", _f32), ("dt_dz", _f32),
-                ("dx", _f32), ("dz", _f32), ("c0", _f32), ("c1", _f32), ("reserved", _i32),
+                ("dx", _f32), ("dz", _f32), ("c0", _f32), ("c1", _f32), ("damping", _i32),
                 ("coef", _vp), ("pml_x", _vp), ("pml_z", _vp), ("src", _vp), ("src_m", _vp),
-                ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp)]
+                ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp), ("damp", _vp)]
 
 
 class ElasticFwdArgs(ctypes.Structure):

@@ -56,6 +56,7 @@ __device__ __forceinline__ void bst(float v, rsrc_t r, unsigned voff, unsigned s
 __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, WAVE); }
 
 enum { TXX_X = 0, TXX_Z, TZZ_X, TZZ_Z, TXZ_X, TXZ_Z, VX_X, VX_Z, VZ_X, VZ_Z, NFIELD };
+enum { TXX = 0, TZZ, TXZ, VX, VZ, NFIELD5 };   // state of the damping (non-PML) scheme
 enum { C11 = 0, C13, C33, C55, BX, BZ, C15, C35, NCOEF };
 
 struct ElDev {
@@ -63,7 +64,7 @@ struct ElDev {
   int tz, nsx, nstrips;
   float dt, dt_dx, dt_dz, dx, dz, c0, c1;
   long long plane, field;
-  const float *coef, *pml_x, *pml_z, *src, *src_m;
+  const float *coef, *pml_x, *pml_z, *src, *src_m, *damp;
   const int *src_z, *src_x, *rcv_z, *rcv_x;
 };
 
@@ -652,6 +653,493 @@ __global__ void el_inject(ElDev d, float *__restrict__ lam, const float *__restr
 }
 
 // ---------------------------------------------------------------------------
+// damping scheme (reference: step_forward_ABL_4order, elastic_kernels.py:585-777)
+// ---------------------------------------------------------------------------
+// State [5][S][Z][pitch] = txx, tzz, txz, vx, vz.  One step, on the window rows [2, Z-3] x cols
+// [2, X-3]:
+//   1. s += dt ((C.Dx v)/dx + (C.Dz v)/dz) for txx, tzz, txz (Dxb/Dzb for the normal stresses,
+//      Dxf/Dzf for the shear stress), then the source -MT/3 w(t);
+//   2. free surface, kept in the state: tzz[2] = 0, tzz[1] = -tzz[3], txz[1] = -txz[2],
+//      txz[0] = -txz[3];
+//   3. vx += (dt bx) ((Dxf txx)/dx + (Dzb txz)/dz), vz += (dt bz) ((Dxb txz)/dx + (Dzf tzz)/dz);
+//   4. free surface on the columns J = [2, X-3], from the new rows 2 and 3: vz[1] = vz[2],
+//      vx[1] = vz[1,j+1] - vz[1,j] + vz[2,j+1] - vz[2,j] + vx[3,j], vz[0] = vz[1];
+//   5. vx, vz *= damp everywhere (points that never change stay zero).
+// The strip march is that of el_fwd_step: velocity rows k-2..k+2 and the new stress rows in
+// registers; the first strip also writes the free-surface rows 0 and 1.
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el_abl_fwd(ElDev d, const float *__restrict__ in, float *__restrict__ out,
+                                                  int t) {
+  int z0, x0;
+  if (!strip_of(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - EHL + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= EHL && lane < EHL + ESX && c <= d.X - 3;
+  const bool cwin = c >= 2 && c <= d.X - 3;
+  const int zmax = d.Z - 1, zw1 = d.Z - 3;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rIn = rsrc(in, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rOut = rsrc(out, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const rsrc_t rD = rsrc(d.damp, planeb);
+  const float c0 = d.c0, c1 = d.c1, dt = d.dt, dx = d.dx, dz = d.dz;
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const bool jj = c >= 2 && c <= d.X - 3;
+  const bool jj1 = c + 1 >= 2 && c + 1 <= d.X - 3;
+  const bool first = d.fsurf && z0 == 2;         // the strip that writes the free-surface rows
+
+  const int s_first = (blockIdx.y * EWPB + wv) * d.spw;
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = s_first + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float wsrc = d.src[(long long)s * d.nt + t];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;
+    float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
+    float xz0 = 0.f, xz1 = 0.f, xz2 = 0.f, xz3 = 0.f, xz4 = 0.f;
+    float zz0 = 0.f, zz1 = 0.f, zz2 = 0.f, zz3 = 0.f, zz4 = 0.f;
+    float xx0 = 0.f, xx1 = 0.f, xx2 = 0.f;
+    float nvz2 = 0.f;                            // new vz of row 2 before damping (free surface)
+    for (int rr = z0 - 4; rr <= z0 - 1; ++rr) {
+      const unsigned o = rb(rr);
+      const bool ok = rr >= 0;
+      const float a = bld(rIn, vc, F(VX) + o), b = bld(rIn, vc, F(VZ) + o);
+      vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
+      vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
+    }
+    struct Row {
+      float v[2], s[3], cm[ANISO ? 6 : 4], b[2], dm;
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o2 = rb(k + 2), o0 = rb(k), om = rb(k - 2);
+      R.v[0] = bld(rIn, vc, F(VX) + o2); R.v[1] = bld(rIn, vc, F(VZ) + o2);
+      R.s[0] = bld(rIn, vc, F(TXX) + o0); R.s[1] = bld(rIn, vc, F(TZZ) + o0); R.s[2] = bld(rIn, vc, F(TXZ) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+      R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
+      R.dm = bld(rD, vc, om);
+    };
+    Row cur;
+    load(z0 - 2, cur);
+#pragma unroll 1
+    for (int kk = 0; kk < d.tz + 4; ++kk) {
+      const int k = z0 - 2 + kk;
+      Row nxt;
+      load(k + 1, nxt);
+      {
+        const bool ok = k + 2 <= zmax;
+        vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? cur.v[0] : 0.f;
+        vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? cur.v[1] : 0.f;
+      }
+      // 1. stress of row k
+      const unsigned ok_ = rb(k);
+      float sxx = cur.s[0], szz = cur.s[1], sxz = cur.s[2];
+      const float vxm1 = shfl(vx2, lane - 1), vxm2 = shfl(vx2, lane - 2), vxp1 = shfl(vx2, lane + 1);
+      const float vzm1 = shfl(vz2, lane - 1), vzp1 = shfl(vz2, lane + 1), vzp2 = shfl(vz2, lane + 2);
+      float vzm2 = 0.0f, vxp2 = 0.0f;
+      if constexpr (ANISO) {
+        vzm2 = shfl(vz2, lane - 2);
+        vxp2 = shfl(vx2, lane + 2);
+      }
+      if (k >= 2 && k <= zw1 && cwin) {
+        const float dxb_vx = c0 * (vx2 - vxm1) + c1 * (vxp1 - vxm2);
+        const float dzb_vz = c0 * (vz2 - vz1) + c1 * (vz3 - vz0);
+        const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
+        const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
+        if constexpr (ANISO) {
+          const float dxb_vz = c0 * (vz2 - vzm1) + c1 * (vzp1 - vzm2);
+          const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
+          const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
+          const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
+          sxx = sxx + dt * ((C11v * dxb_vx + C15v * dxb_vz) / dx + (C15v * dzb_vx + C13v * dzb_vz) / dz);
+          szz = szz + dt * ((C13v * dxb_vx + C35v * dxb_vz) / dx + (C35v * dzb_vx + C33v * dzb_vz) / dz);
+          sxz = sxz + dt * ((C15v * dxf_vx + C55v * dxf_vz) / dx + (C55v * dzf_vx + C35v * dzf_vz) / dz);
+        } else {
+          sxx = sxx + dt * ((C11v * dxb_vx) / dx + (C13v * dzb_vz) / dz);
+          szz = szz + dt * ((C13v * dxb_vx) / dx + (C33v * dzb_vz) / dz);
+          sxz = sxz + dt * ((C55v * dxf_vz) / dx + (C55v * dzf_vx) / dz);
+        }
+      }
+      if (k == zs && c == xs) {
+        sxx = sxx + mxx * wsrc;
+        szz = szz + mzz * wsrc;
+        sxz = sxz + mxz * wsrc;
+      }
+      if (d.fsurf && k == 2) szz = 0.0f;        // 2. tzz[2] = 0 (the value before it is not used)
+      const bool own_k = out_col && k >= z0 && k < end_row;
+      if (own_k) {
+        bst(sxx, rOut, vs, F(TXX) + ok_); bst(szz, rOut, vs, F(TZZ) + ok_); bst(sxz, rOut, vs, F(TXZ) + ok_);
+      }
+      xx0 = xx1; xx1 = xx2; xx2 = sxx;
+      xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = sxz;
+      zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = szz;
+      if (d.fsurf && k == 3) {                   // rows -1..3 = *0..*4: the rows 0 and 1
+        zz2 = -zz4;
+        xz2 = -xz3;
+        xz1 = -xz4;
+        if (out_col) {
+          bst(zz2, rOut, vs, F(TZZ) + rb(1));
+          bst(xz2, rOut, vs, F(TXZ) + rb(1));
+          bst(xz1, rOut, vs, F(TXZ) + rb(0));
+        }
+      }
+      // 3. velocity of row r = k-2 (old values: the window rows vx0, vz0)
+      const int r = k - 2;
+      const float txp1 = shfl(xx0, lane + 1), txp2 = shfl(xx0, lane + 2), txm1 = shfl(xx0, lane - 1);
+      const float tzm1 = shfl(xz2, lane - 1), tzm2 = shfl(xz2, lane - 2), tzp1 = shfl(xz2, lane + 1);
+      const bool own_r = out_col && r >= z0 && r < end_row;
+      float nvx = 0.0f, nvz = 0.0f;
+      if (r >= 2 && r <= zw1 && cwin) {
+        const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
+        const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
+        const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
+        const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
+        nvx = vx0 + (dt * cur.b[0]) * (dxf_txx / dx + dzb_txz / dz);
+        nvz = vz0 + (dt * cur.b[1]) * (dxb_txz / dx + dzf_tzz / dz);
+      }
+      if (own_r) {
+        const unsigned orr = rb(r);
+        bst(nvx * cur.dm, rOut, vs, F(VX) + orr);
+        bst(nvz * cur.dm, rOut, vs, F(VZ) + orr);
+      }
+      // 4. free-surface velocity rows from the new rows 2 and 3 (every lane computes its column)
+      if (first && r == 2) nvz2 = nvz;
+      if (first && r == 3) {
+        const float vz2n = shfl(nvz2, lane + 1);
+        const float vz1m_n = jj1 ? vz2n : 0.0f;
+        const float vz1m = jj ? nvz2 : 0.0f;
+        if (out_col) {
+          const float d1 = bld(rD, vc, rb(1)), d0 = bld(rD, vc, rb(0));
+          const float nvx1 = (((vz1m_n - vz1m) + vz2n) - nvz2) + nvx;
+          bst(nvx1 * d1, rOut, vs, F(VX) + rb(1));
+          bst(vz1m * d1, rOut, vs, F(VZ) + rb(1));
+          bst(vz1m * d0, rOut, vs, F(VZ) + rb(0));
+        }
+      }
+      cur = nxt;
+    }
+  }
+}
+
+// five records of step t from the state after step t (damping scheme: stored totals)
+__global__ void el_gather5(ElDev d, const float *__restrict__ st, float *__restrict__ r_txx,
+                           float *__restrict__ r_tzz, float *__restrict__ r_txz, float *__restrict__ r_vx,
+                           float *__restrict__ r_vz, int t) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)d.S * d.nrec) return;
+  const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
+  const long long o = (long long)s * d.plane + (long long)d.rcv_z[r] * d.pitch + d.rcv_x[r];
+  const long long f = d.field;
+  const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
+  r_txx[ro] = st[TXX * f + o];
+  r_tzz[ro] = st[TZZ * f + o];
+  r_txz[ro] = st[TXZ * f + o];
+  r_vx[ro] = st[VX * f + o];
+  r_vz[ro] = st[VZ * f + o];
+}
+
+// record adjoints of step t into the five-field adjoint state (tzz on the free-surface row 2
+// is identically zero there, so it receives nothing)
+__global__ void el_inject5(ElDev d, float *__restrict__ lam, const float *__restrict__ g_txx,
+                           const float *__restrict__ g_tzz, const float *__restrict__ g_txz,
+                           const float *__restrict__ g_vx, const float *__restrict__ g_vz, int t) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)d.S * d.nrec) return;
+  const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
+  const int z = d.rcv_z[r];
+  const long long o = (long long)s * d.plane + (long long)z * d.pitch + d.rcv_x[r];
+  const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
+  const float *gs[5] = {g_txx, g_tzz, g_txz, g_vx, g_vz};
+  for (int q = 0; q < 5; ++q) {
+    if (!gs[q] || (q == TZZ && d.fsurf && z == 2)) continue;
+    atomicAdd(lam + q * d.field + o, gs[q][ro]);
+  }
+}
+
+// Adjoint of one damping-scheme step (the forward's stages in reverse), for the adjoint L of the
+// state after step t (record adjoints of step t already added by el_inject5):
+//   5'. L2(v) = damp * L(v) everywhere;
+//   4'. free surface of the velocities transposed into L1(v): on J, Lvz[2] += Lvz2[1] + Lvz2[0]
+//       - 2 Lvx2[1,j] + 2 Lvx2[1,j-1], Lvx[3] += Lvx2[1]; the overwritten rows 0, 1 get zero;
+//   3'. e = L1(v) dt b / d feeds the stress adjoints Lt (as el_adj_step), b gradients;
+//   2'. stress free surface transposed; the stored tzz[2] = 0 is a constant, so L1(tzz)[2] = 0;
+//   1'. L(s before) = L1(s); the velocity adjoints L(v before) = L1(v) + D^T(C dt/d L1(s));
+//       C gradients; the source gradient sum_c m_c L1(s_c)(source).
+// Rows and lanes march as in el_adj_step; the first strip also writes the velocity adjoints of
+// the free-surface rows 0 and 1 (those rows are state here).
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el_abl_adj(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
+                                                  const float *__restrict__ fv, const float *__restrict__ fsn, int t,
+                                                  float *__restrict__ gacc, float *__restrict__ gsrc) {
+  constexpr int NGF = ANISO ? 8 : 6;
+  __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  int z0, x0;
+  if (!strip_of(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int c = x0 - EHL + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const bool out_col = lane >= EHL && lane < EHL + ESX && c <= d.X - 3;
+  const bool cwin = c >= 2 && c <= d.X - 3;
+  const int zmax = d.Z - 1, zw1 = d.Z - 3;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rLi = rsrc(lin, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rLo = rsrc(lout, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rFv = rsrc(fv, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const rsrc_t rD = rsrc(d.damp, planeb);
+  const float c0 = d.c0, c1 = d.c1, dt = d.dt, dx = d.dx, dz = d.dz;
+  const float dtdx = d.dt_dx, dtdz = d.dt_dz;
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const bool jj = c >= 2 && c <= d.X - 3;
+  const bool jm1 = c - 1 >= 2 && c - 1 <= d.X - 3;
+  const bool first = d.fsurf && z0 == 2;
+  const int grp = blockIdx.y * EWPB + wv;
+  float *sl = slab + wv * (NGF * ETZA * WAVE) + lane;
+  for (int q = 0; q < NGF * ETZA; ++q) sl[q * WAVE] = 0.0f;
+  auto acc = [&](int f, int row, float v) { sl[(f * ETZA + (row - z0)) * WAVE] += v; };
+
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = grp * d.spw + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;     // forward velocities before the step
+    float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
+    float xz0 = 0.f, xz1 = 0.f, xz2 = 0.f, xz3 = 0.f, xz4 = 0.f;     // forward stresses after the step
+    float zz0 = 0.f, zz1 = 0.f, zz2 = 0.f, zz3 = 0.f, zz4 = 0.f;
+    float xx0 = 0.f, xx1 = 0.f, xx2 = 0.f;
+    float ux0 = 0.f, ux1 = 0.f, ux2 = 0.f, ux3 = 0.f, ux4 = 0.f;     // L1(vx) rows k-2..k+2
+    float uz0 = 0.f, uz1 = 0.f, uz2 = 0.f, uz3 = 0.f, uz4 = 0.f;     // L1(vz)
+    float e1a = 0.f, e1b = 0.f, e1c = 0.f, e3a = 0.f, e3b = 0.f, e3c = 0.f;
+    float e20 = 0.f, e21 = 0.f, e22 = 0.f, e23 = 0.f, e24 = 0.f;
+    float e40 = 0.f, e41 = 0.f, e42 = 0.f, e43 = 0.f, e44 = 0.f;
+    float p1a = 0.f, p1b = 0.f, p1c = 0.f, q3a = 0.f, q3b = 0.f, q3c = 0.f;
+    float p40 = 0.f, p41 = 0.f, p42 = 0.f, p43 = 0.f, p44 = 0.f;
+    float q20 = 0.f, q21 = 0.f, q22 = 0.f, q23 = 0.f, q24 = 0.f;
+    float p3a = 0.f, p3b = 0.f, p3c = 0.f, q1a = 0.f, q1b = 0.f, q1c = 0.f;
+    float p20 = 0.f, p21 = 0.f, p22 = 0.f, p23 = 0.f, p24 = 0.f;
+    float q40 = 0.f, q41 = 0.f, q42 = 0.f, q43 = 0.f, q44 = 0.f;
+    float ltzz1 = 0.f, ltxz0 = 0.f, ltxz1 = 0.f;     // stress adjoints of rows 0, 1 (free surface)
+    float wz0 = 0.f, wz1 = 0.f, wx1 = 0.f;           // damped velocity adjoints of rows 0, 1
+
+    // row rr of: forward velocities, L1(v), and e (zero outside the grid / window)
+    struct VRow {
+      float fv[2], lv[2], b[2], dm;
+    };
+    auto vload = [&](int rr, VRow &R) {
+      const unsigned o = rb(rr);
+      R.fv[0] = bld(rFv, vc, F(VX) + o); R.fv[1] = bld(rFv, vc, F(VZ) + o);
+      R.lv[0] = bld(rLi, vc, F(VX) + o); R.lv[1] = bld(rLi, vc, F(VZ) + o);
+      R.b[0] = bld(rC, vc, BX * planeb + o); R.b[1] = bld(rC, vc, BZ * planeb + o);
+      R.dm = bld(rD, vc, o);
+    };
+    auto vpush = [&](int rr, const VRow &R) {
+      const bool in_grid = rr >= 0 && rr <= zmax;
+      float lx = in_grid ? R.lv[0] * R.dm : 0.f, lz = in_grid ? R.lv[1] * R.dm : 0.f;   // 5'
+      if (d.fsurf) {                                                                     // 4'
+        if (rr == 0) { wz0 = lz; if (jj) lz = 0.f; }
+        if (rr == 1) { wz1 = lz; wx1 = lx; if (jj) { lz = 0.f; lx = 0.f; } }
+        if (rr == 2) {
+          const float wx1m = shfl(wx1, lane - 1);
+          if (jj) lz = lz + (((wz1 + wz0) - 2.0f * wx1) + (jm1 ? 2.0f * wx1m : 0.0f));
+        }
+        if (rr == 3 && jj) lx = lx + wx1;
+      }
+      float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
+      if (rr >= 2 && rr <= zw1 && cwin) {
+        ea = lx * ((dt * R.b[0]) / dx);
+        eb = lx * ((dt * R.b[0]) / dz);
+        ec = lz * ((dt * R.b[1]) / dx);
+        ed = lz * ((dt * R.b[1]) / dz);
+      }
+      vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = in_grid ? R.fv[0] : 0.f;
+      vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = in_grid ? R.fv[1] : 0.f;
+      ux0 = ux1; ux1 = ux2; ux2 = ux3; ux3 = ux4; ux4 = lx;
+      uz0 = uz1; uz1 = uz2; uz2 = uz3; uz3 = uz4; uz4 = lz;
+      e1a = e1b; e1b = e1c; e1c = ea;
+      e20 = e21; e21 = e22; e22 = e23; e23 = e24; e24 = eb;
+      e3a = e3b; e3b = e3c; e3c = ec;
+      e40 = e41; e41 = e42; e42 = e43; e43 = e44; e44 = ed;
+    };
+    for (int rr = z0 - 4; rr <= z0 - 1; ++rr) {
+      VRow R;
+      vload(rr, R);
+      vpush(rr, R);
+    }
+    struct SRow {
+      float ls[3], cm[ANISO ? 6 : 4], fs[3];
+    };
+    auto sload = [&](int k, SRow &R) {
+      const unsigned o = rb(k);
+      R.ls[0] = bld(rLi, vc, F(TXX) + o); R.ls[1] = bld(rLi, vc, F(TZZ) + o); R.ls[2] = bld(rLi, vc, F(TXZ) + o);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o); R.cm[1] = bld(rC, vc, C13 * planeb + o);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o); R.cm[3] = bld(rC, vc, C55 * planeb + o);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o); R.cm[5] = bld(rC, vc, C35 * planeb + o);
+      }
+      R.fs[0] = bld(rFs, vc, F(TXX) + o); R.fs[1] = bld(rFs, vc, F(TZZ) + o); R.fs[2] = bld(rFs, vc, F(TXZ) + o);
+    };
+    VRow vcur;
+    SRow scur;
+    vload(z0, vcur);
+    sload(z0 - 2, scur);
+#pragma unroll 1
+    for (int kk = 0; kk < d.tz + 4; ++kk) {
+      const int k = z0 - 2 + kk;
+      VRow vnxt;
+      SRow snxt;
+      vload(k + 3, vnxt);
+      sload(k + 1, snxt);
+      vpush(k + 2, vcur);
+      // 3'. stress adjoints of row k from e
+      const float e1m1 = shfl(e1a, lane - 1), e1m2 = shfl(e1a, lane - 2), e1p1 = shfl(e1a, lane + 1);
+      const float e3p1 = shfl(e3a, lane + 1), e3m1 = shfl(e3a, lane - 1), e3p2 = shfl(e3a, lane + 2);
+      float lt_xx = c0 * (e1m1 - e1a) + c1 * (e1m2 - e1p1);
+      float lt_xz = (c0 * (e22 - e23) + c1 * (e21 - e24)) + (c0 * (e3a - e3p1) + c1 * (e3m1 - e3p2));
+      float lt_zz = c0 * (e41 - e42) + c1 * (e40 - e43);
+      // 2'. stress free surface transposed
+      if (d.fsurf) {
+        if (k == 0) ltxz0 = lt_xz;
+        if (k == 1) { ltzz1 = lt_zz; ltxz1 = lt_xz; }
+        if (k == 2) lt_xz = lt_xz - ltxz1;
+        if (k == 3) { lt_zz = lt_zz - ltzz1; lt_xz = lt_xz - ltxz0; }
+      }
+      const float vxm1 = shfl(vx2, lane - 1), vxm2 = shfl(vx2, lane - 2), vxp1 = shfl(vx2, lane + 1);
+      const float vzm1 = shfl(vz2, lane - 1), vzp1 = shfl(vz2, lane + 1), vzp2 = shfl(vz2, lane + 2);
+      float vzm2 = 0.0f, vxp2 = 0.0f;
+      if constexpr (ANISO) {
+        vzm2 = shfl(vz2, lane - 2);
+        vxp2 = shfl(vx2, lane + 2);
+      }
+      // 1'. stress row k
+      float P1 = 0.f, Q2 = 0.f, Q3 = 0.f, P4 = 0.f, P2 = 0.f, P3 = 0.f, Q1 = 0.f, Q4 = 0.f;
+      if (k >= 2 && k <= zw1 && cwin) {
+        const unsigned o = rb(k);
+        const float lxx = scur.ls[0] + lt_xx;
+        const float lzz = (d.fsurf && k == 2) ? 0.0f : scur.ls[1] + lt_zz;
+        const float lxz = scur.ls[2] + lt_xz;
+        const bool own_k = out_col && k >= z0 && k < end_row;
+        if (own_k && k == zs && c == xs && gsrc)
+          gsrc[(long long)s * d.nt + t] = (mxx * lxx + mzz * lzz) + mxz * lxz;
+        if (own_k) {
+          bst(lxx, rLo, vs, F(TXX) + o); bst(lzz, rLo, vs, F(TZZ) + o); bst(lxz, rLo, vs, F(TXZ) + o);
+        }
+        const float axx_x = dtdx * lxx, axx_z = dtdz * lxx;
+        const float azz_x = dtdx * lzz, azz_z = dtdz * lzz;
+        const float axz_x = dtdx * lxz, axz_z = dtdz * lxz;
+        const float C11v = scur.cm[0], C13v = scur.cm[1], C33v = scur.cm[2], C55v = scur.cm[3];
+        P1 = C11v * axx_x + C13v * azz_x;
+        Q2 = C13v * axx_z + C33v * azz_z;
+        Q3 = C55v * axz_x;
+        P4 = C55v * axz_z;
+        const float dxb_vx = c0 * (vx2 - vxm1) + c1 * (vxp1 - vxm2);
+        const float dzb_vz = c0 * (vz2 - vz1) + c1 * (vz3 - vz0);
+        const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
+        const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
+        if constexpr (ANISO) {
+          const float C15v = scur.cm[4], C35v = scur.cm[5];
+          P2 = C15v * axx_z + C35v * azz_z;
+          P3 = C15v * axz_x;
+          Q1 = C15v * axx_x + C35v * azz_x;
+          Q4 = C35v * axz_z;
+          if (own_k) {
+            const float dxb_vz = c0 * (vz2 - vzm1) + c1 * (vzp1 - vzm2);
+            const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
+            const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
+            const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
+            acc(6, k, axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx);
+            acc(7, k, azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz);
+          }
+        }
+        if (own_k) {
+          acc(0, k, axx_x * dxb_vx);
+          acc(1, k, axx_z * dzb_vz + azz_x * dxb_vx);
+          acc(2, k, azz_z * dzb_vz);
+          acc(3, k, axz_x * dxf_vz + axz_z * dzf_vx);
+        }
+      }
+      p1a = p1b; p1b = p1c; p1c = P1;
+      q3a = q3b; q3b = q3c; q3c = Q3;
+      p40 = p41; p41 = p42; p42 = p43; p43 = p44; p44 = P4;
+      q20 = q21; q21 = q22; q22 = q23; q23 = q24; q24 = Q2;
+      if constexpr (ANISO) {
+        p3a = p3b; p3b = p3c; p3c = P3;
+        q1a = q1b; q1b = q1c; q1c = Q1;
+        p20 = p21; p21 = p22; p22 = p23; p23 = p24; p24 = P2;
+        q40 = q41; q41 = q42; q42 = q43; q43 = q44; q44 = Q4;
+      }
+      // forward stresses of row k after the step (stored with the free surface applied)
+      {
+        const bool in_grid = k >= 0 && k <= zmax;
+        xx0 = xx1; xx1 = xx2; xx2 = in_grid ? scur.fs[0] : 0.f;
+        zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = in_grid ? scur.fs[1] : 0.f;
+        xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = in_grid ? scur.fs[2] : 0.f;
+      }
+      // velocity adjoints of row r = k-2: L1(v) + D^T(products)
+      const int r = k - 2;
+      const float p1m1 = shfl(p1a, lane - 1), p1p1 = shfl(p1a, lane + 1), p1p2 = shfl(p1a, lane + 2);
+      const float q3m1 = shfl(q3a, lane - 1), q3m2 = shfl(q3a, lane - 2), q3p1 = shfl(q3a, lane + 1);
+      float lvx = (c0 * (p1a - p1p1) + c1 * (p1m1 - p1p2)) + (c0 * (p41 - p42) + c1 * (p40 - p43));
+      float lvz = (c0 * (q22 - q23) + c1 * (q21 - q24)) + (c0 * (q3m1 - q3a) + c1 * (q3m2 - q3p1));
+      if constexpr (ANISO) {
+        const float p3m1 = shfl(p3a, lane - 1), p3m2 = shfl(p3a, lane - 2), p3p1 = shfl(p3a, lane + 1);
+        const float q1m1 = shfl(q1a, lane - 1), q1p1 = shfl(q1a, lane + 1), q1p2 = shfl(q1a, lane + 2);
+        lvx = lvx + ((c0 * (p22 - p23) + c1 * (p21 - p24)) + (c0 * (p3m1 - p3a) + c1 * (p3m2 - p3p1)));
+        lvz = lvz + ((c0 * (q1a - q1p1) + c1 * (q1m1 - q1p2)) + (c0 * (q41 - q42) + c1 * (q40 - q43)));
+      }
+      const float txp1 = shfl(xx0, lane + 1), txp2 = shfl(xx0, lane + 2), txm1 = shfl(xx0, lane - 1);
+      const float tzm1 = shfl(xz2, lane - 1), tzm2 = shfl(xz2, lane - 2), tzp1 = shfl(xz2, lane + 1);
+      const bool own_r = out_col && ((r >= z0 && r < end_row) || (first && r >= 0 && r < 2));
+      if (own_r) {
+        const unsigned o = rb(r);
+        bst(ux0 + lvx, rLo, vs, F(VX) + o);
+        bst(uz0 + lvz, rLo, vs, F(VZ) + o);
+        if (r >= z0) {
+          const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
+          const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
+          const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
+          const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
+          acc(4, r, ux0 * (dt * (dxf_txx / dx + dzb_txz / dz)));
+          acc(5, r, uz0 * (dt * (dxb_txz / dx + dzf_tzz / dz)));
+        }
+      }
+      vcur = vnxt;
+      scur = snxt;
+    }
+  }
+  __syncthreads();
+  if (!out_col) return;
+  const long long planef = (long long)d.Z * d.pitch;
+  float *g = gacc + (long long)blockIdx.y * NCOEF * planef + c;
+  const float *sb = slab + lane;
+  for (int f = wv; f < NGF; f += EWPB)
+    for (int row = z0; row < end_row; ++row) {
+      const int o = (f * ETZA + (row - z0)) * WAVE;
+      const float v = (sb[o] + sb[NGF * ETZA * WAVE + o]) + (sb[2 * NGF * ETZA * WAVE + o] + sb[3 * NGF * ETZA * WAVE + o]);
+      float *p = g + (long long)f * planef + (long long)row * d.pitch;
+      *p = *p + v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
@@ -665,8 +1153,9 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
   ADFWI_CHECK_ARG(d->nz >= 9 && d->nx >= 9, "padded grid %d x %d too small (need >= 9 x 9)", d->nz, d->nx);
   ADFWI_CHECK_ARG(d->pitch >= d->nx, "pitch %d < nx %d", d->pitch, d->nx);
   ADFWI_CHECK_ARG(d->nt >= 1 && d->nrec >= 0, "nt %d / nrec %d invalid", d->nt, d->nrec);
-  ADFWI_CHECK_ARG(d->shots_per_wave >= 0 && d->tile_z >= 0 && d->tile_z <= ETZ_MAX,
-                  "shots_per_wave %d / tile_z %d invalid", d->shots_per_wave, d->tile_z);
+  ADFWI_CHECK_ARG(d->shots_per_wave >= 0 && (d->tile_z == 0 || (d->tile_z >= 4 && d->tile_z <= ETZ_MAX)),
+                  "shots_per_wave %d / tile_z %d invalid (tile_z: 0 or 4..%d)", d->shots_per_wave, d->tile_z,
+                  ETZ_MAX);
   const int rows = d->nz - 4;   // window rows [2, Z-3]
   L->zbase = 2;
   L->tile_x = ESX;
@@ -691,13 +1180,15 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
   }
   L->shots_per_block = spw * EWPB;
   L->ngroups = (d->nshots + L->shots_per_block - 1) / L->shots_per_block;
-  L->nfields = NFIELD;
+  ADFWI_CHECK_ARG(d->damping == 0 || d->damping == 1, "damping must be 0 (PML) or 1 (got %d)", d->damping);
+  const int nf = d->damping ? NFIELD5 : NFIELD;
+  L->nfields = nf;
   L->reserved = spw;
   L->field_floats = (int64_t)d->nshots * d->nz * d->pitch;
-  ADFWI_CHECK_ARG(L->field_floats * 4 * NFIELD < ((int64_t)1 << 32) - 1,
+  ADFWI_CHECK_ARG(L->field_floats * 4 * nf < ((int64_t)1 << 32) - 1,
                   "state of %d shots x %d x %d exceeds the 4 GiB of one buffer descriptor; "
                   "advance the shots in smaller launches", d->nshots, d->nz, d->pitch);
-  L->state_floats = NFIELD * L->field_floats;
+  L->state_floats = nf * L->field_floats;
   L->accum_floats = (int64_t)L->ngroups * NCOEF * d->nz * d->pitch;
   return ADFWI_OK;
 }
@@ -711,6 +1202,7 @@ ElDev el_dev(const adfwi_elastic_desc *d, const adfwi_layout &L) {
   v.plane = (long long)d->nz * d->pitch;
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.pml_x = d->pml_x; v.pml_z = d->pml_z; v.src = d->src; v.src_m = d->src_m;
+  v.damp = d->damp;
   v.src_z = d->src_z; v.src_x = d->src_x; v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
   return v;
 }
@@ -730,8 +1222,9 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
   int rc = el_layout(d, &L);
   if (rc) return rc;
   ADFWI_CHECK_ARG(a != nullptr, "null forward args");
-  ADFWI_CHECK_ARG(d->coef && d->pml_x && d->pml_z && d->src && d->src_m && d->src_z && d->src_x,
-                  "null coefficient/source pointer");
+  ADFWI_CHECK_ARG(d->coef && d->src && d->src_m && d->src_z && d->src_x, "null coefficient/source pointer");
+  ADFWI_CHECK_ARG(d->damping ? d->damp != nullptr : (d->pml_x && d->pml_z),
+                  d->damping ? "null damping plane" : "null PML factor table");
   const bool recs = a->rec[0] || a->rec[1] || a->rec[2] || a->rec[3] || a->rec[4];
   ADFWI_CHECK_ARG(!recs || (a->rec[0] && a->rec[1] && a->rec[2] && a->rec[3] && a->rec[4]),
                   "records must be all set or all NULL");
@@ -758,14 +1251,26 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
     } else {
       nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
     }
-    if (d->aniso)
-      hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
-    else
-      hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
-    if ((rc = adfwi::check_launch("el_fwd_step"))) return rc;
+    if (d->damping) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el_abl_fwd<true>, grid, block, 0, st, v, cur, nxt, t);
+      else
+        hipLaunchKernelGGL(el_abl_fwd<false>, grid, block, 0, st, v, cur, nxt, t);
+      if ((rc = adfwi::check_launch("el_abl_fwd"))) return rc;
+    } else {
+      if (d->aniso)
+        hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
+      else
+        hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
+      if ((rc = adfwi::check_launch("el_fwd_step"))) return rc;
+    }
     if (recs && ng > 0) {
-      hipLaunchKernelGGL(el_gather, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
-                         a->rec[4], t);
+      if (d->damping)
+        hipLaunchKernelGGL(el_gather5, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
+                           a->rec[4], t);
+      else
+        hipLaunchKernelGGL(el_gather, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
+                           a->rec[4], t);
       if ((rc = adfwi::check_launch("el_gather"))) return rc;
     }
     cur = nxt;
@@ -780,8 +1285,9 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   int rc = el_layout(d, &L);
   if (rc) return rc;
   ADFWI_CHECK_ARG(a != nullptr, "null adjoint args");
-  ADFWI_CHECK_ARG(d->coef && d->pml_x && d->pml_z && d->src_m && d->src_z && d->src_x,
-                  "null coefficient/source pointer");
+  ADFWI_CHECK_ARG(d->coef && d->src_m && d->src_z && d->src_x, "null coefficient/source pointer");
+  ADFWI_CHECK_ARG(d->damping ? d->damp != nullptr : (d->pml_x && d->pml_z),
+                  d->damping ? "null damping plane" : "null PML factor table");
   ADFWI_CHECK_ARG(a->work[0] && a->work[1] && a->work[0] != a->work[1], "two distinct work buffers are required");
   ADFWI_CHECK_ARG(a->lam_in == a->work[0] || a->lam_in == a->work[1], "lam_in must be work[0] or work[1]");
   ADFWI_CHECK_ARG(a->fwd0 && a->trace && a->grad_coef, "null forward state, trace or gradient buffer");
@@ -792,7 +1298,9 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   ElDev v = el_dev(d, L);
   // the adjoint marches strips of 4..8 rows (its gradient slab lives in LDS; the free-surface
   // transposes need rows 0..3 inside the first strip)
-  v.tz = min(max(L.tile_z, 4), ETZA);
+  // (damping scheme: strips of ETZA rows, so that no strip but the first reaches the
+  // free-surface rows, whose velocity adjoints are state there)
+  v.tz = d->damping ? ETZA : min(max(L.tile_z, 4), ETZA);
   v.nstrips = ((d->nz - 4 + v.tz - 1) / v.tz) * v.nsx;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid(v.nstrips, L.ngroups), block(ENT);
@@ -803,18 +1311,30 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   for (int i = a->nsteps - 1; i >= 0; --i) {
     const int t = t_lo + i;
     if (recs && ng > 0) {
-      hipLaunchKernelGGL(el_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec[0], a->grad_rec[1], a->grad_rec[2],
-                         a->grad_rec[3], a->grad_rec[4], t);
+      if (d->damping)
+        hipLaunchKernelGGL(el_inject5, ggrid, gblock, 0, st, v, cur, a->grad_rec[0], a->grad_rec[1],
+                           a->grad_rec[2], a->grad_rec[3], a->grad_rec[4], t);
+      else
+        hipLaunchKernelGGL(el_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec[0], a->grad_rec[1],
+                           a->grad_rec[2], a->grad_rec[3], a->grad_rec[4], t);
       if ((rc = adfwi::check_launch("el_inject"))) return rc;
     }
     const float *fv = i == 0 ? a->fwd0 : a->trace + (long long)(i - 1) * L.state_floats;
     const float *fsn = a->trace + (long long)i * L.state_floats;
     float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
-    if (d->aniso)
-      hipLaunchKernelGGL(el_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
-    else
-      hipLaunchKernelGGL(el_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
-    if ((rc = adfwi::check_launch("el_adj_step"))) return rc;
+    if (d->damping) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el_abl_adj<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      else
+        hipLaunchKernelGGL(el_abl_adj<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      if ((rc = adfwi::check_launch("el_abl_adj"))) return rc;
+    } else {
+      if (d->aniso)
+        hipLaunchKernelGGL(el_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      else
+        hipLaunchKernelGGL(el_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      if ((rc = adfwi::check_launch("el_adj_step"))) return rc;
+    }
     cur = nxt;
   }
   a->lam_out = cur;

@@ -15,9 +15,9 @@ from .acoustic_kernels import TIMELINE
 NCOEF = 8
 
 
-def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: torch.Tensor, pml_z: torch.Tensor,
-                 scalars, ckpt: Optional[torch.Tensor], k: int, g_rec: Sequence[Optional[torch.Tensor]],
-                 needs: Sequence[bool]):
+def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[torch.Tensor],
+                 pml_z: Optional[torch.Tensor], scalars, ckpt: Optional[torch.Tensor], k: int,
+                 g_rec: Sequence[Optional[torch.Tensor]], needs: Sequence[bool], damp: Optional[torch.Tensor] = None):
     """Adjoint of all nt steps.
 
     ckpt: the forward's checkpoints (slot j = state after step (j+1)k-1), g_rec: five (S, nt, R)
@@ -38,7 +38,7 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: torch.Tenso
     gacc = torch.zeros(int(L.accum_floats), device=dev, dtype=torch.float32)
     gsrc = torch.zeros((S, nt), device=dev, dtype=torch.float32)
     g = [None if x is None else x.detach().to(torch.float32).contiguous() for x in g_rec]
-    d = geom.desc(coef, pml_x, pml_z, src, scalars)
+    d = geom.desc(coef, pml_x, pml_z, src, scalars, damp)
     stream = nat.stream_handle(dev)
     cur = lam[0]
     for seg in reversed(range(nseg)):
@@ -54,7 +54,7 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: torch.Tenso
         fa.state_in = fwd0.data_ptr()
         fa.trace = trace.data_ptr()
         fa.t0, fa.nsteps = t0, n
-        TIMELINE.call("recompute", "el_fwd_step", n, dev, lambda: nat.check(
+        TIMELINE.call("recompute", "el_abl_fwd" if geom.damping else "el_fwd_step", n, dev, lambda: nat.check(
             lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(fa), stream), "adfwi_elastic_forward (recompute)"))
         aa = nat.ElasticAdjArgs()
         aa.lam_in = cur.data_ptr()
@@ -65,7 +65,7 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: torch.Tenso
             aa.grad_rec[i] = None if g[i] is None else g[i].data_ptr()
         aa.grad_coef = gacc.data_ptr()
         aa.grad_src = gsrc.data_ptr()
-        TIMELINE.call("adjoint", "el_adj_step", n, dev, lambda: nat.check(
+        TIMELINE.call("adjoint", "el_abl_adj" if geom.damping else "el_adj_step", n, dev, lambda: nat.check(
             lib.adfwi_elastic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream), "adfwi_elastic_adjoint"))
         cur = lam[0] if aa.lam_out == lam[0].data_ptr() else lam[1]
     planes = gacc.view(-1, NCOEF, Z, pitch).sum(0)

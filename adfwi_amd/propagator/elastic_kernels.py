@@ -59,12 +59,15 @@ class ElasticGeometry:
     moment-tensor factors), receivers, tiling."""
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, src_z, src_x, src_m: torch.Tensor,
-                 rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0):
+                 rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0,
+                 damping: bool = False):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
         self.free_surface = bool(free_surface)
         self.aniso = bool(aniso)
+        self.damping = bool(damping)          # multiplicative damping scheme (5 fields) instead of PML
+        self.nfield = 5 if self.damping else NFIELD
         self.pitch = -(-self.X // PITCH_ALIGN) * PITCH_ALIGN
         src_z = np.asarray(src_z, dtype=np.int64).reshape(-1)
         src_x = np.asarray(src_x, dtype=np.int64).reshape(-1)
@@ -88,22 +91,30 @@ class ElasticGeometry:
         nat.check(nat.load().adfwi_elastic_get_layout(ctypes.byref(d), ctypes.byref(self.layout)),
                   "adfwi_elastic_get_layout")
 
-    def desc(self, coef, pml_x, pml_z, src, scalars) -> nat.ElasticDesc:
+    def desc(self, coef, pml_x, pml_z, src, scalars, damp=None) -> nat.ElasticDesc:
+        """pml_x, pml_z: PML factor tables (PML scheme); damp: [Z][pitch] taper (damping scheme)."""
         d = nat.ElasticDesc()
         d.abi_version, d.struct_size = nat.ABI_VERSION, ctypes.sizeof(nat.ElasticDesc)
         d.nshots, d.nz, d.nx, d.pitch = self.S, self.Z, self.X, self.pitch
         d.free_surface, d.nt, d.nrec, d.aniso = int(self.free_surface), self.nt, self.R, int(self.aniso)
         d.shots_per_wave, d.tile_z = self.shots_per_wave, self.tile_z
         d.dt, d.dt_dx, d.dt_dz, d.dx, d.dz, d.c0, d.c1 = scalars
+        d.damping = int(self.damping)
         ptr = lambda t: None if t is None else t.data_ptr()
         d.coef, d.pml_x, d.pml_z, d.src = ptr(coef), ptr(pml_x), ptr(pml_z), ptr(src)
+        d.damp = ptr(damp)
         d.src_m = self.src_m.data_ptr()
         d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
         d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
         return d
 
     def state(self, n: int = 1) -> torch.Tensor:
-        return torch.zeros((n, NFIELD, self.S, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+        return torch.zeros((n, self.nfield, self.S, self.Z, self.pitch), device=self.device, dtype=torch.float32)
+
+    def pack_plane(self, f: torch.Tensor) -> torch.Tensor:
+        out = torch.zeros((self.Z, self.pitch), device=self.device, dtype=torch.float32)
+        out[:f.shape[0], :f.shape[1]] = f
+        return out
 
     def pack_coef(self, fields) -> torch.Tensor:
         """fields: C11, C13, C33, C55, bx, bz, C15, C35 padded arrays (each from its own shape)."""
@@ -132,10 +143,11 @@ def pml_factors(bcx_p: torch.Tensor, bcz_p: torch.Tensor, dt: float, pitch: int)
 
 
 def _energy(state: torch.Tensor, X: int, free_surface: bool):
-    """sum over shots of each total squared (elastic_kernels.py:414-418), free-surface tzz row 2 = 0."""
+    """sum over shots of each total squared (elastic_kernels.py:414-418), free-surface tzz row 2 = 0.
+    A 5-field state (damping scheme) holds the totals themselves."""
     out = []
     for k in range(5):
-        tot = state[2 * k] + state[2 * k + 1]
+        tot = state[k] if state.shape[0] == 5 else state[2 * k] + state[2 * k + 1]
         if k == 1 and free_surface:
             tot = tot.clone()
             tot[:, 2, :] = 0.0
@@ -144,7 +156,7 @@ def _energy(state: torch.Tensor, X: int, free_surface: bool):
 
 
 def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, record=True, energy_ends=None,
-                ckpt_every: int = 0):
+                ckpt_every: int = 0, damp=None):
     """All nt steps from a zero state.  Returns dict(rec=[5 x (S, nt, R)] or None,
     energy=[5 x (Z, X)] summed over the states after the steps in energy_ends, ckpt)."""
     lib = nat.load()
@@ -155,7 +167,7 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
     if ckpt_every:
         ckpt = geom.state(max(-(-nt // ckpt_every) - 1, 1))
     rec = [torch.zeros((S, nt, R), device=dev) for _ in range(5)] if record and R else None
-    d = geom.desc(coef, pml_x, pml_z, src, scalars)
+    d = geom.desc(coef, pml_x, pml_z, src, scalars, damp)
     ends = sorted(set(energy_ends or [])) or [nt - 1]
     energy = None
     cur = work[0]
@@ -171,7 +183,7 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
         if rec is not None:
             for i in range(5):
                 a.rec[i] = rec[i].data_ptr()
-        TIMELINE.call("forward", "el_fwd_step", a.nsteps, dev, lambda: nat.check(
+        TIMELINE.call("forward", "el_abl_fwd" if geom.damping else "el_fwd_step", a.nsteps, dev, lambda: nat.check(
             lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream), "adfwi_elastic_forward"))
         cur = work[0] if a.state_out == work[0].data_ptr() else work[1]
         if energy_ends is not None:
@@ -185,7 +197,8 @@ class ElasticWave(torch.autograd.Function):
     """Five records (txx, tzz, txz, vx, vz) and energy maps from padded coefficient fields.
 
     forward(c11, c13, c33, c55, bx, bz, c15, c35, wavelet, pml_x, pml_z, geom, scalars,
-            energy_ends, ckpt_every); c15/c35 may be None (isotropic / VTI).
+            energy_ends, ckpt_every); c15/c35 may be None (isotropic / VTI).  With a damping-scheme
+    geometry, pml_x is the [Z][pitch] damping taper and pml_z is None.
     """
 
     @staticmethod
@@ -196,8 +209,9 @@ class ElasticWave(torch.autograd.Function):
         src = wav.detach().to(torch.float32).contiguous()
         need_grad = any(ctx.needs_input_grad[:9])
         k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
-        res = run_forward(geom, coef, pml_x, pml_z, src, scalars, record=True, energy_ends=energy_ends,
-                          ckpt_every=k if need_grad else 0)
+        absorb = (None, None, pml_x) if geom.damping else (pml_x, pml_z, None)
+        res = run_forward(geom, coef, absorb[0], absorb[1], src, scalars, record=True, energy_ends=energy_ends,
+                          ckpt_every=k if need_grad else 0, damp=absorb[2])
         rec = res["rec"] or [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(5)]
         ctx.geom, ctx.k, ctx.scalars = geom, k, scalars
         ctx.ckpt = res["ckpt"] if need_grad else None
@@ -213,8 +227,9 @@ class ElasticWave(torch.autograd.Function):
             return (None,) * 15
         from .elastic_adjoint import run_backward
         coef, src, pml_x, pml_z = ctx.saved_tensors
-        gcoef, gsrc = run_backward(ctx.geom, coef, src, pml_x, pml_z, ctx.scalars, ctx.ckpt, ctx.k, grads[:5],
-                                   ctx.needs_input_grad[:9])
+        absorb = (None, None, pml_x) if ctx.geom.damping else (pml_x, pml_z, None)
+        gcoef, gsrc = run_backward(ctx.geom, coef, src, absorb[0], absorb[1], ctx.scalars, ctx.ckpt, ctx.k, grads[:5],
+                                   ctx.needs_input_grad[:9], damp=absorb[2])
         ctx.ckpt = None
         out = []
         for i, shp in enumerate(ctx.shapes):
@@ -235,15 +250,21 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    device: torch.device = torch.device("cpu"), dtype=torch.float32,
                    geometry: Optional[ElasticGeometry] = None, ckpt_every: int = 0) -> Dict[str, torch.Tensor]:
     """Elastic forward modelling with the reference's signature and result dictionary
-    (elastic_kernels.py:917-1037).  ``n_segments`` keeps its meaning for the energy maps
-    (summed over the last step of every time chunk); memory is governed by the sqrt(nt)
-    wavefield checkpoints of the HIP adjoint."""
+    (elastic_kernels.py:917-1037), 4th order: the split-field PML scheme for abc_type 'pml' (any
+    case, as the reference's driver tests it), the multiplicative-damping scheme with ``damp``
+    otherwise.  ``n_segments`` keeps its meaning for the energy maps (summed over the last step of
+    every time chunk); memory is governed by the sqrt(nt) wavefield checkpoints of the HIP
+    adjoint."""
     nat.require_gpu(device)
     if dtype != torch.float32:
         raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
-    if fd_order != 4 or abc_type.lower() != "pml":
-        raise NotImplementedError("the HIP elastic propagator implements the 4th-order PML branch "
-                                  f"(got fd_order={fd_order}, abc_type={abc_type!r})")
+    if fd_order != 4:
+        raise NotImplementedError(f"the HIP elastic propagator implements the 4th-order scheme (got fd_order={fd_order})")
+    use_pml = abc_type.lower() == "pml"
+    if use_pml and (bcx is None or bcz is None):
+        raise ValueError("the PML scheme needs the bcx / bcz damping profiles")
+    if not use_pml and damp is None:
+        raise ValueError(f"abc_type {abc_type!r} needs the damping taper 'damp'")
     fso = fd_order // 2
     X = nx + 2 * nabc
     Z = nz + (nabc + fso if free_surface else 2 * nabc + fso)
@@ -251,19 +272,27 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
     c11, c13, c15, c33, c35, c55 = (pad(CC[i]) for i in (0, 2, 4, 11, 13, 18))
     bxp, bzp = pad(bx), pad(bz)
     aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
-    bcx_p = pad_torchSingle(bcx.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
-    bcz_p = pad_torchSingle(bcz.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
-    if geometry is None:
+    if geometry is None or geometry.damping == use_pml:
         zoff = fso if free_surface else nabc + fso
         MTd = MT.to(device=device, dtype=torch.float32)
-        half = MTd / 2
-        src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+        if use_pml:
+            half = MTd / 2
+            src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+        else:
+            src_m = torch.stack([(-1 / 3) * MTd[:, 0, 0], (-1 / 3) * MTd[:, 2, 2], (-1 / 3) * MTd[:, 0, 2]], dim=1)
         geometry = ElasticGeometry(Z, X, nt, free_surface, (src_z + zoff).cpu().numpy(),
                                    (src_x + nabc).cpu().numpy(), src_m, (rcv_z + zoff).cpu().numpy(),
-                                   (rcv_x + nabc).cpu().numpy(), device, aniso=aniso)
+                                   (rcv_x + nabc).cpu().numpy(), device, aniso=aniso, damping=not use_pml)
     elif geometry.aniso != aniso:
         geometry.aniso = aniso
-    pml_x, pml_z = pml_factors(bcx_p, bcz_p, dt, geometry.pitch)
+    if use_pml:
+        bcx_p = pad_torchSingle(bcx.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
+        bcz_p = pad_torchSingle(bcz.to(device=device, dtype=torch.float32), 0, fso, free_surface, device)
+        absorb_a, absorb_b = pml_factors(bcx_p, bcz_p, dt, geometry.pitch)
+    else:
+        dmp = pad_torchSingle(torch.as_tensor(damp).to(device=device, dtype=torch.float32), 0, fso, free_surface,
+                              device)
+        absorb_a, absorb_b = geometry.pack_plane(dmp), None
     fdc = DiffCoef(fso, "s")
     f32 = lambda v: float(np.float32(v))
     scalars = (f32(dt), f32(dt / dx), f32(dt / dz), f32(dx), f32(dz), float(fdc[0]), float(fdc[1]))
@@ -271,7 +300,7 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
         src_v = src_v.unsqueeze(0).expand(src_n, -1)
     ends = chunk_ends(nt, n_segments)
     outs = ElasticWave.apply(c11, c13, c33, c55, bxp, bzp, c15 if aniso else None, c35 if aniso else None,
-                             src_v, pml_x, pml_z, geometry, scalars, ends, ckpt_every)
+                             src_v, absorb_a, absorb_b, geometry, scalars, ends, ckpt_every)
     row0 = fso if free_surface else fso + nabc
     res = {n: outs[i] for i, n in enumerate(NAMES)}
     for i, n in enumerate(NAMES):

@@ -1,4 +1,4 @@
-"""ElasticPropagator: drop-in for ADFWI/propagator/elastic_propagator.py:20-142.
+"""ElasticPropagator: drop-in for ADFWI/propagator/elastic_propagator.py:152-261.
 
 Same constructor, attributes (src_n, bcx, bcz, damp, wavelet, moment_tensor, rcv_*),
 forward(model, shot_index, fd_order, checkpoint_segments) and record dictionary; the
@@ -74,8 +74,8 @@ class ElasticPropagator(torch.nn.Module):
             damp = bc_sincos(self.nx, self.nz, self.dx, self.dz, pml=self.nabc, free_surface=self.free_surface)
             self.damp = numpy2tensor(damp, self.dtype).to(self.device)
 
-    def _geometry(self, shots, aniso):
-        key = (tuple(int(s) for s in shots), bool(aniso))
+    def _geometry(self, shots, aniso, damping):
+        key = (tuple(int(s) for s in shots), bool(aniso), bool(damping))
         g = self._geoms.get(key)
         if g is None:
             fso = 2
@@ -83,10 +83,15 @@ class ElasticPropagator(torch.nn.Module):
             X = self.nx + 2 * self.nabc
             zoff = fso if self.free_surface else self.nabc + fso
             sl = self.src_loc[shots]
-            half = self.moment_tensor[torch.as_tensor(shots, device=self.moment_tensor.device)] / 2
-            src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+            mt = self.moment_tensor[torch.as_tensor(shots, device=self.moment_tensor.device)]
+            if damping:   # -MT/3 (elastic_kernels.py:703)
+                src_m = torch.stack([(-1 / 3) * mt[:, 0, 0], (-1 / 3) * mt[:, 2, 2], (-1 / 3) * mt[:, 0, 2]], dim=1)
+            else:         # -MT/2 into each split (elastic_kernels.py:358-372)
+                half = mt / 2
+                src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
             g = ElasticGeometry(Z, X, self.nt, self.free_surface, sl[:, 1] + zoff, sl[:, 0] + self.nabc, src_m,
-                                self.rcv_loc[:, 1] + zoff, self.rcv_loc[:, 0] + self.nabc, self.device, aniso=aniso)
+                                self.rcv_loc[:, 1] + zoff, self.rcv_loc[:, 0] + self.nabc, self.device, aniso=aniso,
+                                damping=damping)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)
@@ -104,8 +109,10 @@ class ElasticPropagator(torch.nn.Module):
         CC = model.CC
         aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
         geom = None
-        if fd_order == 4 and self.abc_type == "PML":
-            geom = self._geometry(shots, aniso)
+        if fd_order == 4 and (self.abc_type == "PML") == (self.abc_type.lower() == "pml"):
+            # the reference's propagator builds bcx/bcz only for 'PML' exactly (elastic_propagator.py
+            # :219) while its driver takes the PML branch for any case of 'pml' (elastic_kernels.py:948)
+            geom = self._geometry(shots, aniso, damping=self.abc_type != "PML")
         return forward_kernel(self.nx, self.nz, self.dx, self.dz, self.nt, self.dt, self.nabc, self.free_surface,
                               src_x, src_z, len(shots), wavelet, mt, self.rcv_x, self.rcv_z, self.rcv_n,
                               self.abc_type, self.bcx, self.bcz, self.damp, model.lamu, model.lam,

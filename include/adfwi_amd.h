@@ -158,25 +158,28 @@ typedef struct adfwi_elastic_desc {
   int32_t tile_z;          /* strip rows, 0 = library choice */
   float dt, dt_dx, dt_dz, dx, dz;  /* fp32 scalars as the reference rounds them */
   float c0, c1;            /* staggered FD coefficients DiffCoef(2, 's') */
-  int32_t reserved;
+  int32_t damping;         /* 0: split-field PML (step_forward_PML_4order, ten split fields);
+                              1: multiplicative damping (step_forward_ABL_4order, elastic_kernels.py
+                              :585-777: five fields txx, tzz, txz, vx, vz, -MT/3 source) */
   const float *coef;       /* [8][nz][pitch]: C11, C13, C33, C55, bx, bz, C15, C35 (padded as in
                               the reference, each from its own shape; row/col 0 based) */
-  const float *pml_x;      /* [3][pitch]: 1 - dt/2 BCx, 1 + dt/2 BCx, 1/(1 + dt/2 BCx) per column */
-  const float *pml_z;      /* [3][nz]: the same for BCz per row */
+  const float *pml_x;      /* PML: [3][pitch]: 1 - dt/2 BCx, 1 + dt/2 BCx, 1/(1 + dt/2 BCx) per column */
+  const float *pml_z;      /* PML: [3][nz]: the same for BCz per row */
   const float *src;        /* [nshots][nt] wavelets */
-  const float *src_m;      /* [nshots][3]: -MT_xx/2, -MT_zz/2, -MT_xz/2 */
+  const float *src_m;      /* [nshots][3]: PML -MT_xx/2, -MT_zz/2, -MT_xz/2; damping -MT/3 likewise */
   const int32_t *src_z, *src_x;   /* [nshots] padded source indices (inside the window) */
   const int32_t *rcv_z, *rcv_x;   /* [nrec] padded receiver indices (inside the window) */
+  const float *damp;       /* damping: [nz][pitch] multiplicative taper of vx, vz; PML: unused */
 } adfwi_elastic_desc;
 
-/* Tiling and buffer sizes; layout.nfields = 10 (split fields), layout.reserved = shots per
+/* Tiling and buffer sizes; layout.nfields = 10 (PML split fields) or 5 (damping), layout.reserved = shots per
  * wavefront, layout.shots_per_block = shots per workgroup, layout.ngroups = workgroups along
  * the shots, layout.accum_floats = size of the gradient accumulator of the adjoint,
  * [ngroups][8][nz][pitch] (one set of planes per workgroup's shots). */
 int adfwi_elastic_get_layout(const adfwi_elastic_desc *d, adfwi_layout *out);
 
 typedef struct adfwi_elastic_fwd_args {
-  const float *state_in;   /* [10][S][Z][pitch] split fields before step t0 */
+  const float *state_in;   /* [nfields][S][Z][pitch] state before step t0 */
   float *work[2];
   float *ckpt;             /* [n][10][S][Z][pitch] or NULL */
   int32_t ckpt_every;

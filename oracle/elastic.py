@@ -167,6 +167,91 @@ def _steps(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int
             vx_x, vx_z, vz_x, vz_z, vx, vz] + recs + ens
 
 
+@torch.jit.script
+def _steps_abl(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int,
+               src_x: torch.Tensor, src_z: torch.Tensor, dt: float, src_v: torch.Tensor, mt: torch.Tensor,
+               rcv_x: torch.Tensor, rcv_z: torch.Tensor, damp: torch.Tensor,
+               c11: torch.Tensor, c13: torch.Tensor, c15: torch.Tensor, c33: torch.Tensor,
+               c35: torch.Tensor, c55: torch.Tensor, bx: torch.Tensor, bz: torch.Tensor, fdc: torch.Tensor,
+               txx: torch.Tensor, tzz: torch.Tensor, txz: torch.Tensor,
+               vx: torch.Tensor, vz: torch.Tensor) -> List[torch.Tensor]:
+    """``src_v.shape[-1]`` steps of step_forward_ABL_4order (elastic_kernels.py:585-777): unsplit
+    stress and velocity updates, -MT/3 source (:703, :727-735), stress and velocity free-surface
+    rows kept in the state (:738-742, :753-756), multiplicative damping of vx, vz (:759-760)."""
+    S = vx.shape[0]
+    Z = vx.shape[1]
+    X = vx.shape[2]
+    nt = src_v.shape[-1]
+    R = rcv_x.shape[0]
+    txx, tzz, txz, vx, vz = txx.clone(), tzz.clone(), txz.clone(), vx.clone(), vz.clone()
+    recs = [torch.zeros((S, nt, R), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    ens = [torch.zeros((nz, nx), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    W0, WZ, WX = 2, Z - 2, X - 2
+    shots = torch.arange(S, device=vx.device)
+    scaling = -1.0 / 3
+    row0 = 2 if free_surface else 2 + nabc
+    C11 = c11[W0:WZ, W0:WX]
+    C13 = c13[W0:WZ, W0:WX]
+    C15 = c15[W0:WZ, W0:WX]
+    C33 = c33[W0:WZ, W0:WX]
+    C35 = c35[W0:WZ, W0:WX]
+    C55 = c55[W0:WZ, W0:WX]
+    BX = bx[W0:WZ, W0:WX]
+    BZ = bz[W0:WZ, W0:WX]
+    for t in range(nt):
+        dxb_vx = _dxb(vx, fdc, Z, X)
+        dxb_vz = _dxb(vz, fdc, Z, X)
+        dzb_vx = _dzb(vx, fdc, Z, X)
+        dzb_vz = _dzb(vz, fdc, Z, X)
+        dxf_vx = _dxf(vx, fdc, Z, X)
+        dzf_vx = _dzf(vx, fdc, Z, X)
+        dxf_vz = _dxf(vz, fdc, Z, X)
+        dzf_vz = _dzf(vz, fdc, Z, X)
+        txx[:, W0:WZ, W0:WX] = txx[:, W0:WZ, W0:WX] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dx
+                                                            + (C15 * dzb_vx + C13 * dzb_vz) / dz)
+        tzz[:, W0:WZ, W0:WX] = tzz[:, W0:WZ, W0:WX] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dx
+                                                            + (C35 * dzb_vx + C33 * dzb_vz) / dz)
+        txz[:, W0:WZ, W0:WX] = txz[:, W0:WZ, W0:WX] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dx
+                                                            + (C55 * dzf_vx + C35 * dzf_vz) / dz)
+        sv = src_v[shots, t]
+        txx[shots, src_z, src_x] += scaling * mt[:, 0, 0] * sv
+        tzz[shots, src_z, src_x] += scaling * mt[:, 2, 2] * sv
+        txz[shots, src_z, src_x] += scaling * mt[:, 0, 2] * sv
+        if free_surface:
+            tzz[:, 2, :] = 0
+            tzz[:, 1, :] = -tzz[:, 3, :]
+            txz[:, 1, :] = -txz[:, 2, :]
+            txz[:, 0, :] = -txz[:, 3, :]
+        dxf_txx = _dxf(txx, fdc, Z, X)
+        dzb_txz = _dzb(txz, fdc, Z, X)
+        dxb_txz = _dxb(txz, fdc, Z, X)
+        dzf_tzz = _dzf(tzz, fdc, Z, X)
+        vx[:, W0:WZ, W0:WX] += dt * BX * (dxf_txx / dx + dzb_txz / dz)
+        vz[:, W0:WZ, W0:WX] += dt * BZ * (dxb_txz / dx + dzf_tzz / dz)
+        if free_surface:
+            vz[:, 1, W0:WX] = vz[:, 2, W0:WX]
+            vx[:, 1, W0:WX] = vz[:, 1, W0 + 1:WX + 1] - vz[:, 1, W0:WX] + vz[:, 2, W0 + 1:WX + 1] \
+                - vz[:, 2, W0:WX] + vx[:, 3, W0:WX]
+            vz[:, 0, W0:WX] = vz[:, 1, W0:WX]
+        vx = vx * damp
+        vz = vz * damp
+        recs[0][:, t, :] = txx[:, rcv_z, rcv_x]
+        recs[1][:, t, :] = tzz[:, rcv_z, rcv_x]
+        recs[2][:, t, :] = txz[:, rcv_z, rcv_x]
+        recs[3][:, t, :] = vx[:, rcv_z, rcv_x]
+        recs[4][:, t, :] = vz[:, rcv_z, rcv_x]
+        ens[0] = torch.sum(txx * txx, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+        ens[1] = torch.sum(tzz * tzz, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+        ens[2] = torch.sum(txz * txz, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+        ens[3] = torch.sum(vx * vx, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+        ens[4] = torch.sum(vz * vz, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+    return [txx, tzz, txz, vx, vz] + recs + ens
+
+
+def _steps_abl_tuple(*args):
+    return tuple(_steps_abl(*args))
+
+
 def _steps_tuple(*args):
     """Tuple-returning wrapper: checkpoint() tracks tensors returned in a tuple, not a list."""
     return tuple(_steps(*args))
@@ -179,8 +264,8 @@ def padded_fields(bx, bz, CC, bcx, bcz, nabc: int, free_surface: bool, fd_order:
     out = {k: pad_elastic(a, nabc, fso, free_surface) for k, a in
            (("c11", c11), ("c13", c13), ("c15", c15), ("c33", c33), ("c35", c35), ("c55", c55),
             ("bx", bx), ("bz", bz))}
-    out["bcx"] = pad_elastic(bcx, 0, fso, free_surface)
-    out["bcz"] = pad_elastic(bcz, 0, fso, free_surface)
+    out["bcx"] = None if bcx is None else pad_elastic(bcx, 0, fso, free_surface)
+    out["bcz"] = None if bcz is None else pad_elastic(bcz, 0, fso, free_surface)
     return out
 
 
@@ -192,31 +277,40 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    lamu, lam, bx: torch.Tensor, bz: torch.Tensor, CC: List[torch.Tensor],
                    fd_order: int = 4, n_segments: int = 1,
                    device=torch.device("cpu"), dtype=torch.float32) -> Dict[str, torch.Tensor]:
-    """Elastic forward modelling, elastic_kernels.py:917-1037 semantics (PML, order 4)."""
-    if abc_type.lower() != "pml" or fd_order != 4:
-        raise NotImplementedError("oracle covers the 4th-order PML branch")
+    """Elastic forward modelling, elastic_kernels.py:917-1037 semantics (order 4): the split-field
+    PML branch for abc_type 'pml' (any case, :948, :976), the damping (ABL) branch otherwise."""
+    if fd_order != 4:
+        raise NotImplementedError("oracle covers the 4th-order branches")
+    pml = abc_type.lower() == "pml"
     fso = fd_order // 2
     X = nx + 2 * nabc
     Z = nz + (nabc + fso if free_surface else 2 * nabc + fso)
-    f = padded_fields(bx, bz, CC, bcx, bcz, nabc, free_surface, fd_order)
+    f = padded_fields(bx, bz, CC, bcx if pml else None, bcz if pml else None, nabc, free_surface, fd_order)
+    dmp = None if pml else pad_elastic(damp, 0, fso, free_surface)
     fdc = diff_coef(fso)
     sx = src_x + nabc
     sz = src_z + fso if free_surface else src_z + nabc + fso
     rx = rcv_x + nabc
     rz = rcv_z + fso if free_surface else rcv_z + nabc + fso
-    state = [torch.zeros((src_n, Z, X), dtype=dtype, device=device) for _ in range(15)]
+    nstate = 15 if pml else 5
+    state = [torch.zeros((src_n, Z, X), dtype=dtype, device=device) for _ in range(nstate)]
     recs = [torch.zeros((src_n, nt, rcv_n), dtype=dtype, device=device) for _ in range(5)]
     ens = [torch.zeros((nz, nx), dtype=dtype, device=device) for _ in range(5)]
     t0 = 0
     for chunk in torch.chunk(src_v, n_segments, dim=-1):
-        out = checkpoint(_steps_tuple, free_surface, nx, nz, dx, dz, nabc, sx, sz, dt, chunk, MT, rx, rz,
-                         f["bcx"], f["bcz"], f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
-                         f["bx"], f["bz"], fdc, *state, use_reentrant=True)
-        state = list(out[:15])
+        if pml:
+            out = checkpoint(_steps_tuple, free_surface, nx, nz, dx, dz, nabc, sx, sz, dt, chunk, MT, rx, rz,
+                             f["bcx"], f["bcz"], f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
+                             f["bx"], f["bz"], fdc, *state, use_reentrant=True)
+        else:
+            out = checkpoint(_steps_abl_tuple, free_surface, nx, nz, dx, dz, nabc, sx, sz, dt, chunk, MT, rx, rz,
+                             dmp, f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
+                             f["bx"], f["bz"], fdc, *state, use_reentrant=True)
+        state = list(out[:nstate])
         n = chunk.shape[-1]
         for k in range(5):
-            recs[k][:, t0:t0 + n] = out[15 + k]
-            ens[k] += out[20 + k].detach()
+            recs[k][:, t0:t0 + n] = out[nstate + k]
+            ens[k] += out[nstate + 5 + k].detach()
         t0 += n
     names = ("txx", "tzz", "txz", "vx", "vz")
     res = {n_: r for n_, r in zip(names, recs)}

@@ -3,7 +3,7 @@
 Run in the development container only (the reference tree is not present where
 the tests run):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 It loads three self-contained reference files by path -- they import only
 torch/numpy/matplotlib, so no stubs are needed -- and records, on tiny grids,
@@ -13,10 +13,12 @@ the inputs and outputs of:
   p/u/w, the three energy maps, and autograd gradients of a seeded linear
   functional of all records with respect to vp, rho and the wavelet, for
   free surface off/on and checkpoint_segments 1 and 3;
-* ADFWI/propagator/elastic_kernels.py   forward_kernel (:917-1037, 4th-order
-  PML branch): five records, five energy maps and gradients with respect to
-  C11, C13, C33, C55, bx, bz and the wavelet, isotropic with free surface and
-  VTI without, segments 1 and 3; DiffCoef(2, 's') (:20-58);
+* ADFWI/propagator/elastic_kernels.py   forward_kernel (:917-1037, 4th order):
+  five records, five energy maps and gradients with respect to C11, C13, C33,
+  C55, bx, bz and the wavelet, isotropic with free surface and VTI without,
+  segments 1 and 3, for the split-field PML branch and for the damping (ABL)
+  branch with Cerjan (free surface) and sin^2 (no free surface) tapers;
+  DiffCoef(2, 's') (:20-58);
 * ADFWI/propagator/boundary_condition.py bc_pml, bc_pml_xz, bc_sincos,
   bc_gerjan (:15-120).
 
@@ -90,7 +92,7 @@ def acoustic_case(ak, bc, free_surface, seed):
     return out
 
 
-def elastic_case(ek, bc, free_surface, vti, seed):
+def elastic_case(ek, bc, free_surface, vti, seed, abc="PML"):
     rng = np.random.default_rng(seed)
     nz, nx, nabc, dx, dt, nt = 16, 24, 6, 5.0, 4e-4, 120
     vp = smooth_field(rng, nz, nx, 1800.0, 3000.0)
@@ -113,11 +115,17 @@ def elastic_case(ek, bc, free_surface, vti, seed):
     wav = np.stack([1e6 * ricker(nt, dt, 60.0), 0.8e6 * ricker(nt, dt, 50.0)]).astype(np.float32)
     mt = np.stack([np.eye(3), np.array([[1.0, 0, 0.5], [0, 1, 0], [0.5, 0, -0.7]])]).astype(np.float32)
     bcx, bcz = bc.bc_pml_xz(nx, nz, dx, dx, pml=nabc, vmax=float(vp.max()), free_surface=free_surface)
+    if abc == "gerjan":
+        damp = bc.bc_gerjan(nx, nz, dx, dx, pml=nabc, alpha=0.0053, free_surface=free_surface)
+    elif abc != "PML":
+        damp = bc.bc_sincos(nx, nz, dx, dx, pml=nabc, free_surface=free_surface)
+    else:
+        damp = np.zeros((1, 1))
     wts = rng.standard_normal((5, 2, nt, nx)).astype(np.float32)
     out = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, free_surface=int(free_surface),
                c11=c11, c13=c13, c33=c33, c55=c55, bx=bx, bz=bz, src_x=src_x, src_z=src_z,
                rcv_x=rcv_x, rcv_z=rcv_z, wavelet=wav, mt=mt, bcx=bcx, bcz=bcz, weights=wts,
-               vp=vp, vs=vs, rho=rho, eps=eps, delta=delta)
+               vp=vp, vs=vs, rho=rho, eps=eps, delta=delta, abc=abc, damp=damp)
     names = ("txx", "tzz", "txz", "vx", "vz")
     for seg in (1, 3):
         leaves = {k: torch.tensor(out[k], requires_grad=True) for k in ("c11", "c13", "c33", "c55", "bx", "bz")}
@@ -128,8 +136,9 @@ def elastic_case(ek, bc, free_surface, vti, seed):
         res = ek.forward_kernel(nx, nz, dx, dx, nt, dt, nabc, free_surface,
                                 torch.tensor(src_x), torch.tensor(src_z), 2, w, torch.tensor(mt),
                                 torch.tensor(rcv_x), torch.tensor(rcv_z), nx,
-                                "PML", torch.tensor(bcx, dtype=torch.float32),
-                                torch.tensor(bcz, dtype=torch.float32), None,
+                                abc, torch.tensor(bcx, dtype=torch.float32),
+                                torch.tensor(bcz, dtype=torch.float32),
+                                None if abc == "PML" else torch.tensor(damp, dtype=torch.float32),
                                 z, z, leaves["bx"], leaves["bz"], CC, fd_order=4, n_segments=seg)
         loss = sum((res[k] * torch.tensor(wts[i])).sum() for i, k in enumerate(names))
         loss.backward()
@@ -161,12 +170,19 @@ def main():
     ak = _load("_ref_acoustic_kernels", "acoustic_kernels.py")
     ek = _load("_ref_elastic_kernels", "elastic_kernels.py")
     bc = _load("_ref_boundary_condition", "boundary_condition.py")
-    np.savez_compressed(os.path.join(OUT, "acoustic_nofs.npz"), **acoustic_case(ak, bc, False, 11))
-    np.savez_compressed(os.path.join(OUT, "acoustic_fs.npz"), **acoustic_case(ak, bc, True, 12))
-    np.savez_compressed(os.path.join(OUT, "elastic_iso_fs.npz"), **elastic_case(ek, bc, True, False, 21))
-    np.savez_compressed(os.path.join(OUT, "elastic_vti_nofs.npz"), **elastic_case(ek, bc, False, True, 22))
-    np.savez_compressed(os.path.join(OUT, "boundary.npz"), **boundary_case(bc))
-    np.savez_compressed(os.path.join(OUT, "diffcoef.npz"), fdc=ek.DiffCoef(2, "s").numpy())
+    cases = {
+        "acoustic_nofs": lambda: acoustic_case(ak, bc, False, 11),
+        "acoustic_fs": lambda: acoustic_case(ak, bc, True, 12),
+        "elastic_iso_fs": lambda: elastic_case(ek, bc, True, False, 21),
+        "elastic_vti_nofs": lambda: elastic_case(ek, bc, False, True, 22),
+        "elastic_abl_gerjan_fs": lambda: elastic_case(ek, bc, True, False, 23, abc="gerjan"),
+        "elastic_abl_sincos_nofs": lambda: elastic_case(ek, bc, False, True, 24, abc="sincos"),
+        "boundary": lambda: boundary_case(bc),
+        "diffcoef": lambda: dict(fdc=ek.DiffCoef(2, "s").numpy()),
+    }
+    names = sys.argv[1:] or list(cases)   # write only the named cases when given
+    for name in names:
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **cases[name]())
     print("golden vectors written to", OUT, file=sys.stderr)
 
 

@@ -49,6 +49,15 @@ def elastic_cc(leaves, nz, nx):
             leaves["c55"], z, z]
 
 
+def elastic_abc(g):
+    """abc_type of a golden elastic case ('PML' unless the case stores another)."""
+    return str(g["abc"]) if "abc" in g else "PML"
+
+
+def elastic_damp(g):
+    return None if elastic_abc(g) == "PML" else torch.tensor(g["damp"], dtype=torch.float32)
+
+
 def run_elastic_oracle(g, m, seg=1, want_grad=True):
     from oracle import elastic as oe
     leaves = {k: torch.tensor(g[k], requires_grad=want_grad) for k in ELASTIC_LEAVES}
@@ -58,8 +67,8 @@ def run_elastic_oracle(g, m, seg=1, want_grad=True):
     res = oe.forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
                             torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
                             torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
-                            len(g["rcv_x"]), "PML", torch.tensor(g["bcx"], dtype=torch.float32),
-                            torch.tensor(g["bcz"], dtype=torch.float32), None, z, z,
+                            len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
+                            torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z,
                             leaves["bx"], leaves["bz"], elastic_cc(leaves, nz, nx), n_segments=seg)
     out = {k: res[k].detach().numpy() for k in ELASTIC_NAMES}
     out.update({"fw_" + k: res["forward_wavefield_" + k].numpy() for k in ELASTIC_NAMES})
@@ -79,7 +88,7 @@ def _smooth(rng, nz, nx, lo, hi, passes=4):
     return lo + (hi - lo) * a
 
 
-def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=False, dx=5.0, dt=4e-4):
+def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=False, dx=5.0, dt=4e-4, abc="PML"):
     """A golden-shaped elastic case on a larger grid (no reference outputs): smooth fields,
     staggered shapes as the reference's models build them, shots spread over the grid,
     receivers along one row.  aniso adds small C15/C35 fields."""
@@ -114,6 +123,10 @@ def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=Fals
         mt[s] = np.array([[1.0, 0, 0.3 * (s % 3)], [0, 1, 0], [0.3 * (s % 3), 0, 1.0 - 0.5 * (s % 2)]])
     g["mt"] = mt
     g["bcx"], g["bcz"] = ob.bc_pml_xz(nx, nz, dx, dx, pml=nabc, vmax=float(vp.max()), free_surface=free_surface)
+    if abc != "PML":
+        g["abc"] = np.array(abc)
+        g["damp"] = (ob.bc_gerjan(nx, nz, dx, dx, nabc, alpha=0.0053, free_surface=free_surface) if abc == "gerjan"
+                     else ob.bc_sincos(nx, nz, dx, dx, nabc, free_surface=free_surface))
     g["weights"] = rng.standard_normal((5, nshots, nt, len(g["rcv_x"]))).astype(np.float32)
     meta = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, fs=bool(free_surface))
     return g, meta

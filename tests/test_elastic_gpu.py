@@ -6,7 +6,8 @@ import pytest
 import torch
 
 from conftest import rel_l2
-from helpers import ELASTIC_NAMES, elastic_cc_general, elastic_inputs, elastic_random_case
+from helpers import (ELASTIC_NAMES, elastic_abc, elastic_cc_general, elastic_damp, elastic_inputs,
+                     elastic_random_case)
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -26,22 +27,37 @@ def _oracle_setup(g, m, *, shots_per_wave=0, tile_z=0, fdc=None):
     X = nx + 2 * nabc
     zoff = 2 if fs else nabc + 2
     mt = torch.tensor(g["mt"])
-    half = mt / 2
-    src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+    damping = elastic_abc(g) != "PML"
+    if damping:
+        src_m = torch.stack([(-1 / 3) * mt[:, 0, 0], (-1 / 3) * mt[:, 2, 2], (-1 / 3) * mt[:, 0, 2]], dim=1)
+    else:
+        half = mt / 2
+        src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
     aniso = "c15" in g
     geom = ElasticGeometry(Z, X, m["nt"], fs, g["src_z"] + zoff, g["src_x"] + nabc, src_m,
                            g["rcv_z"] + zoff, g["rcv_x"] + nabc, DEV, aniso=aniso,
-                           shots_per_wave=shots_per_wave, tile_z=tile_z)
+                           shots_per_wave=shots_per_wave, tile_z=tile_z, damping=damping)
     fields = [f[k] for k in ("c11", "c13", "c33", "c55", "bx", "bz")]
     fields += [f["c15"], f["c35"]] if aniso else [None, None]
     coef = geom.pack_coef([None if t is None else t.to(DEV) for t in fields])
-    px, pz = pml_factors(f["bcx"], f["bcz"], m["dt"], geom.pitch)
+    if damping:
+        px, pz = geom.pack_plane(oe.pad_elastic(elastic_damp(g), 0, 2, fs).to(DEV)), None
+    else:
+        px, pz = pml_factors(f["bcx"], f["bcz"], m["dt"], geom.pitch)
     fdc = oe.diff_coef(2) if fdc is None else fdc
     f32 = lambda v: float(np.float32(v))
     scalars = (f32(m["dt"]), f32(m["dt"] / m["dx"]), f32(m["dt"] / m["dx"]), f32(m["dx"]), f32(m["dx"]),
                float(fdc[0]), float(fdc[1]))
     src = torch.tensor(g["wavelet"]).to(DEV)
-    return geom, coef, px.to(DEV), pz.to(DEV), src, scalars
+    return geom, coef, px.to(DEV), None if pz is None else pz.to(DEV), src, scalars
+
+
+def _run(geom, coef, px, pz, src, scalars, **kw):
+    """run_forward with the absorbing-boundary tables of either scheme."""
+    from adfwi_amd.propagator.elastic_kernels import run_forward
+    if geom.damping:
+        return run_forward(geom, coef, None, None, src, scalars, damp=px, **kw)
+    return run_forward(geom, coef, px, pz, src, scalars, **kw)
 
 
 LEAVES = ("c11", "c13", "c33", "c55", "bx", "bz", "c15", "c35")
@@ -66,8 +82,8 @@ def _run_oracle(g, m, seg=1, want_grad=False):
     res = oe.forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
                             torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
                             torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
-                            len(g["rcv_x"]), "PML", torch.tensor(g["bcx"], dtype=torch.float32),
-                            torch.tensor(g["bcz"], dtype=torch.float32), None, z, z,
+                            len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
+                            torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z,
                             leaves["bx"], leaves["bz"], cc, n_segments=seg)
     out = {k: res[k].detach().numpy() for k in ELASTIC_NAMES}
     out.update({"fw_" + k: res["forward_wavefield_" + k].numpy() for k in ELASTIC_NAMES})
@@ -89,31 +105,33 @@ def _dropin(g, m, seg=1, want_grad=False, wavelet=None, **leaf_kw):
     res = forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
                          torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
                          torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
-                         len(g["rcv_x"]), "PML", torch.tensor(g["bcx"], dtype=torch.float32),
-                         torch.tensor(g["bcz"], dtype=torch.float32), None, z, z, leaves["bx"], leaves["bz"],
+                         len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
+                         torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z, leaves["bx"], leaves["bz"],
                          cc, fd_order=4, n_segments=seg, device=DEV, **leaf_kw)
     return res, leaves, w
 
 
-@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+ALL_CASES = ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs", "elastic_abl_sincos_nofs"]
+
+
+@pytest.mark.parametrize("name", ALL_CASES)
 def test_kernel_bit_exact_given_oracle_coefficients(name):
     """Same coefficient bits in -> same record bits out: the fused step keeps the reference's
     fp32 association, stage order, split-field PML and free-surface rows.  The FD coefficients
     are the reference's own (a float32 matrix solve whose last bit can depend on the host's
     LAPACK), taken from the fixture."""
-    from adfwi_amd.propagator.elastic_kernels import run_forward
     from conftest import golden
     g, m = elastic_inputs(name)
     fdc = torch.tensor(golden("diffcoef")["fdc"])
     geom, coef, px, pz, src, scalars = _oracle_setup(g, m, fdc=fdc)
-    res = run_forward(geom, coef, px, pz, src, scalars, record=True)
+    res = _run(geom, coef, px, pz, src, scalars, record=True)
     torch.cuda.synchronize()
     for i, k in enumerate(ELASTIC_NAMES):
         got = res["rec"][i].cpu().numpy()
         assert np.array_equal(got, g[f"seg1_rec_{k}"]), (k, rel_l2(got, g[f"seg1_rec_{k}"]))
 
 
-@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+@pytest.mark.parametrize("name", ALL_CASES)
 @pytest.mark.parametrize("seg", [1, 3])
 def test_dropin_forward_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
@@ -128,17 +146,19 @@ def test_dropin_forward_matches_reference_vectors(name, seg):
         assert rel_l2(fw, g[f"seg{seg}_fw_{k}"]) <= 1e-5, ("energy", k)
 
 
-@pytest.mark.parametrize("fs,aniso,spw,tz", [(True, False, 1, 4), (False, False, 2, 8), (True, True, 1, 16),
-                                              (False, True, 0, 0)])
-def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz):
+@pytest.mark.parametrize("fs,aniso,spw,tz,abc", [(True, False, 1, 4, "PML"), (False, False, 2, 8, "PML"),
+                                                  (True, True, 1, 16, "PML"), (False, True, 0, 0, "PML"),
+                                                  (True, False, 1, 4, "gerjan"), (False, True, 2, 8, "sincos"),
+                                                  (True, True, 0, 0, "gerjan")])
+def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz, abc):
     """A grid of several 55-column strips and row tiles, several shots per wavefront: strip
-    edges, tile edges, PML corners and the anisotropic C15/C35 terms against the oracle."""
-    from adfwi_amd.propagator.elastic_kernels import run_forward
-    g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso)
+    edges, tile edges, PML corners, both absorbing schemes and the anisotropic C15/C35 terms
+    against the oracle."""
+    g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso, abc=abc)
     ref = _run_oracle(g, m)
     geom, coef, px, pz, src, scalars = _oracle_setup(g, m, shots_per_wave=spw, tile_z=tz)
     assert geom.layout.ntiles_x >= 3 and geom.layout.ntiles_z >= 2 or tz == 0
-    res = run_forward(geom, coef, px, pz, src, scalars, record=True)
+    res = _run(geom, coef, px, pz, src, scalars, record=True)
     torch.cuda.synchronize()
     for i, k in enumerate(ELASTIC_NAMES):
         got = res["rec"][i].cpu().numpy()
@@ -149,7 +169,7 @@ def _loss(res, weights):
     return sum((res[k] * torch.as_tensor(weights[i], device=res[k].device)).sum() for i, k in enumerate(ELASTIC_NAMES))
 
 
-@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+@pytest.mark.parametrize("name", ALL_CASES)
 @pytest.mark.parametrize("seg", [1, 3])
 def test_dropin_gradients_match_reference_vectors(name, seg):
     """Hand-written adjoint vs the reference's autograd gradients of a seeded linear functional
@@ -166,11 +186,12 @@ def test_dropin_gradients_match_reference_vectors(name, seg):
     assert rel_l2(w.grad.cpu().numpy(), g[f"seg{seg}_grad_wavelet"]) <= 1e-4
 
 
-@pytest.mark.parametrize("fs,aniso", [(True, False), (False, True), (True, True)])
-def test_gradients_multi_strip_match_oracle(fs, aniso):
-    """Several strips and row tiles, several shot groups, anisotropic C15/C35 gradients too,
-    against the oracle's autograd gradients."""
-    g, m = elastic_random_case(44, 150, 8, 60, 9, fs, seed=21 + int(fs) + 2 * int(aniso), aniso=aniso)
+@pytest.mark.parametrize("fs,aniso,abc", [(True, False, "PML"), (False, True, "PML"), (True, True, "PML"),
+                                          (True, False, "gerjan"), (False, True, "sincos"), (True, True, "sincos")])
+def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
+    """Several strips and row tiles, several shot groups, anisotropic C15/C35 gradients too, both
+    absorbing schemes, against the oracle's autograd gradients."""
+    g, m = elastic_random_case(44, 150, 8, 60, 9, fs, seed=21 + int(fs) + 2 * int(aniso), aniso=aniso, abc=abc)
     ref = _run_oracle(g, m, want_grad=True)
     res, leaves, w = _dropin(g, m, want_grad=True)
     _loss(res, g["weights"]).backward()
@@ -181,10 +202,11 @@ def test_gradients_multi_strip_match_oracle(fs, aniso):
     assert rel_l2(w.grad.cpu().numpy(), ref["grad_wavelet"]) <= 1e-4
 
 
-def test_adjoint_dot_product_in_the_wavelet():
+@pytest.mark.parametrize("abc", ["PML", "gerjan"])
+def test_adjoint_dot_product_in_the_wavelet(abc):
     """Records are linear in the wavelet: <J dw, dd> = <dw, J^T dd> with J^T dd the wavelet
     gradient of the hand-written adjoint (free surface and anisotropy on, several tiles)."""
-    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True)
+    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True, abc=abc)
     rng = np.random.default_rng(3)
     dw = rng.standard_normal(g["wavelet"].shape).astype(np.float32)
     dd = rng.standard_normal(g["weights"].shape).astype(np.float32)
@@ -233,6 +255,32 @@ def test_elastic_propagator_class_matches_reference_vectors(name):
         model = IsotropicElasticModel(0, 0, nx, nz, m["dx"], m["dx"], vp=f32("vp"), vs=f32("vs"), rho=f32("rho"),
                                       **common)
     prop = ElasticPropagator(model, Survey(src, rcv), device=DEV)
+    with torch.no_grad():
+        res = prop.forward()
+    for k in ELASTIC_NAMES:
+        got = res[k].cpu().numpy()
+        assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))
+
+
+def test_elastic_propagator_damping_scheme_matches_reference_vectors():
+    """ElasticPropagator with abc_type 'gerjan' (Cerjan taper from the model's alpha, the
+    damping scheme of the reference) reproduces the reference's records of that case."""
+    from adfwi_amd.model import IsotropicElasticModel
+    from adfwi_amd.propagator import ElasticPropagator
+    from adfwi_amd.survey import Receiver, Source, Survey
+    g, m = elastic_inputs("elastic_abl_gerjan_fs")
+    nz, nx = m["nz"], m["nx"]
+    src = Source(m["nt"], m["dt"], 60.0)
+    for s_ in range(len(g["src_x"])):
+        src.add_source(int(g["src_x"][s_]), int(g["src_z"][s_]), g["wavelet"][s_], "mt", g["mt"][s_])
+    rcv = Receiver(m["nt"], m["dt"])
+    rcv.add_receivers(g["rcv_x"], g["rcv_z"], "vz")
+    f32 = lambda k: g[k].astype(np.float32)
+    model = IsotropicElasticModel(0, 0, nx, nz, m["dx"], m["dx"], vp=f32("vp"), vs=f32("vs"), rho=f32("rho"),
+                                  free_surface=m["fs"], nabc=m["nabc"], auto_update_rho=False, abc_type="gerjan",
+                                  device=DEV)
+    prop = ElasticPropagator(model, Survey(src, rcv), device=DEV)
+    assert np.allclose(prop.damp.cpu().numpy(), g["damp"].astype(np.float32))
     with torch.no_grad():
         res = prop.forward()
     for k in ELASTIC_NAMES:

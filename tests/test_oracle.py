@@ -21,7 +21,8 @@ def test_acoustic_oracle_matches_reference_vectors(name, seg):
         assert rel_l2(out["grad_" + k], g[f"seg{seg}_grad_{k}"]) < 1e-6, k
 
 
-@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs",
+                                  "elastic_abl_sincos_nofs"])
 @pytest.mark.parametrize("seg", [1, 3])
 def test_elastic_oracle_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
