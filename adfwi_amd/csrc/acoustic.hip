@@ -53,7 +53,9 @@ __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane
 
 // Raw buffer access: a scalar resource descriptor + a wave-uniform byte offset (SGPR) + the
 // lane's column byte offset (VGPR), so no per-lane 64-bit address arithmetic is needed.
-// Accesses past num_records read zero / are dropped by the hardware.
+// Every access stays inside its buffer by construction: loads use clamped rows and the clamped
+// column (vc), stores use the lane's own column (vs) only on lanes that own an output point.
+// The hardware range check is relied upon only for absent buffers (num_records 0).
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 constexpr int RSRC_DW3 = 0x00020000;  // raw 32-bit data, gfx9 family
 __device__ __forceinline__ rsrc_t rsrc(const void *p, unsigned bytes) {
@@ -151,6 +153,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
   float ck1 = bld(rC, vc, rb(z0 - 1)), ca1 = bld(rC, vc, planeb + rb(z0 - 1));
   float ck2 = bld(rC, vc, 2u * planeb + rb(z0 - 3)), ca2 = bld(rC, vc, 3u * planeb + rb(z0 - 3));
   float ck3 = bld(rC, vc, 4u * planeb + rb(z0 - 3));
+  // energy rows are read-modify-written once per row: their old values are loaded one
+  // iteration ahead too, so that no load is waited for in the iteration that issues it
+  float ce0 = energy ? bld(rE, vc, eb + rb(z0 - 3)) : 0.0f;
+  float ce1 = (energy && want_u) ? bld(rE, vc, eb + planeb + rb(z0 - 3)) : 0.0f;
 
 #pragma unroll UNR
   for (int kk = 0; kk < d.tz + 3; ++kk) {
@@ -170,6 +176,8 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
     const float nk1 = bld(rC, vc, rn), na1 = bld(rC, vc, planeb + rn);
     const float nk2 = bld(rC, vc, 2u * planeb + rrn), na2 = bld(rC, vc, 3u * planeb + rrn);
     const float nk3 = bld(rC, vc, 4u * planeb + rrn);
+    const float ne0 = energy ? bld(rE, vc, eb + rrn) : 0.0f;
+    const float ne1 = (energy && want_u) ? bld(rE, vc, eb + planeb + rrn) : 0.0f;
 
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
     const bool rowu = cu && r >= fs && r <= d.Z - 2;
@@ -223,9 +231,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
     }
     ck1 = nk1; ca1 = na1; ck2 = nk2; ca2 = na2; ck3 = nk3;
     if (energy && wout) {
-      bst(bld(rE, vs, eb + rr) + ep, rE, vs, eb + rr);
-      if (want_u) bst(bld(rE, vs, eb + planeb + rr) + eu, rE, vs, eb + planeb + rr);
+      bst(ce0 + ep, rE, vs, eb + rr);
+      if (want_u) bst(ce1 + eu, rE, vs, eb + planeb + rr);
     }
+    ce0 = ne0; ce1 = ne1;
   }
 }
 
@@ -241,6 +250,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
 // on each side (3 per step), so a wavefront writes 52 of its 64 columns.
 constexpr int HALO2 = 6;
 constexpr int SX2 = WAVE - 2 * HALO2;   // 52
+
+// feature sets of the two-step forward kernel: records and energy maps, stencil images,
+// pressure images
+constexpr int PF_REC = 1, PF_IMGD = 2, PF_IMGP = 4;
 
 struct PairIO {
   const float *in;
@@ -266,10 +279,17 @@ __device__ __forceinline__ bool strip_of2(const AcDev &d, int &z0, int &x0) {
   return true;
 }
 
-template <int SPB>
+template <int SPB, int F>
 __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   int z0, x0;
   if (!strip_of2(d, z0, x0)) return;
+  // features compiled out are null here, so their descriptors and branches fold away and
+  // the scalar registers they would hold stay free
+  constexpr bool FR = (F & PF_REC) != 0, FD = (F & PF_IMGD) != 0, FP = (F & PF_IMGP) != 0;
+  float *const energy = FR ? io.energy : nullptr;
+  float *const rec_p = FR ? io.rec_p : nullptr;
+  float *const img_d0 = FD ? io.img_d0 : nullptr, *const img_d1 = FD ? io.img_d1 : nullptr;
+  float *const img_p0 = FP ? io.img_p0 : nullptr, *const img_p1 = FP ? io.img_p1 : nullptr;
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - HALO2 + lane;
   const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
@@ -291,11 +311,11 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const rsrc_t rP = rsrc(io.in, fieldb), rU = rsrc(io.in + d.field, fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
   const rsrc_t oP = rsrc(io.out, fieldb), oU = rsrc(io.out + d.field, fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
   const rsrc_t rC = rsrc(d.coef, 5u * planeb);
-  const rsrc_t rI0 = rsrc(io.img_d0, io.img_d0 ? fieldb : 0u), rI1 = rsrc(io.img_d1, io.img_d1 ? fieldb : 0u);
-  const rsrc_t rJ0 = rsrc(io.img_p0, io.img_p0 ? fieldb : 0u), rJ1 = rsrc(io.img_p1, io.img_p1 ? fieldb : 0u);
-  const rsrc_t rE = rsrc(io.energy, io.energy ? 2u * planeb * gridDim.y : 0u);
+  const rsrc_t rI0 = rsrc(img_d0, img_d0 ? fieldb : 0u), rI1 = rsrc(img_d1, img_d1 ? fieldb : 0u);
+  const rsrc_t rJ0 = rsrc(img_p0, img_p0 ? fieldb : 0u), rJ1 = rsrc(img_p1, img_p1 ? fieldb : 0u);
+  const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
-  const bool recs = io.rec_p != nullptr && d.nrec > 0;
+  const bool recs = rec_p != nullptr && d.nrec > 0;
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -344,6 +364,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   float bk3 = bld(rC, vc, 4u * planeb + rb(kst - 5));
   // energy of A's emitted rows, waiting for B's rows three iterations later
   float epa0 = 0.f, epa1 = 0.f, epa2 = 0.f, eua0 = 0.f, eua1 = 0.f, eua2 = 0.f;
+  // old energy of row qq, loaded one iteration ahead (read-modify-write without a wait)
+  const bool want_eu = io.want_u0 || io.want_u1;
+  float ce0 = energy ? bld(rE, vc, eb + rb(kst - 5)) : 0.0f;
+  float ce1 = (energy && want_eu) ? bld(rE, vc, eb + planeb + rb(kst - 5)) : 0.0f;
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -365,6 +389,8 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const float nbk1 = bld(rC, vc, rb(q + 1)), nba1 = bld(rC, vc, planeb + rb(q + 1));
     const float nbk2 = bld(rC, vc, 2u * planeb + rb(qq + 1)), nba2 = bld(rC, vc, 3u * planeb + rb(qq + 1));
     const float nbk3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+    const float ne0 = energy ? bld(rE, vc, eb + rb(qq + 1)) : 0.0f;
+    const float ne1 = (energy && want_eu) ? bld(rE, vc, eb + planeb + rb(qq + 1)) : 0.0f;
 
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
     const bool rowu = cu && r >= fs && r <= d.Z - 2;
@@ -399,7 +425,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
         if (k == zs[j] && c == xs[j]) pn = pn + sv0[j];
         pw0[j] = pw1[j]; pw1[j] = pw2[j]; pw2[j] = pw3[j]; pw3[j] = pn;
         if (d.fsurf && k == fs + 1) pw1[j] = -pn;
-        if (io.img_d0 && own_k) bst(dd, rI0, vs, so[j] + rk);
+        if (img_d0 && own_k) bst(dd, rI0, vs, so[j] + rk);
         // ---- step A: u', w' at row r; emit (p', u', w') of row r
         const float pr = pw1[j];
         const float pl1 = shfl(pr, lane - 1), pr1 = shfl(pr, lane + 1), pr2 = shfl(pr, lane + 2);
@@ -408,13 +434,13 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
         if (roww) wn = ak3 * wn - aa2 * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
         if (!u_ok) un = 0.0f;
         if (emit && own_r) {
-          if (io.img_p0) bst(pr, rJ0, vs, so[j] + rr);
+          if (img_p0) bst(pr, rJ0, vs, so[j] + rr);
           epa += pr * pr;
           eua += un * un;
         }
         if (ridx >= 0) {
           const long long ro = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
-          io.rec_p[ro] = pr;
+          rec_p[ro] = pr;
           io.rec_u[ro] = un;
           io.rec_w[ro] = (d.fsurf && r == fs - 1) ? 0.0f : wn;   // row fs-1 is not sampled here
         }
@@ -439,7 +465,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
           if (q == zs[j] && c == xs[j]) pn2 = pn2 + sv1[j];
           qw0[j] = qw1[j]; qw1[j] = qw2[j]; qw2[j] = qw3[j]; qw3[j] = pn2;
           if (d.fsurf && q == fs + 1) qw1[j] = -pn2;
-          if (io.img_d1 && own_q) bst(dd2, rI1, vs, so[j] + rq);
+          if (img_d1 && own_q) bst(dd2, rI1, vs, so[j] + rq);
         }
         // ---- step B: u'', w'' at row qq = k-5; store the final rows
         {
@@ -450,7 +476,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
           if (roww2) wn2 = bk3 * wn2 - ba2 * (C1 * (qw2[j] - ps) + C2 * (qw3[j] - qw0[j]));
           if (bstep && own_qq) {
             bst(ps, oP, vs, so[j] + rqq);
-            if (io.img_p1) bst(ps, rJ1, vs, so[j] + rqq);
+            if (img_p1) bst(ps, rJ1, vs, so[j] + rqq);
             if (u_ok) bst(un2, oU, vs, so[j] + rqq); else un2 = 0.0f;
             if (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) bst(wn2, oW, vs, so[j] + rqq);
             if (d.fsurf && qq == fs) bst(wn2, oW, vs, so[j] + rqq - rowb);
@@ -464,12 +490,13 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     ak1 = nak1; aa1 = naa1; ak2 = nak2; aa2 = naa2; ak3 = nak3;
     bk1 = nbk1; ba1 = nba1; bk2 = nbk2; ba2 = nba2; bk3 = nbk3;
     // energy: A's row r waits three iterations for B's row qq = r-3
-    if (io.energy && bstep && own_qq) {
+    if (energy && bstep && own_qq) {
       const float ep = epa0 + epb;
-      bst(bld(rE, vs, eb + rqq) + ep, rE, vs, eb + rqq);
+      bst(ce0 + ep, rE, vs, eb + rqq);
       const float eu = (io.want_u0 ? eua0 : 0.0f) + (io.want_u1 ? eub : 0.0f);
-      if (io.want_u0 || io.want_u1) bst(bld(rE, vs, eb + planeb + rqq) + eu, rE, vs, eb + planeb + rqq);
+      if (want_eu) bst(ce1 + eu, rE, vs, eb + planeb + rqq);
     }
+    ce0 = ne0; ce1 = ne1;
     epa0 = epa1; epa1 = epa2; epa2 = epa;
     eua0 = eua1; eua1 = eua2; eua2 = eua;
   }
@@ -563,6 +590,10 @@ __global__ __launch_bounds__(NT) void ac_adj_step(
   float cA1 = bld(rC, vc, planeb + rb(z0 - 1));
   float cK1 = bld(rC, vc, rb(z0 - 3)), cK2 = bld(rC, vc, 2u * planeb + rb(z0 - 3));
   float cK3 = bld(rC, vc, 4u * planeb + rb(z0 - 3));
+  // old gradient-plane values of rows k (A1) and r (A2), loaded one iteration ahead so that the
+  // read-modify-write waits for no load issued in the same iteration
+  float cG1 = want_a1 ? bld(rG1, vc, gb + rb(z0 - 1)) : 0.0f;
+  float cG2 = want_a2 ? bld(rG2, vc, gb + rb(z0 - 3)) : 0.0f;
 
 #pragma unroll UNR
   for (int kk = 0; kk < d.tz + 3; ++kk) {
@@ -584,6 +615,8 @@ __global__ __launch_bounds__(NT) void ac_adj_step(
     const float nA2w = wmask(k + 2) ? bld(rC, vc, 3u * planeb + rn1) : 0.0f;
     const float nA2u = bld(rC, vc, 3u * planeb + rn), nA1 = bld(rC, vc, planeb + rn);
     const float nK1 = bld(rC, vc, rrn), nK2 = bld(rC, vc, 2u * planeb + rrn), nK3 = bld(rC, vc, 4u * planeb + rrn);
+    const float nG1 = want_a1 ? bld(rG1, vc, gb + rn) : 0.0f;
+    const float nG2 = want_a2 ? bld(rG2, vc, gb + rrn) : 0.0f;
 
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
     const bool rowu_k = cu && k >= fs && k <= d.Z - 2;
@@ -649,8 +682,9 @@ __global__ __launch_bounds__(NT) void ac_adj_step(
     }
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
     // image terms of row k (stencil image) and row r (pressure image) into the group planes
-    if (want_a1 && own && rowp) bst(bld(rG1, vs, gb + rk) + ga1, rG1, vs, gb + rk);
-    if (want_a2 && wout) bst(bld(rG2, vs, gb + rr) + ga2, rG2, vs, gb + rr);
+    if (want_a1 && own && rowp) bst(cG1 + ga1, rG1, vs, gb + rk);
+    if (want_a2 && wout) bst(cG2 + ga2, rG2, vs, gb + rr);
+    cG1 = nG1; cG2 = nG2;
   }
 }
 
@@ -775,6 +809,9 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   float dK3 = bld(rC, vc, 4u * planeb + rb(qst - 2));
   // B's image terms of rows k and r waiting three iterations for A's rows q and qq
   float ga1d0 = 0.f, ga1d1 = 0.f, ga1d2 = 0.f, ga2d0 = 0.f, ga2d1 = 0.f, ga2d2 = 0.f;
+  // old gradient-plane values of rows q (A1) and qq (A2), loaded one iteration ahead
+  float cG1 = want_a1 ? bld(rG1, vc, gb + rb(qst)) : 0.0f;
+  float cG2 = A2 ? bld(rG2, vc, gb + rb(kst - 5)) : 0.0f;
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -802,6 +839,8 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float mA2u = bld(rC, vc, 3u * planeb + rb(q + 1)), mA1 = bld(rC, vc, planeb + rb(q + 1));
     const float mK1 = bld(rC, vc, rb(qq + 1)), mK2 = bld(rC, vc, 2u * planeb + rb(qq + 1));
     const float mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+    const float nG1 = want_a1 ? bld(rG1, vc, gb + rb(q + 1)) : 0.0f;
+    const float nG2 = A2 ? bld(rG2, vc, gb + rb(qq + 1)) : 0.0f;
 
     // step B masks (rows k, r) and step A masks (rows q, qq)
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
@@ -917,8 +956,9 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
     dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
     // image terms: B's row k waits three iterations for A's row q, B's row r for A's row qq
-    if (want_a1 && own_q && rowp_q) bst(bld(rG1, vs, gb + rq) + (ga1d0 + ga1a), rG1, vs, gb + rq);
-    if (A2 && own_qq) bst(bld(rG2, vs, gb + rqq) + (ga2d0 + ga2a), rG2, vs, gb + rqq);
+    if (want_a1 && own_q && rowp_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
+    if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
+    cG1 = nG1; cG2 = nG2;
     ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
     ga2d0 = ga2d1; ga2d1 = ga2d2; ga2d2 = ga2b;
   }
@@ -1075,12 +1115,23 @@ AcDev pair_dev(const AcDev &v) {
   return w;
 }
 
-void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
+template <int F>
+void launch_pair_f(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
   switch (w.spb) {
-    case 4: hipLaunchKernelGGL((ac_fwd_pair<4>), g, b, 0, st, w, io); break;
-    case 2: hipLaunchKernelGGL((ac_fwd_pair<2>), g, b, 0, st, w, io); break;
-    default: hipLaunchKernelGGL((ac_fwd_pair<1>), g, b, 0, st, w, io);
+    case 4: hipLaunchKernelGGL((ac_fwd_pair<4, F>), g, b, 0, st, w, io); break;
+    case 2: hipLaunchKernelGGL((ac_fwd_pair<2, F>), g, b, 0, st, w, io); break;
+    default: hipLaunchKernelGGL((ac_fwd_pair<1, F>), g, b, 0, st, w, io);
   }
+}
+
+// the variant without the features this launch does not use (fewer live scalar registers)
+void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
+  const bool rec = io.energy || io.rec_p;
+  const bool imd = io.img_d0 || io.img_d1, imp = io.img_p0 || io.img_p1;
+  if (!imd && !imp) launch_pair_f<PF_REC>(w, g, b, st, io);
+  else if (!rec && !imp) launch_pair_f<PF_IMGD>(w, g, b, st, io);
+  else if (!rec) launch_pair_f<PF_IMGD | PF_IMGP>(w, g, b, st, io);
+  else launch_pair_f<PF_REC | PF_IMGD | PF_IMGP>(w, g, b, st, io);
 }
 
 template <bool A2>
