@@ -709,10 +709,14 @@ struct AdjPairIO {
   const float *g_p, *g_u, *g_w;          // record adjoints [S][nt][R]; step t's are added here
 };
 
-template <int SPB, bool A2>
+template <int SPB, bool A2, bool FULL>
 __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   int z0, x0;
   if (!strip_of2(d, z0, x0)) return;
+  // the lean variant (FULL = false) has no wavelet gradient and pressure record adjoints only:
+  // the pointers it does not use are null constants, so they hold no scalar registers
+  float *const grad_src = FULL ? io.grad_src : nullptr;
+  const float *const g_u = FULL ? io.g_u : nullptr, *const g_w = FULL ? io.g_w : nullptr;
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - HALO2 + lane;
   const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
@@ -884,7 +888,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         float l2 = l3;
         if (d.fsurf && k == fs + 1) { l2 = l3 - l2a[j]; l2a[j] = 0.0f; }
         l2c[j] = l2;
-        if (io.grad_src && own_k && k == zs[j] && c == xs[j]) io.grad_src[(long long)(s0 + j) * d.nt + t + 1] = l2;
+        if (grad_src && own_k && k == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t + 1] = l2;
         g3[j] = a1k * l2;
         if (want_a1 && own_k && rowp) ga1b -= l2 * cI[j];
         // outputs of B at row r: the adjoint state after step t (plus step t's sources)
@@ -906,8 +910,8 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         if (ridx >= 0) {
           const long long gi = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
           if (io.g_p) op = op + io.g_p[gi];
-          if (io.g_u) ou = ou + io.g_u[gi];
-          if (io.g_w) ow = ow + io.g_w[gi];
+          if (g_u) ou = ou + g_u[gi];
+          if (g_w) ow = ow + g_w[gi];
         }
         // ================= step A (t) at row q = k-3 over B's rows
         const float wprev = eW[j];
@@ -925,7 +929,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         float m2 = m3;
         if (d.fsurf && q == fs + 1) { m2 = m3 - m2a[j]; m2a[j] = 0.0f; }
         m2c[j] = m2;
-        if (io.grad_src && own_q && q == zs[j] && c == xs[j]) io.grad_src[(long long)(s0 + j) * d.nt + t] = m2;
+        if (grad_src && own_q && q == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t] = m2;
         h3[j] = a1q * m2;
         if (want_a1 && own_q && rowp_q) ga1a -= m2 * dI[j];
         {
@@ -1134,20 +1138,22 @@ void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &i
   else launch_pair_f<PF_REC | PF_IMGD | PF_IMGP>(w, g, b, st, io);
 }
 
-template <bool A2>
+template <bool A2, bool FULL>
 void launch_adj_pair_a(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
   switch (w.spb) {
-    case 4: hipLaunchKernelGGL((ac_adj_pair<4, A2>), g, b, 0, st, w, io); break;
-    case 2: hipLaunchKernelGGL((ac_adj_pair<2, A2>), g, b, 0, st, w, io); break;
-    default: hipLaunchKernelGGL((ac_adj_pair<1, A2>), g, b, 0, st, w, io);
+    case 4: hipLaunchKernelGGL((ac_adj_pair<4, A2, FULL>), g, b, 0, st, w, io); break;
+    case 2: hipLaunchKernelGGL((ac_adj_pair<2, A2, FULL>), g, b, 0, st, w, io); break;
+    default: hipLaunchKernelGGL((ac_adj_pair<1, A2, FULL>), g, b, 0, st, w, io);
   }
 }
 
 void launch_adj_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
-  if (io.grad_a2 && io.img_p0 && io.img_p1)
-    launch_adj_pair_a<true>(w, g, b, st, io);
+  const bool full = io.grad_src || io.g_u || io.g_w;
+  const bool a2 = io.grad_a2 && io.img_p0 && io.img_p1;
+  if (a2)
+    full ? launch_adj_pair_a<true, true>(w, g, b, st, io) : launch_adj_pair_a<true, false>(w, g, b, st, io);
   else
-    launch_adj_pair_a<false>(w, g, b, st, io);
+    full ? launch_adj_pair_a<false, true>(w, g, b, st, io) : launch_adj_pair_a<false, false>(w, g, b, st, io);
 }
 
 template <int SPB>
