@@ -261,7 +261,7 @@ struct PairIO {
   int t;
   int want_u0, want_u1;                 // u^2 enters the energy map for step t / t+1
   float *energy;
-  float *img_d0, *img_d1, *img_p0, *img_p1;
+  float *img_d0, *img_d1, *img_p0, *img_p1;   // *_1 = *_0 + one field (consecutive steps)
   const int *rmap;                      // [Z] receivers per row, then [Z][pitch] receiver index or -1
   float *rec_p, *rec_u, *rec_w;         // records; step t is written here
 };
@@ -288,8 +288,8 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   constexpr bool FR = (F & PF_REC) != 0, FD = (F & PF_IMGD) != 0, FP = (F & PF_IMGP) != 0;
   float *const energy = FR ? io.energy : nullptr;
   float *const rec_p = FR ? io.rec_p : nullptr;
-  float *const img_d0 = FD ? io.img_d0 : nullptr, *const img_d1 = FD ? io.img_d1 : nullptr;
-  float *const img_p0 = FP ? io.img_p0 : nullptr, *const img_p1 = FP ? io.img_p1 : nullptr;
+  float *const img_d0 = FD ? io.img_d0 : nullptr, *const img_p0 = FP ? io.img_p0 : nullptr;
+  const bool img_d1 = img_d0 != nullptr, img_p1 = img_p0 != nullptr;
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - HALO2 + lane;
   const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
@@ -308,11 +308,14 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const int end_row = min(z0 + d.tz, d.Z);
   const int t = io.t;
   auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
-  const rsrc_t rP = rsrc(io.in, fieldb), rU = rsrc(io.in + d.field, fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
-  const rsrc_t oP = rsrc(io.out, fieldb), oU = rsrc(io.out + d.field, fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
+  // one descriptor spans two consecutive fields (p and u; the images of steps t and t+1): the
+  // second field is reached through the lane offset (vc1 / vs1), which keeps scalar registers
+  // free.  A field is under 2 GiB, so two fit the 32-bit offsets.
+  const unsigned vc1 = vc + fieldb, vs1 = vs + fieldb;
+  const rsrc_t rPU = rsrc(io.in, 2u * fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
+  const rsrc_t oPU = rsrc(io.out, 2u * fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
   const rsrc_t rC = rsrc(d.coef, 5u * planeb);
-  const rsrc_t rI0 = rsrc(img_d0, img_d0 ? fieldb : 0u), rI1 = rsrc(img_d1, img_d1 ? fieldb : 0u);
-  const rsrc_t rJ0 = rsrc(img_p0, img_p0 ? fieldb : 0u), rJ1 = rsrc(img_p1, img_p1 ? fieldb : 0u);
+  const rsrc_t rI = rsrc(img_d0, img_d0 ? 2u * fieldb : 0u), rJ = rsrc(img_p0, img_p0 ? 2u * fieldb : 0u);
   const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
   const bool recs = rec_p != nullptr && d.nrec > 0;
@@ -332,8 +335,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   float bu0[SPB], bu1[SPB], bu2[SPB], bu3[SPB];
   float bpo[SPB];
   float qw0[SPB], qw1[SPB], qw2[SPB], qw3[SPB];
-  float sv0[SPB], sv1[SPB];
-  int zs[SPB], xs[SPB];
+  // sources: the row of each shot (uniform) and a per-lane bit field of the shots whose source
+  // column is this lane; the wavelet sample is read on the source row only
+  int zs[SPB];
+  unsigned srcb = 0u;
   const int kst = z0 - 4;
 #pragma unroll
   for (int j = 0; j < SPB; ++j) {
@@ -345,15 +350,13 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     qw0[j] = qw1[j] = qw2[j] = qw3[j] = 0.0f;
     const int s = s0 + min(j, ns - 1);
     zs[j] = j < ns ? d.src_z[s] : -1000;
-    xs[j] = d.src_x[s];
-    sv0[j] = d.src[(long long)s * d.nt + t];
-    sv1[j] = d.src[(long long)s * d.nt + t + 1];
+    if (c == d.src_x[s]) srcb |= 1u << j;
     ww0[j] = bld(rW, vc, so[j] + rb(kst - 2));
     ww1[j] = bld(rW, vc, so[j] + rb(kst - 1));
     ww2[j] = bld(rW, vc, so[j] + rb(kst));
     cW[j] = bld(rW, vc, so[j] + rb(kst + 1));
-    cU[j] = bld(rU, vc, so[j] + rb(kst));
-    cP[j] = bld(rP, vc, so[j] + rb(kst));
+    cU[j] = bld(rPU, vc1, so[j] + rb(kst));
+    cP[j] = bld(rPU, vc, so[j] + rb(kst));
   }
   // coefficients: A at rows k (1-K1, A1) and k-2 (1-K2, A2, 1-K3); B at rows k-3 and k-5
   float ak1 = bld(rC, vc, rb(kst)), aa1 = bld(rC, vc, planeb + rb(kst));
@@ -380,8 +383,8 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
       nW[j] = bld(rW, vc, so[j] + rb(k + 2));
-      nU[j] = bld(rU, vc, so[j] + rb(k + 1));
-      nP[j] = bld(rP, vc, so[j] + rb(k + 1));
+      nU[j] = bld(rPU, vc1, so[j] + rb(k + 1));
+      nP[j] = bld(rPU, vc, so[j] + rb(k + 1));
     }
     const float nak1 = bld(rC, vc, rb(k + 1)), naa1 = bld(rC, vc, planeb + rb(k + 1));
     const float nak2 = bld(rC, vc, 2u * planeb + rb(r + 1)), naa2 = bld(rC, vc, 3u * planeb + rb(r + 1));
@@ -402,6 +405,16 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const bool rowu2 = cu && qq >= fs && qq <= d.Z - 2;
     const bool roww2 = cw && qq >= fs && qq <= d.Z - 3;
     const bool bstep = kk >= 6;                                      // B's p'' rows q-3..q are complete
+    // Masked updates without branches: on the points a stage leaves alone its coefficients
+    // become (1, 0), and 1*x - 0*d returns x (u is zeroed past column X-2, as the stage
+    // leaves it).  On updated points the expression and its rounding are unchanged.
+    const float pa = rowp ? ak1 : 1.0f, pb = rowp ? aa1 : 0.0f;
+    const float ua = rowu ? ak2 : (u_ok ? 1.0f : 0.0f), ub = rowu ? aa2 : 0.0f;
+    const float wa = roww ? ak3 : 1.0f, wb = roww ? aa2 : 0.0f;
+    const float pa2 = rowp2 ? bk1 : 1.0f, pb2 = rowp2 ? ba1 : 0.0f;
+    const float ua2 = rowu2 ? bk2 : (u_ok ? 1.0f : 0.0f), ub2 = rowu2 ? ba2 : 0.0f;
+    const float wa2 = roww2 ? bk3 : 1.0f, wb2 = roww2 ? ba2 : 0.0f;
+    const float dm = rowp ? 1.0f : 0.0f, dm2 = rowp2 ? 1.0f : 0.0f;   // stencil-image masks
     const bool own_q = out_col && q >= z0 && q < end_row;
     const bool own_qq = out_col && qq >= z0 && qq < end_row;
     // receiver slots of A's emitted row (row counts first: uniform test, then the lane's index)
@@ -417,27 +430,23 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
         float pn = cP[j];
         const float u0 = cU[j];
         const float um2 = shfl(u0, lane - 2), um1 = shfl(u0, lane - 1), up1 = shfl(u0, lane + 1);
-        float dd = 0.0f;
-        if (rowp) {
-          dd = C1 * (((u0 - um1) + ww2[j]) - ww1[j]) + C2 * (((up1 - um2) + w3) - ww0[j]);
-          pn = ak1 * pn - aa1 * dd;
+        const float dd = C1 * (((u0 - um1) + ww2[j]) - ww1[j]) + C2 * (((up1 - um2) + w3) - ww0[j]);
+        pn = pa * pn - pb * dd;
+        if (k == zs[j]) {
+          const float sv = d.src[(long long)(s0 + j) * d.nt + t];
+          if ((srcb >> j) & 1u) pn = pn + sv;
         }
-        if (k == zs[j] && c == xs[j]) pn = pn + sv0[j];
         pw0[j] = pw1[j]; pw1[j] = pw2[j]; pw2[j] = pw3[j]; pw3[j] = pn;
         if (d.fsurf && k == fs + 1) pw1[j] = -pn;
-        if (img_d0 && own_k) bst(dd, rI0, vs, so[j] + rk);
+        if (img_d0 && own_k) bst(dd * dm, rI, vs, so[j] + rk);
         // ---- step A: u', w' at row r; emit (p', u', w') of row r
         const float pr = pw1[j];
         const float pl1 = shfl(pr, lane - 1), pr1 = shfl(pr, lane + 1), pr2 = shfl(pr, lane + 2);
-        float un = uw0[j], wn = ww0[j];
-        if (rowu) un = ak2 * un - aa2 * (C1 * (pr1 - pr) + C2 * (pr2 - pl1));
-        if (roww) wn = ak3 * wn - aa2 * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
-        if (!u_ok) un = 0.0f;
-        if (emit && own_r) {
-          if (img_p0) bst(pr, rJ0, vs, so[j] + rr);
-          epa += pr * pr;
-          eua += un * un;
-        }
+        const float un = ua * uw0[j] - ub * (C1 * (pr1 - pr) + C2 * (pr2 - pl1));
+        const float wn = wa * ww0[j] - wb * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
+        if (img_p0 && emit && own_r) bst(pr, rJ, vs, so[j] + rr);
+        epa += pr * pr;                     // stored for owned points only
+        eua += un * un;
         if (ridx >= 0) {
           const long long ro = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
           rec_p[ro] = pr;
@@ -456,33 +465,31 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
         {
           const float v0 = bu2[j];                           // u'(q)
           const float vm2 = shfl(v0, lane - 2), vm1 = shfl(v0, lane - 1), vp1 = shfl(v0, lane + 1);
-          float pn2 = p_old;
-          float dd2 = 0.0f;
-          if (rowp2) {
-            dd2 = C1 * (((v0 - vm1) + bw2[j]) - bw1[j]) + C2 * (((vp1 - vm2) + bw3[j]) - bw0[j]);
-            pn2 = bk1 * pn2 - ba1 * dd2;
+          const float dd2 = C1 * (((v0 - vm1) + bw2[j]) - bw1[j]) + C2 * (((vp1 - vm2) + bw3[j]) - bw0[j]);
+          float pn2 = pa2 * p_old - pb2 * dd2;
+          if (q == zs[j]) {
+            const float sv = d.src[(long long)(s0 + j) * d.nt + t + 1];
+            if ((srcb >> j) & 1u) pn2 = pn2 + sv;
           }
-          if (q == zs[j] && c == xs[j]) pn2 = pn2 + sv1[j];
           qw0[j] = qw1[j]; qw1[j] = qw2[j]; qw2[j] = qw3[j]; qw3[j] = pn2;
           if (d.fsurf && q == fs + 1) qw1[j] = -pn2;
-          if (img_d1 && own_q) bst(dd2, rI1, vs, so[j] + rq);
+          if (img_d1 && own_q) bst(dd2 * dm2, rI, vs1, so[j] + rq);
         }
         // ---- step B: u'', w'' at row qq = k-5; store the final rows
         {
           const float ps = qw1[j];
           const float sl1 = shfl(ps, lane - 1), sr1 = shfl(ps, lane + 1), sr2 = shfl(ps, lane + 2);
-          float un2 = bu0[j], wn2 = bw0[j];
-          if (rowu2) un2 = bk2 * un2 - ba2 * (C1 * (sr1 - ps) + C2 * (sr2 - sl1));
-          if (roww2) wn2 = bk3 * wn2 - ba2 * (C1 * (qw2[j] - ps) + C2 * (qw3[j] - qw0[j]));
+          const float un2 = ua2 * bu0[j] - ub2 * (C1 * (sr1 - ps) + C2 * (sr2 - sl1));
+          const float wn2 = wa2 * bw0[j] - wb2 * (C1 * (qw2[j] - ps) + C2 * (qw3[j] - qw0[j]));
           if (bstep && own_qq) {
-            bst(ps, oP, vs, so[j] + rqq);
-            if (img_p1) bst(ps, rJ1, vs, so[j] + rqq);
-            if (u_ok) bst(un2, oU, vs, so[j] + rqq); else un2 = 0.0f;
+            bst(ps, oPU, vs, so[j] + rqq);
+            if (img_p1) bst(ps, rJ, vs1, so[j] + rqq);
+            if (u_ok) bst(un2, oPU, vs1, so[j] + rqq);
             if (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) bst(wn2, oW, vs, so[j] + rqq);
             if (d.fsurf && qq == fs) bst(wn2, oW, vs, so[j] + rqq - rowb);
-            epb += ps * ps;
-            eub += un2 * un2;
           }
+          epb += ps * ps;                   // stored for owned points only
+          eub += un2 * un2;
         }
       }
       cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j];
@@ -702,8 +709,8 @@ struct AdjPairIO {
   const float *in;
   float *out;
   int t;                                 // step A; step B is t+1
-  const float *img_d0, *img_d1;          // stencil images of steps t and t+1
-  const float *img_p0, *img_p1;          // pressure images (A2 gradient)
+  const float *img_d0, *img_d1;          // stencil images of steps t and t+1 (d1 = d0 + one field)
+  const float *img_p0, *img_p1;          // pressure images (A2 gradient), p1 = p0 + one field
   float *grad_a1, *grad_a2, *grad_src;
   const int *rmap;
   const float *g_p, *g_u, *g_w;          // record adjoints [S][nt][R]; step t's are added here
@@ -735,13 +742,15 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   const int fsr = d.fsurf ? fs : -100;   // row whose adjoint collects the free-surface copy
   const int end_row = min(z0 + d.tz, d.Z);
   const int t = io.t;
-  const bool want_a1 = io.grad_a1 != nullptr && io.img_d0 != nullptr && io.img_d1 != nullptr;
+  const bool want_a1 = io.grad_a1 != nullptr && io.img_d0 != nullptr;
   auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
-  const rsrc_t rP = rsrc(io.in, fieldb), rU = rsrc(io.in + d.field, fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
-  const rsrc_t oP = rsrc(io.out, fieldb), oU = rsrc(io.out + d.field, fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
+  // one descriptor spans two consecutive fields (lambda_p and lambda_u; the images of steps t
+  // and t+1), the second reached through the lane offset (vc1 / vs1)
+  const unsigned vc1 = vc + fieldb, vs1 = vs + fieldb;
+  const rsrc_t rPU = rsrc(io.in, 2u * fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
+  const rsrc_t oPU = rsrc(io.out, 2u * fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
   const rsrc_t rC = rsrc(d.coef, 5u * planeb);
-  const rsrc_t rI0 = rsrc(io.img_d0, want_a1 ? fieldb : 0u), rI1 = rsrc(io.img_d1, want_a1 ? fieldb : 0u);
-  const rsrc_t rJ0 = rsrc(io.img_p0, A2 ? fieldb : 0u), rJ1 = rsrc(io.img_p1, A2 ? fieldb : 0u);
+  const rsrc_t rI = rsrc(io.img_d0, want_a1 ? 2u * fieldb : 0u), rJ = rsrc(io.img_p0, A2 ? 2u * fieldb : 0u);
   const rsrc_t rG1 = rsrc(io.grad_a1, want_a1 ? planeb * gridDim.y : 0u);
   const rsrc_t rG2 = rsrc(io.grad_a2, A2 ? planeb * gridDim.y : 0u);
   const unsigned gb = planeb * blockIdx.y;
@@ -785,12 +794,12 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     lw2[j] = lwfix(so[j], kst - 1, bld(rW, vc, so[j] + rb(kst - 1)));
     lw3[j] = lwfix(so[j], kst, bld(rW, vc, so[j] + rb(kst)));
     cW[j] = bld(rW, vc, so[j] + rb(kst + 1));
-    cU[j] = bld(rU, vc, so[j] + rb(kst));
-    cP[j] = bld(rP, vc, so[j] + rb(kst));
-    cI[j] = want_a1 ? bld(rI1, vc, so[j] + rb(kst)) : 0.0f;
-    cIp[j] = A2 ? bld(rJ1, vc, so[j] + rb(kst)) : 0.0f;
-    dI[j] = want_a1 ? bld(rI0, vc, so[j] + rb(kst - 3)) : 0.0f;
-    dIp[j] = A2 ? bld(rJ0, vc, so[j] + rb(kst - 3)) : 0.0f;
+    cU[j] = bld(rPU, vc1, so[j] + rb(kst));
+    cP[j] = bld(rPU, vc, so[j] + rb(kst));
+    cI[j] = want_a1 ? bld(rI, vc1, so[j] + rb(kst)) : 0.0f;
+    cIp[j] = A2 ? bld(rJ, vc1, so[j] + rb(kst)) : 0.0f;
+    dI[j] = want_a1 ? bld(rI, vc, so[j] + rb(kst - 3)) : 0.0f;
+    dIp[j] = A2 ? bld(rJ, vc, so[j] + rb(kst - 3)) : 0.0f;
   }
   // shot-independent coefficient windows: A2 on the w rows (B: k-2..k+1, A: q-2..q+1), and per
   // row A2 (u), A1 at k / q and 1-K1, 1-K2, 1-K3 at k-2 / q-2, loaded one iteration ahead
@@ -828,12 +837,12 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
       nW[j] = bld(rW, vc, so[j] + rb(k + 2));
-      nU[j] = bld(rU, vc, so[j] + rb(k + 1));
-      nP[j] = bld(rP, vc, so[j] + rb(k + 1));
-      nI[j] = want_a1 ? bld(rI1, vc, so[j] + rb(k + 1)) : 0.0f;
-      nIp[j] = A2 ? bld(rJ1, vc, so[j] + rb(k + 1)) : 0.0f;
-      mI[j] = want_a1 ? bld(rI0, vc, so[j] + rb(q + 1)) : 0.0f;
-      mIp[j] = A2 ? bld(rJ0, vc, so[j] + rb(q + 1)) : 0.0f;
+      nU[j] = bld(rPU, vc1, so[j] + rb(k + 1));
+      nP[j] = bld(rPU, vc, so[j] + rb(k + 1));
+      nI[j] = want_a1 ? bld(rI, vc1, so[j] + rb(k + 1)) : 0.0f;
+      nIp[j] = A2 ? bld(rJ, vc1, so[j] + rb(k + 1)) : 0.0f;
+      mI[j] = want_a1 ? bld(rI, vc, so[j] + rb(q + 1)) : 0.0f;
+      mIp[j] = A2 ? bld(rJ, vc, so[j] + rb(q + 1)) : 0.0f;
     }
     const float nA2w = wmask(k + 2) ? bld(rC, vc, 3u * planeb + rb(k + 2)) : 0.0f;
     const float nA2u = bld(rC, vc, 3u * planeb + rb(k + 1)), nA1 = bld(rC, vc, planeb + rb(k + 1));
@@ -947,8 +956,8 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
             }
           }
           if (own_qq) {
-            bst(opa, oP, vs, so[j] + rqq);
-            if (c < d.X - 1) bst(oua, oU, vs, so[j] + rqq);
+            bst(opa, oPU, vs, so[j] + rqq);
+            if (c < d.X - 1) bst(oua, oPU, vs1, so[j] + rqq);
             if (qq < d.Z - 1) bst(owa, oW, vs, so[j] + rqq);
           }
         }
