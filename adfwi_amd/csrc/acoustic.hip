@@ -876,6 +876,9 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float k1r = pmr ? cK1 : 1.0f, k2r = um ? cK2 : 1.0f, k3r = wm ? cK3 : 1.0f;
     const float a2q = rowu_q ? dA2u : 0.0f, a1q = rowp_q ? dA1 : 0.0f;
     const float k1q = pmr_q ? dK1 : 1.0f, k2q = um_q ? dK2 : 1.0f, k3q = wm_q ? dK3 : 1.0f;
+    // A1 image-term masks as 0/1 factors on one operand: (x*1)*y rounds as x*y, and a masked
+    // term is +-0, which leaves the sum unchanged; the sums are stored for owned points only
+    const float m1k = rowp ? 1.0f : 0.0f, m1q = rowp_q ? 1.0f : 0.0f;
     // receiver slot of B's emitted row r (step t's sources), on every lane of the strip
     int ridx = -1;
     if (recs && r >= 0 && r < d.Z && io.rmap[r] > 0 && in_col) ridx = io.rmap[d.Z + r * d.pitch + c];
@@ -899,7 +902,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         l2c[j] = l2;
         if (grad_src && own_k && k == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t + 1] = l2;
         g3[j] = a1k * l2;
-        if (want_a1 && own_k && rowp) ga1b -= l2 * cI[j];
+        if (want_a1) ga1b -= l2 * (cI[j] * m1k);
         // outputs of B at row r: the adjoint state after step t (plus step t's sources)
         const float gc = g1[j];
         const float gl1 = shfl(gc, lane - 1), gr1 = shfl(gc, lane + 1), gr2 = shfl(gc, lane + 2);
@@ -940,7 +943,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         m2c[j] = m2;
         if (grad_src && own_q && q == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t] = m2;
         h3[j] = a1q * m2;
-        if (want_a1 && own_q && rowp_q) ga1a -= m2 * dI[j];
+        if (want_a1) ga1a -= m2 * (dI[j] * m1q);
         {
           const float hc = h1[j];
           const float hl1 = shfl(hc, lane - 1), hr1 = shfl(hc, lane + 1), hr2 = shfl(hc, lane + 2);
@@ -969,7 +972,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
     dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
     // image terms: B's row k waits three iterations for A's row q, B's row r for A's row qq
-    if (want_a1 && own_q && rowp_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
+    if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
     if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
     cG1 = nG1; cG2 = nG2;
     ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
