@@ -151,6 +151,11 @@ class AcousticGeometry:
         nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),
                   "adfwi_acoustic_get_layout")
 
+    @property
+    def gradient_group(self) -> int:
+        """Consecutive shots that share one gradient accumulator plane in the adjoint."""
+        return max(1, min(int(self.layout.shots_per_block), int(self.layout.reserved)))
+
     def _receiver_map(self, rcv_z: np.ndarray, rcv_x: np.ndarray) -> Optional[torch.Tensor]:
         """[Z] receivers per row, then [Z][pitch] receiver index (-1 elsewhere); None when
         receivers repeat a point or sit on the free-surface mirror row fs-1 (records are then
@@ -246,11 +251,14 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
 
 
 def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, ckpt: Optional[torch.Tensor],
-                 ckpt_every: int, g_rec, need_a1: bool, need_a2: bool, need_src: bool):
+                 ckpt_every: int, g_rec, need_a1: bool, need_a2: bool, need_src: bool,
+                 group_planes: bool = False):
     """Hand-written adjoint over all steps with segment recomputation from the checkpoints.
 
     g_rec: tuple of three (S, nt, R) adjoint sources (or None each).
     Returns (grad_a1 (Z, X) or None, grad_a2 or None, grad_src (S, nt) or None).
+    group_planes: grad_a1 / grad_a2 are returned unsummed, (G, Z, X) with one plane per group of
+    `geom.gradient_group` consecutive shots (their sum is the gradient of all shots).
     """
     lib = nat.load()
     dev = geom.device
@@ -312,6 +320,8 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
                       lambda: nat.check(lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream),
                                         "adfwi_acoustic_adjoint"), steps=n)
         lam = lw[0] if aa.lam_out == lw[0].data_ptr() else lw[1]
+    if group_planes:
+        return (None if ga1 is None else ga1[:, :, :X], None if ga2 is None else ga2[:, :, :X], gsrc)
     out_a1 = ga1.sum(0)[:, :X] if ga1 is not None else None
     out_a2 = ga2.sum(0)[:, :X] if ga2 is not None else None
     return out_a1, out_a2, gsrc

@@ -2,12 +2,15 @@
 it, the plain path (one request at a time, first come first served); prints one JSON line.
 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
+                                [--model-pool 6]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
 that of a single FIFO server with those service times and the same arrivals.  The scale
 options shrink nt, the shot count and the grid of every request so that a bounded run finishes
-in minutes; the JSON line states them.
+in minutes; the JSON line states them.  --model-pool N draws every request against one of N
+shared models (the shot requests of an inversion), so that requests of one model and mode are
+packed into shared launches.
 """
 import argparse
 import copy
@@ -37,6 +40,7 @@ def main():
     ap.add_argument("--max-shots", type=int, default=8)
     ap.add_argument("--max-nz", type=int, default=256)
     ap.add_argument("--max-nx", type=int, default=768)
+    ap.add_argument("--model-pool", type=int, default=0)
     args = ap.parse_args()
     from .loop import ServingLoop, run_alone
     from .requests import make_stream
@@ -44,7 +48,7 @@ def main():
 
     def stream(rate):
         return make_stream(args.requests, rate, args.seed, nz_range=(128, args.max_nz), nx_range=(256, args.max_nx),
-                           nt_range=nt_range, shots_range=(1, args.max_shots))
+                           nt_range=nt_range, shots_range=(1, args.max_shots), model_pool=args.model_pool)
     # warm-up outside the measurement: one small request of each kind and mode
     warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
                        shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
@@ -61,7 +65,7 @@ def main():
            "stream": {"requests": args.requests, "seed": args.seed,
                       "mix": "60% acoustic / 40% isotropic elastic, 50% forward / 50% gradient",
                       "nz": [128, args.max_nz], "nx": [256, args.max_nx], "nt": list(nt_range),
-                      "shots": [1, args.max_shots], "arrivals": "Poisson"},
+                      "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool},
            "slots": args.slots,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),

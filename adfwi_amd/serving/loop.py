@@ -8,6 +8,10 @@ kernels of several requests execute concurrently and the host-side launch loops 
 the GIL inside the C library) run in parallel.  A request is done when its stream has drained;
 its latency is completion time - arrival time.  Model fields, wavefields, checkpoints and
 records stay in HBM for the request's lifetime and are freed when it completes.
+
+With packing on (the default), requests against one shared model (`Request.model`) and of one
+mode are admitted together and share one sequence of launches (`execute_pack`): their shots
+run side by side in the same kernels, and each member gets its own records or gradient.
 """
 import queue
 import threading
@@ -18,7 +22,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from .requests import Request, execute, footprint_bytes
+from .requests import Request, execute, execute_pack, footprint_bytes, pack_key
 from .scheduler import Scheduler
 
 
@@ -44,6 +48,7 @@ class StreamReport:
     wall_s: float
     peak_admitted_bytes: int
     max_concurrent: int
+    packs: List[List[int]] = field(default_factory=list)   # request ids of every admitted pack
 
     def summary(self) -> dict:
         ok = [o for o in self.outcomes if o.error is None]
@@ -53,7 +58,8 @@ class StreamReport:
                 "throughput_req_per_s": len(ok) / self.wall_s if self.wall_s else 0.0,
                 "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
                 "latency_mean_s": float(lat.mean()), "max_concurrent": self.max_concurrent,
-                "peak_admitted_gib": self.peak_admitted_bytes / 2**30}
+                "peak_admitted_gib": self.peak_admitted_bytes / 2**30,
+                "packs": len(self.packs), "packed_requests": sum(len(p) for p in self.packs if len(p) > 1)}
 
 
 class ServingLoop:
@@ -61,7 +67,8 @@ class ServingLoop:
     `budget_bytes` of admitted footprint (default: 85 % of the free device memory)."""
 
     def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
-                 footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4):
+                 footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4,
+                 pack: bool = True, max_pack_shots: int = 64):
         import torch
         from .. import _native as nat
         nat.require_gpu(device)
@@ -70,7 +77,7 @@ class ServingLoop:
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             budget_bytes = int(0.85 * free)
-        self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass)
+        self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, pack_key if pack else None, max_pack_shots)
         self.footprint = footprint
         self.max_slots = max_slots
         self._local = threading.local()
@@ -83,19 +90,21 @@ class ServingLoop:
             self._local.stream = s
         return s
 
-    def _run_one(self, req: Request, t0: float) -> Outcome:
+    def _run_pack(self, reqs: List[Request], t0: float) -> List[Outcome]:
         import torch
         start = time.perf_counter() - t0
         stream = self._stream()
         try:
             with torch.cuda.device(self.device), torch.cuda.stream(stream):
-                out, digest = execute(req, self.device)
+                res = execute_pack(reqs, self.device)
                 stream.synchronize()
             end = time.perf_counter() - t0
-            return Outcome(req.rid, req.arrival, start, end, digest, out if req.keep else None)
+            return [Outcome(r.rid, r.arrival, start, end, dg, out if r.keep else None)
+                    for r, (out, dg) in zip(reqs, res)]
         except Exception as exc:  # noqa: BLE001 -- reported per request
             stream.synchronize()
-            return Outcome(req.rid, req.arrival, start, time.perf_counter() - t0, {}, None, repr(exc))
+            end = time.perf_counter() - t0
+            return [Outcome(r.rid, r.arrival, start, end, {}, None, repr(exc)) for r in reqs]
 
     def run(self, requests: List[Request], realtime: bool = True) -> StreamReport:
         """Serves the requests.  realtime=False submits every request at t = 0 (arrival times
@@ -107,6 +116,7 @@ class ServingLoop:
         outcomes: List[Outcome] = []
         futures = {}
         max_conc = 0
+        packs = []
         t0 = time.perf_counter()
         i = 0
         with ThreadPoolExecutor(max_workers=self.max_slots) as pool:
@@ -118,8 +128,9 @@ class ServingLoop:
                         outcomes.append(Outcome(r.rid, r.arrival, now, now, {}, None,
                                                 self.scheduler.rejected[-1][1]))
                     i += 1
-                for r in self.scheduler.admit():
-                    futures[pool.submit(self._run_one, r, t0)] = r
+                for pack in self.scheduler.admit_packs():
+                    futures[pool.submit(self._run_pack, pack, t0)] = pack
+                    packs.append([r.rid for r in pack])
                 max_conc = max(max_conc, len(futures))
                 timeout = None
                 if i < len(pending):
@@ -127,16 +138,17 @@ class ServingLoop:
                 if futures:
                     done, _ = wait(list(futures), timeout=timeout, return_when=FIRST_COMPLETED)
                     for f in done:
-                        r = futures.pop(f)
-                        self.scheduler.finish(r.rid)
-                        outcomes.append(f.result())
+                        for r in futures.pop(f):
+                            self.scheduler.finish(r.rid)
+                        outcomes.extend(f.result())
                 elif timeout is not None:
                     time.sleep(timeout)
                 elif self.scheduler.waiting:
                     raise RuntimeError("requests wait but nothing runs: footprint accounting is inconsistent")
         wall = time.perf_counter() - t0
         outcomes.sort(key=lambda o: o.rid)
-        return StreamReport(outcomes, list(self.scheduler.rejected), wall, self.scheduler.peak_bytes, max_conc)
+        return StreamReport(outcomes, list(self.scheduler.rejected), wall, self.scheduler.peak_bytes, max_conc,
+                            packs)
 
 
 def run_alone(requests: List[Request], device="cuda:0") -> List[Outcome]:

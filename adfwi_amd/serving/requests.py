@@ -5,7 +5,9 @@ A request asks for acoustic or isotropic-elastic modelling ("forward": the recor
 gradient ("gradient": d/d model of 0.5 * sum(records^2) over the recorded components, an
 adjoint source that needs no observed data).  Its model comes from its own seed through the
 recipes of ``adfwi_amd.workloads`` (the acoustic benchmark recipe, the elastic layered recipe),
-scaled to the request's grid.
+scaled to the request's grid -- or, when the request names a shared model (`model`), from that
+model's seed, so that requests against one model (the shots of one FWI iteration) can share
+launches: see `pack_key` and `execute_pack`.
 """
 import math
 from dataclasses import dataclass, field
@@ -32,7 +34,12 @@ class Request:
     arrival: float = 0.0      # seconds after the start of the stream
     nabc: int = 20
     keep: bool = False        # keep the full result (otherwise a digest)
+    model: Optional[int] = None   # shared model key (its seed); None: the request's own model
     extra: dict = field(default_factory=dict)
+
+    @property
+    def model_seed(self) -> int:
+        return self.seed if self.model is None else int(self.model)
 
     @property
     def dx(self) -> float:
@@ -92,22 +99,47 @@ def footprint_bytes(req: Request) -> int:
 
 def make_stream(n: int = 2000, rate: float = 1.0, seed: int = 4, *, acoustic_frac: float = 0.6,
                 gradient_frac: float = 0.5, nz_range=(128, 512), nx_range=(256, 2048), nt_range=(1000, 6000),
-                shots_range=(1, 64), nabc: int = 20) -> List[Request]:
+                shots_range=(1, 64), nabc: int = 20, model_pool: int = 0) -> List[Request]:
     """Seeded Poisson stream: kind 60/40 acoustic/elastic, mode 50/50 forward/gradient, nz and
-    nx uniform over multiples of 32 in their ranges, nt and shot count uniform."""
+    nx uniform over multiples of 32 in their ranges, nt and shot count uniform.
+
+    model_pool > 0: every request is drawn against one of that many shared models (kind, grid
+    and nt fixed per model; mode and shot count per request), as the shot requests of an
+    inversion are; requests of one model and mode can then share launches."""
     rng = np.random.default_rng(seed)
+    pool = []
+    for m in range(model_pool):
+        kind = "acoustic" if rng.random() < acoustic_frac else "elastic"
+        nz = int(rng.choice(np.arange(nz_range[0], nz_range[1] + 1, 32)))
+        nx = int(rng.choice(np.arange(nx_range[0], nx_range[1] + 1, 32)))
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        pool.append((kind, nz, nx, nt, int(rng.integers(1 << 30))))
     t = 0.0
     out = []
     for i in range(n):
         t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
-        kind = "acoustic" if rng.random() < acoustic_frac else "elastic"
+        if pool:
+            kind, nz, nx, nt, mseed = pool[int(rng.integers(len(pool)))]
+        else:
+            kind = "acoustic" if rng.random() < acoustic_frac else "elastic"
+            nz = int(rng.choice(np.arange(nz_range[0], nz_range[1] + 1, 32)))
+            nx = int(rng.choice(np.arange(nx_range[0], nx_range[1] + 1, 32)))
+            nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+            mseed = None
         mode = "gradient" if rng.random() < gradient_frac else "forward"
-        nz = int(rng.choice(np.arange(nz_range[0], nz_range[1] + 1, 32)))
-        nx = int(rng.choice(np.arange(nx_range[0], nx_range[1] + 1, 32)))
-        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
         ns = int(rng.integers(shots_range[0], shots_range[1] + 1))
-        out.append(Request(i, kind, mode, nz, nx, nt, ns, int(rng.integers(1 << 30)), t, nabc))
+        out.append(Request(i, kind, mode, nz, nx, nt, ns, int(rng.integers(1 << 30)), t, nabc, model=mseed))
     return out
+
+
+def pack_key(req: Request):
+    """Requests with equal keys can share launches: one model, grid, nt and mode.  Modelling
+    requests of either kind pack (shots are independent); gradient requests pack for the
+    acoustic propagator, whose adjoint keeps one gradient plane per shot group so that every
+    member's gradient is summed from its own groups.  None: the request runs by itself."""
+    if req.model is None or (req.mode == "gradient" and req.kind != "acoustic"):
+        return None
+    return (req.kind, req.mode, req.nz, req.nx, req.nt, req.nabc, int(req.model))
 
 
 # ---------------------------------------------------------------------------
@@ -124,19 +156,25 @@ def _acoustic_vp(nz, nx, rng):
     return np.clip(vp, 1500, 4700).astype(np.float32)
 
 
-def build_problem(req: Request, device):
-    """(propagator, model) of a request; sources evenly spaced 2 cells deep (4 for elastic),
-    receivers on every column 2 cells deep."""
+def shot_positions(req: Request):
+    """(x, z) grid indices of the request's sources: evenly spaced, 2 cells deep (4 elastic)."""
+    sx = np.linspace(min(4, req.nx - 1), max(req.nx - 5, 0), req.nshots).astype(int)
+    sz = np.full(req.nshots, 2 if req.kind == "acoustic" else 4)
+    return sx, sz
+
+
+def build_problem(req: Request, device, shots=None):
+    """(propagator, model) of a request; its sources (`shot_positions`, or the given (x, z)
+    arrays), receivers on every column 2 cells deep."""
     import torch
     from ..model import AcousticModel, IsotropicElasticModel
     from ..propagator import AcousticPropagator, ElasticPropagator
     from ..survey import Receiver, Source, Survey
     from ..workloads import elastic_layers, gardner, ricker
-    rng = np.random.default_rng(req.seed)
+    rng = np.random.default_rng(req.model_seed)
     nz, nx = req.nz, req.nx
     src = Source(req.nt, req.dt, req.f0)
-    sx = np.linspace(min(4, nx - 1), max(nx - 5, 0), req.nshots).astype(int)
-    sz = np.full(req.nshots, 2 if req.kind == "acoustic" else 4)
+    sx, sz = shot_positions(req) if shots is None else shots
     src.add_sources(sx, sz, ricker(req.nt, req.dt, req.f0))
     rcv = Receiver(req.nt, req.dt)
     rcv.add_receivers(np.arange(nx), np.full(nx, 2), "pr" if req.kind == "acoustic" else "vz")
@@ -171,5 +209,94 @@ def execute(req: Request, device):
         loss.backward()
         out = {n: p.grad for n, p in model.named_parameters() if p.grad is not None}
         out["loss"] = loss.detach().reshape(1)
-    digest = {k: (float(v.double().sum()), float(v.double().square().sum())) for k, v in out.items()}
-    return out, digest
+    return out, _digest(out)
+
+
+def _digest(out):
+    return {k: (float(v.double().sum()), float(v.double().square().sum())) for k, v in out.items()}
+
+
+def _packed_acoustic_gradient(reqs: List[Request], device):
+    """Gradients of acoustic requests against one model in shared launches.
+
+    Every member's shots are padded with silent shots (zero wavelet: their wavefield, records,
+    adjoint and gradient terms are exactly zero) to a multiple of the adjoint's gradient group,
+    so that each accumulator plane holds one member's shots only; a member's coefficient
+    gradient is the sum of its own planes, carried to vp through the coefficient construction.
+    Returns None when the layout does not give aligned groups (the members then run alone)."""
+    import torch
+    from ..propagator.acoustic_kernels import (AcousticGeometry, checkpoint_interval, coefficient_fields,
+                                               run_backward, run_forward)
+    r0 = reqs[0]
+    sxs, szs, real = [], [], []
+    group = 2
+    for attempt in range(2):
+        sxs, szs, real, offsets = [], [], [], []
+        for r in reqs:
+            sx, sz = shot_positions(r)
+            pad = (-len(sx)) % group
+            offsets.append(len(real))
+            sxs += list(sx) + [int(sx[0])] * pad
+            szs += list(sz) + [int(sz[0])] * pad
+            real += [True] * len(sx) + [False] * pad
+        prop, model = build_problem(r0, device, shots=(np.array(sxs), np.array(szs)))
+        nabc = r0.nabc
+        geom = AcousticGeometry(prop.nz + 2 * nabc, prop.nx + 2 * nabc, r0.nt, False, nabc,
+                                np.array(szs) + nabc, np.array(sxs) + nabc,
+                                prop.rcv_z.cpu().numpy() + nabc, prop.rcv_x.cpu().numpy() + nabc, device)
+        g = geom.gradient_group
+        if group % g == 0:
+            break
+        group = g
+    else:
+        return None
+    model.forward()
+    a1, k1, a2, k2, k3 = coefficient_fields(model.vp, model.rho, prop.damp, r0.nz, r0.nx, nabc, r0.dt, r0.dx,
+                                            False, device)
+    coef = geom.pack_coef(1.0 - k1, a1.detach(), 1.0 - k2, a2.detach(), 1.0 - k3)
+    mask = torch.tensor(real, device=device, dtype=torch.float32).unsqueeze(1)
+    src = (r0.dt * prop.wavelet.to(torch.float32)) * mask
+    k = checkpoint_interval(r0.nt)
+    res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k)
+    rec_p = res["rec"][0]
+    planes, _, _ = run_backward(geom, coef, src.contiguous(), res["ckpt"], k, (rec_p, None, None), True, False,
+                                False, group_planes=True)
+    g = geom.gradient_group
+    results = []
+    for r, o in zip(reqs, offsets):
+        n = r.nshots + (-r.nshots) % group
+        ga1 = planes[o // g:(o + n) // g].sum(0)
+        grad, = torch.autograd.grad(a1, model.vp, ga1, retain_graph=True)
+        rp = rec_p[o:o + r.nshots]
+        out = {"vp": grad, "loss": (0.5 * rp.square().sum()).detach().reshape(1)}
+        results.append((out, _digest(out)))
+    return results
+
+
+def execute_pack(reqs: List[Request], device):
+    """Runs requests with one `pack_key` in shared launches; returns [(result, digest)] in the
+    members' order.  Each result is the member's own: its records (shots are independent), or
+    its own gradient and loss."""
+    import torch
+    if len(reqs) == 1:
+        return [execute(reqs[0], device)]
+    r0 = reqs[0]
+    key = pack_key(r0)
+    if key is None or any(pack_key(r) != key for r in reqs):
+        raise ValueError("requests of a pack must share one pack key")
+    if r0.mode == "gradient":
+        res = _packed_acoustic_gradient(reqs, device)
+        return res if res is not None else [execute(r, device) for r in reqs]
+    pos = [shot_positions(r) for r in reqs]
+    shots = (np.concatenate([p[0] for p in pos]), np.concatenate([p[1] for p in pos]))
+    prop, _ = build_problem(r0, device, shots=shots)
+    comps = ("p",) if r0.kind == "acoustic" else ("vx", "vz")
+    with torch.no_grad():
+        rec = prop.forward()
+    results = []
+    o = 0
+    for r in reqs:
+        out = {k: rec[k][o:o + r.nshots] for k in comps}
+        results.append((out, _digest(out)))
+        o += r.nshots
+    return results

@@ -5,10 +5,15 @@ footprint fits into what the running requests leave of the budget.  A request th
 fit may be passed by later, smaller ones, but only `max_bypass` times: after that nothing is
 admitted past it until it fits, so large requests cannot starve.  A request whose footprint
 exceeds the whole budget is rejected at submission.
+
+Packing: with a `pack_key`, a request being admitted takes later waiting requests with the
+same key along (up to `max_pack_shots` shots in all, each within the free budget): the pack
+shares one slot and one sequence of launches, and every member keeps its own footprint in the
+accounting until the pack finishes.
 """
 from collections import OrderedDict, deque
 from dataclasses import dataclass
-from typing import Deque, List, Tuple
+from typing import Callable, Deque, Dict, List, Optional, Tuple
 
 
 @dataclass
@@ -19,20 +24,28 @@ class _Waiting:
 
 
 class Scheduler:
-    def __init__(self, budget_bytes: int, max_slots: int, max_bypass: int = 4):
+    def __init__(self, budget_bytes: int, max_slots: int, max_bypass: int = 4,
+                 pack_key: Optional[Callable[[object], object]] = None, max_pack_shots: int = 64):
         if budget_bytes <= 0 or max_slots <= 0:
             raise ValueError("budget and slot count must be positive")
         self.budget = int(budget_bytes)
         self.max_slots = int(max_slots)
         self.max_bypass = int(max_bypass)
+        self.pack_key = pack_key
+        self.max_pack_shots = int(max_pack_shots)
         self.waiting: Deque[_Waiting] = deque()
         self.running: "OrderedDict[int, int]" = OrderedDict()
+        self._pack_of: Dict[int, int] = {}      # running request -> id of its pack (its first member)
         self.rejected: List[Tuple[object, str]] = []
         self.peak_bytes = 0
 
     @property
     def used(self) -> int:
         return sum(self.running.values())
+
+    @property
+    def slots_used(self) -> int:
+        return len(set(self._pack_of.values()))
 
     def submit(self, req, nbytes: int) -> bool:
         if nbytes > self.budget:
@@ -42,19 +55,38 @@ class Scheduler:
         self.waiting.append(_Waiting(req, int(nbytes)))
         return True
 
-    def admit(self) -> List[object]:
-        """Requests to start now, in arrival order."""
+    def admit_packs(self) -> List[List[object]]:
+        """Packs to start now (lists of requests sharing launches), in arrival order of their
+        first members."""
         out = []
         free = self.budget - self.used
-        slots = self.max_slots - len(self.running)
+        slots = self.max_slots - self.slots_used
         kept: Deque[_Waiting] = deque()
         blocked = False
-        for w in self.waiting:
+        queue = list(self.waiting)
+        taken = set()
+        for i, w in enumerate(queue):
+            if id(w) in taken:
+                continue
             if not blocked and slots > 0 and w.nbytes <= free:
-                self.running[w.req.rid] = w.nbytes
+                pack = [w]
                 free -= w.nbytes
+                taken.add(id(w))
+                key = self.pack_key(w.req) if self.pack_key is not None else None
+                if key is not None:
+                    shots = w.req.nshots
+                    for e in queue[i + 1:]:
+                        if (id(e) not in taken and e.nbytes <= free and shots + e.req.nshots <= self.max_pack_shots
+                                and self.pack_key(e.req) == key):
+                            pack.append(e)
+                            taken.add(id(e))
+                            free -= e.nbytes
+                            shots += e.req.nshots
                 slots -= 1
-                out.append(w.req)
+                for m in pack:
+                    self.running[m.req.rid] = m.nbytes
+                    self._pack_of[m.req.rid] = w.req.rid
+                out.append([m.req for m in pack])
                 for e in kept:          # every earlier request still waiting has been passed once more
                     e.bypassed += 1
                     if e.bypassed >= self.max_bypass:
@@ -67,8 +99,13 @@ class Scheduler:
         self.peak_bytes = max(self.peak_bytes, self.used)
         return out
 
+    def admit(self) -> List[object]:
+        """Requests to start now, in arrival order of their packs."""
+        return [r for pack in self.admit_packs() for r in pack]
+
     def finish(self, rid: int) -> None:
         self.running.pop(rid, None)
+        self._pack_of.pop(rid, None)
 
     def idle(self) -> bool:
         return not self.waiting and not self.running

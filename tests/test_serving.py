@@ -3,7 +3,7 @@ admission policy (no device needed)."""
 import numpy as np
 import pytest
 
-from adfwi_amd.serving import Request, Scheduler, footprint_bytes, make_stream
+from adfwi_amd.serving import Request, Scheduler, footprint_bytes, make_stream, pack_key
 
 
 def _req(rid, nbytes_tag=0, arrival=0.0):
@@ -98,3 +98,41 @@ def test_full_stream_admission_never_exceeds_budget():
             s.finish(rid)
             done += 1
     assert done + len(s.rejected) == len(reqs)
+
+
+def test_model_pool_stream_and_pack_keys():
+    reqs = make_stream(400, 5.0, seed=9, model_pool=3)
+    models = {r.model for r in reqs}
+    assert len(models) == 3 and None not in models
+    for m in models:   # kind, grid and nt are the model's
+        assert len({(r.kind, r.nz, r.nx, r.nt) for r in reqs if r.model == m}) == 1
+    for r in reqs:
+        k = pack_key(r)
+        if r.mode == "gradient" and r.kind == "elastic":
+            assert k is None
+        else:
+            assert k == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc, r.model)
+    assert all(pack_key(r) is None for r in make_stream(50, 5.0, seed=9))   # own models never pack
+
+
+def test_packs_share_a_slot_and_respect_shots_and_budget():
+    def req(rid, key, shots, nbytes):
+        r = Request(rid, "acoustic", "forward", 128, 256, 100, shots, rid, 0.0)
+        r.extra.update(key=key, nbytes=nbytes)
+        return r
+    s = Scheduler(budget_bytes=100, max_slots=2, pack_key=lambda r: r.extra["key"], max_pack_shots=8)
+    reqs = [req(0, "a", 4, 10), req(1, "b", 2, 10), req(2, "a", 3, 10), req(3, "a", 2, 10),
+            req(4, None, 1, 10), req(5, "a", 2, 75)]
+    for r in reqs:
+        assert s.submit(r, r.extra["nbytes"])
+    packs = s.admit_packs()
+    # 0 takes 2 along (4 + 3 shots); 3 and 5 would exceed 8 shots
+    assert [[r.rid for r in p] for p in packs] == [[0, 2], [1]]
+    assert s.slots_used == 2 and s.used == 30
+    assert [w.req.rid for w in s.waiting] == [3, 4, 5]
+    for rid in (0, 2):
+        s.finish(rid)
+    assert s.slots_used == 1 and s.used == 10
+    packs = s.admit_packs()   # 3 leads a new pack with 5 (2 + 2 shots, 85 B fits in 90 B); no slot for 4
+    assert [[r.rid for r in p] for p in packs] == [[3, 5]]
+    assert s.used == 95 and [w.req.rid for w in s.waiting] == [4]

@@ -49,3 +49,31 @@ def test_footprint_bounds_device_allocation():
         peak = torch.cuda.max_memory_allocated() - base
         del out
         assert peak <= footprint_bytes(r), (r.kind, r.mode, peak, footprint_bytes(r))
+
+
+def test_packed_requests_equal_their_results_alone():
+    """Requests against one shared model are packed into shared launches (modelling of both
+    kinds, acoustic gradients with members of odd shot counts, which the pack pads with silent
+    shots); every member's records are bit-identical to its run alone, its gradient and loss
+    equal within 1e-6 (the gradient planes of a pack may group the shots differently)."""
+    from adfwi_amd.serving import Request, ServingLoop, run_alone
+    spec = [("acoustic", "forward", 1), ("acoustic", "gradient", 3), ("acoustic", "forward", 2),
+            ("elastic", "forward", 2), ("acoustic", "gradient", 1), ("acoustic", "gradient", 2),
+            ("elastic", "forward", 1), ("acoustic", "forward", 3), ("elastic", "gradient", 1)]
+    reqs = []
+    for i, (kind, mode, ns) in enumerate(spec):
+        model = 101 if kind == "acoustic" else 202
+        reqs.append(Request(i, kind, mode, 128, 288, 90, ns, 1000 + i, 0.0, 20, keep=True, model=model))
+    rep = ServingLoop(DEV, max_slots=2).run(copy.deepcopy(reqs), realtime=False)
+    assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+    sizes = {tuple(p) for p in rep.packs}
+    assert (0, 2, 7) in sizes and (1, 4, 5) in sizes and (3, 6) in sizes and (8,) in sizes, sizes
+    alone = run_alone(reqs, DEV)
+    for a, b in zip(rep.outcomes, alone):
+        assert a.rid == b.rid and a.result.keys() == b.result.keys(), (a.rid, a.result.keys(), b.result.keys())
+        for k in b.result:
+            x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+            if reqs[a.rid].mode == "forward":
+                assert np.array_equal(x, y), (a.rid, k, rel_l2(x, y))
+            else:
+                assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
