@@ -44,7 +44,7 @@ class AcousticFwdArgs(ctypes.Structure):
                 ("t0", _i32), ("nsteps", _i32), ("energy_u_from", _i32),
                 ("rec_p", _vp), ("rec_u", _vp), ("rec_w", _vp), ("energy", _vp),
                 ("image", _vp), ("image_p", _vp), ("state_out", _vp), ("rcv_map", _vp),
-                ("fuse_steps", _i32), ("reserved2", _i32)]
+                ("fuse_steps", _i32), ("reserved2", _i32), ("t_base", _vp)]
 
 
 class AcousticAdjArgs(ctypes.Structure):

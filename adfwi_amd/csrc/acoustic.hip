@@ -47,7 +47,14 @@ struct AcDev {
   const float *coef;
   const float *src;
   const int *src_z, *src_x, *rcv_z, *rcv_x;
+  // forward only: device-side time base added to every step index (NULL: 0), so that one
+  // captured launch sequence serves every segment of a run; first step of the u energy map
+  const int *tbase;
+  int eu_from;
 };
+
+// the time base of a forward launch (one scalar load; 0 without a device time base)
+__device__ __forceinline__ int time_base(const AcDev &d) { return d.tbase ? *d.tbase : 0; }
 
 __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane, WAVE); }
 
@@ -92,9 +99,12 @@ __device__ __forceinline__ bool strip_of(const AcDev &d, int &z0, int &x0) {
 template <int SPB, int UNR>
 __global__ __launch_bounds__(NT) void ac_fwd_step(
     AcDev d, const float *__restrict__ in, float *__restrict__ out, int t,
-    float *__restrict__ energy, int want_u, float *__restrict__ img_d, float *__restrict__ img_p) {
+    float *__restrict__ energy, int acc_u, float *__restrict__ img_d, float *__restrict__ img_p) {
   int z0, x0;
   if (!strip_of(d, z0, x0)) return;
+  t += time_base(d);
+  if (t < 0 || t >= d.nt) return;      // a device time base outside the run: nothing to do
+  const bool want_u = acc_u && t >= d.eu_from;
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - HALO_L + lane;                 // this lane's column
   const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);   // clamped load column (bytes)
@@ -259,7 +269,6 @@ struct PairIO {
   const float *in;
   float *out;
   int t;
-  int want_u0, want_u1;                 // u^2 enters the energy map for step t / t+1
   float *energy;
   float *img_d0, *img_d1, *img_p0, *img_p1;   // *_1 = *_0 + one field (consecutive steps)
   const int *rmap;                      // [Z] receivers per row, then [Z][pitch] receiver index or -1
@@ -306,7 +315,9 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const int zmax = d.Z - 1;
   const int fs = d.fs;
   const int end_row = min(z0 + d.tz, d.Z);
-  const int t = io.t;
+  const int t = io.t + time_base(d);
+  if (t < 0 || t + 1 >= d.nt) return;  // a device time base outside the run: nothing to do
+  const bool want_u0 = t >= d.eu_from, want_u1 = t + 1 >= d.eu_from;   // u^2 enters the energy map
   auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
   // one descriptor spans two consecutive fields (p and u; the images of steps t and t+1): the
   // second field is reached through the lane offset (vc1 / vs1), which keeps scalar registers
@@ -368,7 +379,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   // energy of A's emitted rows, waiting for B's rows three iterations later
   float epa0 = 0.f, epa1 = 0.f, epa2 = 0.f, eua0 = 0.f, eua1 = 0.f, eua2 = 0.f;
   // old energy of row qq, loaded one iteration ahead (read-modify-write without a wait)
-  const bool want_eu = io.want_u0 || io.want_u1;
+  const bool want_eu = want_u0 || want_u1;
   float ce0 = energy ? bld(rE, vc, eb + rb(kst - 5)) : 0.0f;
   float ce1 = (energy && want_eu) ? bld(rE, vc, eb + planeb + rb(kst - 5)) : 0.0f;
 
@@ -500,7 +511,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     if (energy && bstep && own_qq) {
       const float ep = epa0 + epb;
       bst(ce0 + ep, rE, vs, eb + rqq);
-      const float eu = (io.want_u0 ? eua0 : 0.0f) + (io.want_u1 ? eub : 0.0f);
+      const float eu = (want_u0 ? eua0 : 0.0f) + (want_u1 ? eub : 0.0f);
       if (want_eu) bst(ce1 + eu, rE, vs, eb + planeb + rqq);
     }
     ce0 = ne0; ce1 = ne1;
@@ -985,6 +996,8 @@ __global__ void ac_gather(AcDev d, const float *__restrict__ st, float *__restri
                           float *__restrict__ rw, int t) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long long)d.S * d.nrec) return;
+  t += time_base(d);
+  if (t < 0 || t >= d.nt) return;
   const int s = (int)(k / d.nrec), r = (int)(k - (long long)s * d.nrec);
   const long long o = (long long)s * d.plane + (long long)d.rcv_z[r] * d.pitch + d.rcv_x[r];
   const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
@@ -1096,6 +1109,8 @@ AcDev make_dev(const adfwi_acoustic_desc *d, const adfwi_layout &L) {
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.src = d->src; v.src_z = d->src_z; v.src_x = d->src_x;
   v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
+  v.tbase = nullptr;
+  v.eu_from = d->nt;
   return v;
 }
 
@@ -1211,7 +1226,10 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   ADFWI_CHECK_ARG(a->t0 >= 0 && a->nsteps >= 0 && a->t0 + a->nsteps <= d->nt,
                   "steps [%d, %d) outside [0, %d)", a->t0, a->t0 + a->nsteps, d->nt);
   ADFWI_CHECK_ARG(!a->ckpt || a->ckpt_every >= 1, "ckpt_every must be >= 1 with a checkpoint buffer");
+  ADFWI_CHECK_ARG(!a->t_base || !a->ckpt, "checkpoints are not written with a device time base");
   AcDev v = make_dev(d, L);
+  v.tbase = a->t_base;
+  v.eu_from = a->energy_u_from;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid = step_grid(L), block(NT);
   const long long ng = (long long)d->nshots * d->nrec;
@@ -1239,8 +1257,6 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
       PairIO io;
       io.in = cur; io.out = nxt; io.t = t;
       io.energy = a->energy;
-      io.want_u0 = (a->energy && t >= a->energy_u_from) ? 1 : 0;
-      io.want_u1 = (a->energy && t + 1 >= a->energy_u_from) ? 1 : 0;
       io.img_d0 = imd; io.img_d1 = imd ? imd + L.field_floats : nullptr;
       io.img_p0 = imp; io.img_p1 = imp ? imp + L.field_floats : nullptr;
       io.rmap = a->rcv_map;
@@ -1248,7 +1264,7 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
       launch_pair(w, pgrid, block, st, io);
       if ((rc = adfwi::check_launch("ac_fwd_pair"))) return rc;
     } else {
-      const int acc_u = (a->energy && t >= a->energy_u_from) ? 1 : 0;
+      const int acc_u = a->energy ? 1 : 0;   // from step energy_u_from on (tested in the kernel)
       launch_fwd(v, grid, block, st, cur, nxt, t, a->energy, acc_u, imd, imp);
       if ((rc = adfwi::check_launch("ac_fwd_step"))) return rc;
     }

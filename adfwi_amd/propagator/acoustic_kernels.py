@@ -113,7 +113,7 @@ class AcousticGeometry:
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, nabc: int,
                  src_z: np.ndarray, src_x: np.ndarray, rcv_z: np.ndarray, rcv_x: np.ndarray,
-                 device, shots_per_block: int = 0):
+                 device, shots_per_block: int = 0, graph_steps: Optional[int] = None):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
@@ -140,12 +140,18 @@ class AcousticGeometry:
         self.adj_shots_per_block = int(os.environ.get("ADFWI_ADJ_SHOTS_PER_BLOCK", "0"))
         self.row_unroll = int(os.environ.get("ADFWI_ROW_UNROLL", "0"))
         self.tile_z = int(os.environ.get("ADFWI_TILE_Z", "0"))
+        # forward runs without checkpoints replay HIP graphs of this many steps (0: direct launches)
+        self.graph_steps = int(os.environ.get("ADFWI_GRAPH_STEPS", "0") if graph_steps is None else graph_steps)
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
         # two steps per launch (the intermediate step's receivers written from inside the kernel)
         self.fuse_steps = int(os.environ.get("ADFWI_FUSE_STEPS", "1"))
         self.rcv_map = self._receiver_map(rcv_z, rcv_x) if self.fuse_steps else None
+        # what a captured launch sequence depends on besides its buffers
+        self.content_key = (self.Z, self.X, self.nt, self.free_surface, self.fs, self.shots_per_block,
+                            self.adj_shots_per_block, self.row_unroll, self.tile_z, self.fuse_steps,
+                            src_z.tobytes(), src_x.tobytes(), rcv_z.tobytes(), rcv_x.tobytes())
         desc = self.desc(None, None)
         self.layout = nat.Layout()
         nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),
@@ -344,8 +350,12 @@ class AcousticWave(torch.autograd.Function):
         src = src.detach().to(torch.float32).contiguous()
         need_grad = any(ctx.needs_input_grad[:3])  # grad mode is off inside forward()
         k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
-        res = run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from,
-                          ckpt_every=k if need_grad else 0)
+        if not need_grad and geom.graph_steps and geom.R:
+            from .graphs import graphed_forward
+            res = graphed_forward(geom, coef, src, energy_u_from, geom.graph_steps)
+        else:
+            res = run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from,
+                              ckpt_every=k if need_grad else 0)
         rec = res["rec"]
         if rec is None:
             rec = [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(3)]

@@ -80,6 +80,9 @@ class AcousticPropagator(torch.nn.Module):
         self.rcv_n = self.receiver.num
         self._geoms = OrderedDict()
         self.ckpt_every = 0  # 0 = ceil(sqrt(nt)) wavefield checkpoint spacing
+        # forward runs without gradient replay HIP graphs of this many steps (None: the
+        # ADFWI_GRAPH_STEPS environment variable, default 0 = direct launches)
+        self.graph_steps = None
 
     def boundary_condition(self, vmax=None):
         """Damping profile (acoustic_propagator.py:102-118); PML vmax frozen at construction."""
@@ -103,7 +106,7 @@ class AcousticPropagator(torch.nn.Module):
             nabc = self.nabc
             g = AcousticGeometry(self.nz + 2 * nabc, self.nx + 2 * nabc, self.nt, self.free_surface, nabc,
                                  sl[:, 1] + nabc, sl[:, 0] + nabc, self.rcv_loc[:, 1] + nabc,
-                                 self.rcv_loc[:, 0] + nabc, self.device)
+                                 self.rcv_loc[:, 0] + nabc, self.device, graph_steps=self.graph_steps)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)

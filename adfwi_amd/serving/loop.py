@@ -12,6 +12,9 @@ records stay in HBM for the request's lifetime and are freed when it completes.
 With packing on (the default), requests against one shared model (`Request.model`) and of one
 mode are admitted together and share one sequence of launches (`execute_pack`): their shots
 run side by side in the same kernels, and each member gets its own records or gradient.
+Acoustic modelling replays HIP graphs of `graph_steps` steps (default 64; 0 = direct
+launches): a worker thread keeps the captured graphs of the shapes it has run, so later
+requests of the same shape replay them.
 """
 import queue
 import threading
@@ -68,7 +71,7 @@ class ServingLoop:
 
     def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
                  footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4,
-                 pack: bool = True, max_pack_shots: int = 64):
+                 pack: bool = True, max_pack_shots: int = 64, graph_steps: int = 64):
         import torch
         from .. import _native as nat
         nat.require_gpu(device)
@@ -77,9 +80,13 @@ class ServingLoop:
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             budget_bytes = int(0.85 * free)
+            if graph_steps:       # the graph contexts each worker thread keeps
+                from ..propagator import graphs
+                budget_bytes -= max_slots * graphs.MAX_CONTEXTS * graphs.MAX_CONTEXT_BYTES
         self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, pack_key if pack else None, max_pack_shots)
         self.footprint = footprint
         self.max_slots = max_slots
+        self.graph_steps = int(graph_steps)
         self._local = threading.local()
 
     def _stream(self):
@@ -96,7 +103,7 @@ class ServingLoop:
         stream = self._stream()
         try:
             with torch.cuda.device(self.device), torch.cuda.stream(stream):
-                res = execute_pack(reqs, self.device)
+                res = execute_pack(reqs, self.device, self.graph_steps)
                 stream.synchronize()
             end = time.perf_counter() - t0
             return [Outcome(r.rid, r.arrival, start, end, dg, out if r.keep else None)

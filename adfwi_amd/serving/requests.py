@@ -163,9 +163,10 @@ def shot_positions(req: Request):
     return sx, sz
 
 
-def build_problem(req: Request, device, shots=None):
+def build_problem(req: Request, device, shots=None, graph_steps: int = 0):
     """(propagator, model) of a request; its sources (`shot_positions`, or the given (x, z)
-    arrays), receivers on every column 2 cells deep."""
+    arrays), receivers on every column 2 cells deep.  graph_steps > 0: acoustic modelling
+    replays HIP graphs of that many steps (`adfwi_amd.propagator.graphs`)."""
     import torch
     from ..model import AcousticModel, IsotropicElasticModel
     from ..propagator import AcousticPropagator, ElasticPropagator
@@ -185,6 +186,7 @@ def build_problem(req: Request, device, shots=None):
         model = AcousticModel(0, 0, nx, nz, req.dx, req.dx, vp=vp, rho=gardner(vp), vp_grad=grad,
                               nabc=req.nabc, free_surface=False, device=device)
         prop = AcousticPropagator(model, survey, device=device)
+        prop.graph_steps = int(graph_steps)
     else:
         vp, vs = elastic_layers(rng, nz, nx, 6, 1800, 4500, 200.0)
         model = IsotropicElasticModel(0, 0, nx, nz, req.dx, req.dx, vp=vp, vs=vs, rho=gardner(vp),
@@ -194,10 +196,11 @@ def build_problem(req: Request, device, shots=None):
     return prop, model
 
 
-def execute(req: Request, device):
-    """Runs one request on the current stream; returns (tensors of the result, digest dict)."""
+def execute(req: Request, device, graph_steps: int = 0):
+    """Runs one request on the current stream; returns (tensors of the result, digest dict).
+    graph_steps > 0: an acoustic modelling request replays HIP graphs of that many steps."""
     import torch
-    prop, model = build_problem(req, device)
+    prop, model = build_problem(req, device, graph_steps=graph_steps if req.mode == "forward" else 0)
     comps = ("p",) if req.kind == "acoustic" else ("vx", "vz")
     if req.mode == "forward":
         with torch.no_grad():
@@ -273,13 +276,13 @@ def _packed_acoustic_gradient(reqs: List[Request], device):
     return results
 
 
-def execute_pack(reqs: List[Request], device):
+def execute_pack(reqs: List[Request], device, graph_steps: int = 0):
     """Runs requests with one `pack_key` in shared launches; returns [(result, digest)] in the
     members' order.  Each result is the member's own: its records (shots are independent), or
-    its own gradient and loss."""
+    its own gradient and loss.  graph_steps: as in `execute`, for modelling packs."""
     import torch
     if len(reqs) == 1:
-        return [execute(reqs[0], device)]
+        return [execute(reqs[0], device, graph_steps)]
     r0 = reqs[0]
     key = pack_key(r0)
     if key is None or any(pack_key(r) != key for r in reqs):
@@ -289,7 +292,7 @@ def execute_pack(reqs: List[Request], device):
         return res if res is not None else [execute(r, device) for r in reqs]
     pos = [shot_positions(r) for r in reqs]
     shots = (np.concatenate([p[0] for p in pos]), np.concatenate([p[1] for p in pos]))
-    prop, _ = build_problem(r0, device, shots=shots)
+    prop, _ = build_problem(r0, device, shots=shots, graph_steps=graph_steps)
     comps = ("p",) if r0.kind == "acoustic" else ("vx", "vz")
     with torch.no_grad():
         rec = prop.forward()

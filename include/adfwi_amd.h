@@ -110,6 +110,11 @@ typedef struct adfwi_acoustic_fwd_args {
                               surface) */
   int32_t fuse_steps;      /* 1: advance two steps per launch where possible; 0: one */
   int32_t reserved2;
+  const int32_t *t_base;   /* device int or NULL: added at run time to every step index of the
+                              call (t0 .. t0+nsteps-1 are then relative), so that one captured
+                              launch sequence (a HIP graph) serves every segment of a run; the
+                              caller keeps *t_base + t0 + nsteps <= nt (steps outside [0, nt)
+                              are skipped).  Requires ckpt == NULL. */
 } adfwi_acoustic_fwd_args;
 
 /* Forward time stepping (replaces step_forward, acoustic_kernels.py:40-176). */

@@ -167,3 +167,36 @@ def test_product_refuses_cpu_device():
     from adfwi_amd import _native as nat
     with pytest.raises(nat.NativeLibraryError):
         nat.require_gpu("cpu")
+
+
+def test_ctypes_structures_match_the_c_header(tmp_path):
+    """Every ctypes mirror in adfwi_amd/_native.py has the size and field offsets of its struct
+    in include/adfwi_amd.h (compiled with the host C compiler; field names must match)."""
+    import os
+    import shutil
+    import subprocess
+    from adfwi_amd import _native as nat
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    pairs = {"AcousticDesc": "adfwi_acoustic_desc", "Layout": "adfwi_layout",
+             "AcousticFwdArgs": "adfwi_acoustic_fwd_args", "AcousticAdjArgs": "adfwi_acoustic_adj_args",
+             "ElasticDesc": "adfwi_elastic_desc", "ElasticFwdArgs": "adfwi_elastic_fwd_args",
+             "ElasticAdjArgs": "adfwi_elastic_adj_args"}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "adfwi_amd.h"', "int main(void) {"]
+    expect = []
+    for py, c in pairs.items():
+        cls = getattr(nat, py)
+        lines.append(f'printf("{py} sizeof %zu\\n", sizeof({c}));')
+        expect.append(f"{py} sizeof {ctypes.sizeof(cls)}")
+        for f in cls._fields_:
+            lines.append(f'printf("{py} {f[0]} %zu\\n", offsetof({c}, {f[0]}));')
+            expect.append(f"{py} {f[0]} {getattr(cls, f[0]).offset}")
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert [g for g in got if g] == expect
