@@ -148,10 +148,11 @@ class AcousticGeometry:
         # two steps per launch (the intermediate step's receivers written from inside the kernel)
         self.fuse_steps = int(os.environ.get("ADFWI_FUSE_STEPS", "1"))
         self.rcv_map = self._receiver_map(rcv_z, rcv_x) if self.fuse_steps else None
-        # what a captured launch sequence depends on besides its buffers
-        self.content_key = (self.Z, self.X, self.nt, self.free_surface, self.fs, self.shots_per_block,
+        # what a captured launch sequence depends on besides its buffers and the source positions
+        # (the kernels read those from device memory, so a replay can take new ones)
+        self.content_key = (self.Z, self.X, self.nt, self.S, self.free_surface, self.fs, self.shots_per_block,
                             self.adj_shots_per_block, self.row_unroll, self.tile_z, self.fuse_steps,
-                            src_z.tobytes(), src_x.tobytes(), rcv_z.tobytes(), rcv_x.tobytes())
+                            rcv_z.tobytes(), rcv_x.tobytes())
         desc = self.desc(None, None)
         self.layout = nat.Layout()
         nat.check(nat.load().adfwi_acoustic_get_layout(ctypes.byref(desc), ctypes.byref(self.layout)),

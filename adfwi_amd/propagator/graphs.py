@@ -9,12 +9,14 @@ run is then `nt // segment` replays of that graph plus one graph for the last, s
 window.  The same launches run in the same order as in `run_forward`, so records and energy
 maps are bit-identical to it.
 
-A context owns static buffers (coefficients, wavelets, state, records, energy) and its graphs;
-`graphed_forward` keeps a few contexts per host thread, keyed by the geometry's content, so
-that later runs of the same shape (the requests against one model in the serving loop) only
-copy their inputs in and replay.  Graphs record no memset: buffers are cleared outside the
-graphs, and the time-base update is an elementwise kernel.
+A context owns static buffers (coefficients, wavelets, source positions, state, records,
+energy) and its graphs; `graphed_forward` keeps a few contexts per host thread, keyed by the
+geometry's content (grid, nt, shot count, receivers, tiling), so that later runs of the same
+shape only copy their inputs in and replay.  A shape is captured on its second run (the first
+runs directly), so one-off shapes pay no capture.  Graphs record no memset: buffers are
+cleared outside the graphs, and the time-base update is an elementwise kernel.
 """
+import copy
 import ctypes
 import threading
 from collections import OrderedDict
@@ -34,6 +36,12 @@ class GraphedForward:
         if segment < 4 or segment % 4:
             raise ValueError("the graph window must be a positive multiple of 4 steps")
         dev = geom.device
+        # a private copy of the device index tables: replays write the source positions of
+        # each run into it, never into a geometry that a propagator still holds
+        geom = copy.copy(geom)
+        geom.src_z, geom.src_x = geom.src_z.clone(), geom.src_x.clone()
+        geom.rcv_z, geom.rcv_x = geom.rcv_z.clone(), geom.rcv_x.clone()
+        geom.rcv_map = None if geom.rcv_map is None else geom.rcv_map.clone()
         self.geom, self.K = geom, int(segment)
         S, Z, X, nt, R, pitch = geom.S, geom.Z, geom.X, geom.nt, geom.R, geom.pitch
         L = geom.layout
@@ -93,10 +101,14 @@ class GraphedForward:
         a.t_base = self.tb.data_ptr()
         return a
 
-    def __call__(self, coef: torch.Tensor, src: torch.Tensor) -> dict:
-        """Same result dictionary as run_forward(record=True, energy_u_from=..., ckpt_every=0)."""
+    def __call__(self, coef: torch.Tensor, src: torch.Tensor, geom=None) -> dict:
+        """Same result dictionary as run_forward(record=True, energy_u_from=..., ckpt_every=0);
+        `geom` (same content key) supplies the source positions of this run."""
         self.coef.copy_(coef)
         self.src.copy_(src)
+        if geom is not None:
+            self.geom.src_z.copy_(geom.src_z)
+            self.geom.src_x.copy_(geom.src_x)
         self.work[0].zero_()
         self.tb.zero_()
         if self.energy is not None:
@@ -113,20 +125,31 @@ class GraphedForward:
         return out
 
 
-def graphed_forward(geom, coef: torch.Tensor, src: torch.Tensor, energy_u_from: Optional[int], segment: int):
+def graphed_forward(geom, coef: torch.Tensor, src: torch.Tensor, energy_u_from: Optional[int], segment: int,
+                    capture_after: int = 1):
     """run_forward(record=True, energy_u_from, ckpt_every=0) through a cached GraphedForward of
-    this thread (built on first use for the geometry's content, window and energy cut)."""
+    this thread.  A shape (geometry content, window, energy cut) runs directly its first
+    `capture_after` times and is captured on the next run; contexts over MAX_CONTEXT_BYTES
+    are never built."""
+    from .acoustic_kernels import run_forward
     cache = getattr(_LOCAL, "cache", None)
     if cache is None:
         cache = _LOCAL.cache = OrderedDict()
+        _LOCAL.seen = OrderedDict()
     key = (geom.content_key, int(segment), energy_u_from, geom.device.index)
     ctx = cache.get(key)
     if ctx is None:
+        seen = _LOCAL.seen
+        n = seen.pop(key, 0)
+        seen[key] = n + 1
+        while len(seen) > 64:
+            seen.popitem(last=False)
+        if n < capture_after:
+            return run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from, ckpt_every=0)
         ctx = GraphedForward(geom, segment, energy_u_from)
-        if ctx.nbytes > MAX_CONTEXT_BYTES:     # not kept: its graphs must finish before it goes
-            out = ctx(coef, src)
-            torch.cuda.current_stream(geom.device).synchronize()
-            return out
+        if ctx.nbytes > MAX_CONTEXT_BYTES:
+            del ctx
+            return run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from, ckpt_every=0)
         cache[key] = ctx
         while len(cache) > MAX_CONTEXTS:
             _, old = cache.popitem(last=False)
@@ -134,7 +157,7 @@ def graphed_forward(geom, coef: torch.Tensor, src: torch.Tensor, energy_u_from: 
             del old
     else:
         cache.move_to_end(key)
-    return ctx(coef, src)
+    return ctx(coef, src, geom)
 
 
 def clear_cache() -> None:
@@ -143,3 +166,5 @@ def clear_cache() -> None:
     if cache:
         torch.cuda.synchronize()
         cache.clear()
+    if getattr(_LOCAL, "seen", None) is not None:
+        _LOCAL.seen.clear()

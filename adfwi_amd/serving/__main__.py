@@ -2,7 +2,7 @@
 it, the plain path (one request at a time, first come first served); prints one JSON line.
 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
-                                [--model-pool 6]
+                                [--model-pool 6] [--graph-steps 64] [--no-pack] [--burst]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
@@ -10,7 +10,9 @@ that of a single FIFO server with those service times and the same arrivals.  Th
 options shrink nt, the shot count and the grid of every request so that a bounded run finishes
 in minutes; the JSON line states them.  --model-pool N draws every request against one of N
 shared models (the shot requests of an inversion), so that requests of one model and mode are
-packed into shared launches.
+packed into shared launches (--no-pack turns packing off); --graph-steps sets the HIP-graph
+window of acoustic modelling (0: direct launches).  --burst submits the whole stream at once
+(saturated throughput; the plain FIFO then starts every request at t = 0 too).
 """
 import argparse
 import copy
@@ -41,6 +43,9 @@ def main():
     ap.add_argument("--max-nz", type=int, default=256)
     ap.add_argument("--max-nx", type=int, default=768)
     ap.add_argument("--model-pool", type=int, default=0)
+    ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--no-pack", action="store_true")
+    ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     from .loop import ServingLoop, run_alone
     from .requests import make_stream
@@ -52,7 +57,7 @@ def main():
     # warm-up outside the measurement: one small request of each kind and mode
     warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
                        shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
-    ServingLoop("cuda:0", max_slots=2).run(warm, realtime=False)
+    ServingLoop("cuda:0", max_slots=2, graph_steps=args.graph_steps).run(warm, realtime=False)
     base = stream(1.0)
     t0 = time.perf_counter()
     alone = run_alone(copy.deepcopy(base))
@@ -66,21 +71,26 @@ def main():
                       "mix": "60% acoustic / 40% isotropic elastic, 50% forward / 50% gradient",
                       "nz": [128, args.max_nz], "nx": [256, args.max_nx], "nt": list(nt_range),
                       "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool},
-           "slots": args.slots,
+           "slots": args.slots, "packing": not args.no_pack, "graph_steps": args.graph_steps,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),
            "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
     digests = {o.rid: o.digest for o in alone}
-    for load in [float(x) for x in args.load.split(",")]:
+    loads = [0.0] if args.burst else [float(x) for x in args.load.split(",")]
+    for load in loads:
         rate = load * capacity
-        reqs = stream(rate)
-        rep = ServingLoop("cuda:0", max_slots=args.slots).run(reqs, realtime=True)
+        reqs = stream(rate if not args.burst else 1.0)
+        if args.burst:
+            for r in reqs:
+                r.arrival = 0.0
+        rep = ServingLoop("cuda:0", max_slots=args.slots, pack=not args.no_pack,
+                          graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
         s = rep.summary()
         fifo = fifo_latencies([r.arrival for r in reqs], service)
         same = all(o.digest.keys() == digests[o.rid].keys() and all(
             abs(o.digest[k][1] - digests[o.rid][k][1]) <= 1e-6 * max(abs(digests[o.rid][k][1]), 1e-30)
             for k in o.digest) for o in rep.outcomes if o.error is None)
-        run = {"load": load, "rate_req_per_s": rate, "serving": s,
+        run = {"load": "burst" if args.burst else load, "rate_req_per_s": None if args.burst else rate, "serving": s,
                "plain_fifo": {"latency_p50_s": float(np.percentile(fifo, 50)),
                               "latency_p99_s": float(np.percentile(fifo, 99)),
                               "throughput_req_per_s": len(reqs) / max(fifo[-1] + reqs[-1].arrival, 1e-9)},

@@ -12,12 +12,13 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-def _problem(name, vp_scale=1.0):
+def _problem(name, vp_scale=1.0, src_shift=0):
     from adfwi_amd.propagator.acoustic_kernels import AcousticGeometry, coefficient_fields
     g, m = acoustic_inputs(name)
     nabc = m["nabc"]
     geom = AcousticGeometry(m["nz"] + 2 * nabc, m["nx"] + 2 * nabc, m["nt"], m["fs"], nabc,
-                            g["src_z"] + nabc, g["src_x"] + nabc, g["rcv_z"] + nabc, g["rcv_x"] + nabc, DEV)
+                            g["src_z"] + nabc, g["src_x"] + nabc + src_shift, g["rcv_z"] + nabc,
+                            g["rcv_x"] + nabc, DEV)
     vp = torch.tensor(g["vp"], device=DEV) * vp_scale
     a1, k1, a2, k2, k3 = coefficient_fields(vp, torch.tensor(g["rho"], device=DEV),
                                             torch.tensor(g["damp"], dtype=torch.float32, device=DEV),
@@ -51,11 +52,17 @@ def test_graph_replays_equal_direct_launches(name, window):
         _same(res, ref)
         run_forward(geom, coef_b if trial % 2 == 0 else coef_a, src, energy_u_from=eu)   # work between replays
         torch.cuda.synchronize()
-    # a new geometry object with the same content replays the cached graphs
+    # new geometry objects with the same content replay the cached graphs, also with other
+    # source positions (read from device memory); the first geometry keeps its own sources
+    assert len(graphs._LOCAL.cache) == 1
     geom2, _, _, _ = _problem(name)
-    n = len(graphs._LOCAL.cache)
     _same(graphed_forward_checked(geom2, coef_a, src, eu, window), ref_a)
-    assert len(graphs._LOCAL.cache) == n
+    geom3, _, _, _ = _problem(name, src_shift=3)
+    ref_3 = run_forward(geom3, coef_a, src, energy_u_from=eu)
+    assert not np.array_equal(ref_3["rec"][0].cpu().numpy(), ref_a["rec"][0].cpu().numpy())
+    _same(graphed_forward_checked(geom3, coef_a, src, eu, window), ref_3)
+    assert len(graphs._LOCAL.cache) == 1
+    _same(run_forward(geom, coef_a, src, energy_u_from=eu), ref_a)
     graphs.clear_cache()
 
 
@@ -81,7 +88,7 @@ def test_propagator_forward_with_graphs_equals_direct():
     rcv = Receiver(nt, 1e-3)
     rcv.add_receivers(np.arange(nx), np.full(nx, 3), "pr")
     outs = []
-    for steps in (0, 32, 32):
+    for steps in (0, 32, 32, 32):   # direct, then the first graph run (direct), capture, replay
         model = AcousticModel(0, 0, nx, nz, 10.0, 10.0, vp=vp, rho=np.full((nz, nx), 2000.0, np.float32),
                               vp_grad=False, nabc=10, free_surface=False, device=DEV)
         prop = AcousticPropagator(model, Survey(src, rcv), device=DEV)
