@@ -90,6 +90,7 @@ _SIGNATURES = {
     "adfwi_elastic_get_layout": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(Layout)]),
     "adfwi_elastic_forward": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticFwdArgs), _vp]),
     "adfwi_elastic_adjoint": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticAdjArgs), _vp]),
+    "adfwi_trace_l2": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, ctypes.c_float, _i32, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

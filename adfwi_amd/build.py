@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libadfwi_amd.so")
-SOURCES = ["common.hip", "acoustic.hip", "elastic.hip"]
+SOURCES = ["common.hip", "acoustic.hip", "elastic.hip", "records.hip"]
 ARCH = os.environ.get("ADFWI_OFFLOAD_ARCH", "gfx950")
 
 

@@ -57,9 +57,10 @@ def profiled_traffic(kernel, workload):
 
 
 def l2_misfit(syn, obs, dt):
-    """Waveform L2 misfit sum_{s,r} sqrt(sum_t rsd^2 dt) (ADFWI/fwi/misfit/L2.py:26-29)."""
-    rsd = obs - syn
-    return torch.sum(torch.sqrt(torch.sum(rsd * rsd * dt, dim=1)))
+    """Waveform L2 misfit sum_{s,r} sqrt(sum_t rsd^2 dt) (ADFWI/fwi/misfit/L2.py:26-29), its
+    value and adjoint source from one kernel (adfwi_amd.fwi.Misfit_waveform_L2)."""
+    from adfwi_amd.fwi import Misfit_waveform_L2
+    return Misfit_waveform_L2(dt).forward(obs, syn)
 
 
 def build_problem(wl, dev):

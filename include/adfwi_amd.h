@@ -219,6 +219,19 @@ typedef struct adfwi_elastic_adj_args {
  * autograd replay of step_forward_PML_4order (elastic_kernels.py:975-1008 checkpoints). */
 int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a, adfwi_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Record post-processing                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Waveform L2 misfit per trace of records a against b ([nshots][nt][nrec], device):
+ * trace_loss[s][r] = sqrt(sum_t (n - b)^2 dt) with n = a / max_t |a| per trace when
+ * normalize_a (the FWI drivers' trace normalisation, fwi/acoustic_fwi.py:149-150) or n = a
+ * (Misfit_waveform_L2, fwi/misfit/L2.py:26-29); the misfit is the sum of trace_loss.
+ * grad_a / grad_b (or NULL): d misfit / d a and d misfit / d b, through the normalisation
+ * (whose max routes its gradient to the first sample reaching it). */
+int adfwi_trace_l2(const float *a, const float *b, int32_t nshots, int32_t nt, int32_t nrec, float dt,
+                   int32_t normalize_a, float *trace_loss, float *grad_a, float *grad_b, adfwi_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
