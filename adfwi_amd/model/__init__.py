@@ -98,7 +98,14 @@ class AbstractModel(torch.nn.Module):
         return kw
 
     def save(self, filename: str) -> None:
-        np.savez(filename, **self.get_clone_data())
+        """The clone data as an .npz (base.py:284-296) with plain arrays only: a bound is a
+        float pair, NaN for "no bound", so `load_model_data` reads it without unpickling."""
+        kw = self.get_clone_data()
+        for par in self.pars:
+            lo, hi = kw[par + "_bound"]
+            kw[par + "_bound"] = np.array([np.nan if lo is None else lo, np.nan if hi is None else hi],
+                                          dtype=np.float64)
+        np.savez(filename, **kw)
 
     def clip_params(self) -> None:
         """In-place clamp of every bounded parameter (base.py:298-308)."""
@@ -256,4 +263,21 @@ class AnisotropicElasticModel(_ElasticBase):
         return elastic_moduli_for_TI(CC, anisotropic_type=self.anisotropic_type)
 
 
-__all__ = ["AbstractModel", "AcousticModel", "IsotropicElasticModel", "AnisotropicElasticModel"]
+__all__ = ["AbstractModel", "AcousticModel", "IsotropicElasticModel", "AnisotropicElasticModel", "load_model_data"]
+
+
+def load_model_data(filename: str) -> dict:
+    """Keyword arguments of a model saved by `save` (read with allow_pickle=False): every
+    parameter array, its bound ([None, None] when unbounded) and grad flag, and the grid."""
+    out = {}
+    with np.load(filename, allow_pickle=False) as f:
+        for k in f.files:
+            v = f[k]
+            if k.endswith("_bound"):
+                lo, hi = (float(x) for x in v)
+                out[k] = [None, None] if np.isnan(lo) or np.isnan(hi) else [lo, hi]
+            elif v.ndim == 0:
+                out[k] = v.item()
+            else:
+                out[k] = v
+    return out

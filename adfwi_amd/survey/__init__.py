@@ -136,3 +136,6 @@ class Survey:
 
     def __repr__(self):
         return "Survey Information:\n" + repr(self.source) + "\n" + repr(self.receiver)
+
+
+from .data import SeismicData  # noqa: E402  (uses Survey above)
