@@ -65,6 +65,7 @@ struct ElDev {
   float dt, dt_dx, dt_dz, dx, dz, c0, c1;
   long long plane, field;
   const float *coef, *pml_x, *pml_z, *src, *src_m, *damp;
+  const float *src_fz;   // [S] vertical-force weight of each shot's wavelet, or NULL
   const int *src_z, *src_x, *rcv_z, *rcv_x;
 };
 
@@ -265,8 +266,13 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
         const float nx_x = (pxn * cur.u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
         const float nx_z = (pzn * cur.u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
-        const float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
-        const float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
+          const float hf = 0.5f * (d.src_fz[s] * wsrc);
+          nz_x = nz_x + hf;
+          nz_z = nz_z + hf;
+        }
         bst(nx_x, rOut, vs, F(VX_X) + orr); bst(nx_z, rOut, vs, F(VX_Z) + orr);
         bst(nz_x, rOut, vs, F(VZ_X) + orr); bst(nz_z, rOut, vs, F(VZ_Z) + orr);
       }
@@ -364,6 +370,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
     auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
     const int zs = d.src_z[s], xs = d.src_x[s];
     const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float gmt = 0.0f;   // wavelet gradient of the moment-tensor injection, when a force term follows
     // forward velocity totals (rows k-2..k+2) and stress totals (txz/tzz rows k-4..k, txx k-2..k)
     float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;
     float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
@@ -505,8 +512,11 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float lzz_x = cur.ls[2] + lt_zz, lzz_z = cur.ls[3] + lt_zz;
         const float lxz_x = cur.ls[4] + lt_xz, lxz_z = cur.ls[5] + lt_xz;
         const bool own_k = out_col && k >= z0 && k < end_row;
-        if (own_k && k == zs && c == xs && gsrc)
-          gsrc[(long long)s * d.nt + t] = (mxx * (lxx_x + lxx_z) + mzz * (lzz_x + lzz_z)) + mxz * (lxz_x + lxz_z);
+        if (own_k && k == zs && c == xs && gsrc) {
+          const float gm = (mxx * (lxx_x + lxx_z) + mzz * (lzz_x + lzz_z)) + mxz * (lxz_x + lxz_z);
+          if (d.src_fz) gmt = gm;        // the force term of row zs follows two rows later
+          else gsrc[(long long)s * d.nt + t] = gm;
+        }
         const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
         const float sz_keep = pzn * pzi, sz_a = d.dt_dz * pzi;
         if (own_k) {
@@ -601,6 +611,8 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
         const float l1 = cur.lr[0], l2 = cur.lr[1], l3 = cur.lr[2], l4 = cur.lr[3];
+        if (d.src_fz && gsrc && r == zs && c == xs)   // vertical force: f w / 2 entered both vz splits
+          gsrc[(long long)s * d.nt + t] = gmt + 0.5f * (d.src_fz[s] * (l3 + l4));
         bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);
         bst(vx_keep * l3 + lvz, rLo, vs, F(VZ_X) + o); bst(vz_keep * l4 + lvz, rLo, vs, F(VZ_Z) + o);
         const float dxf_txx = c0 * (txp1 - xx0) + c1 * (txp2 - txm1);
@@ -809,6 +821,7 @@ __global__ __launch_bounds__(ENT) void el_abl_fwd(ElDev d, const float *__restri
         nvx = vx0 + (dt * cur.b[0]) * (dxf_txx / dx + dzb_txz / dz);
         nvz = vz0 + (dt * cur.b[1]) * (dxb_txz / dx + dzf_tzz / dz);
       }
+      if (d.src_fz && r == zs && c == xs) nvz = nvz + d.src_fz[s] * wsrc;   // vertical force
       if (own_r) {
         const unsigned orr = rb(r);
         bst(nvx * cur.dm, rOut, vs, F(VX) + orr);
@@ -921,6 +934,7 @@ __global__ __launch_bounds__(ENT) void el_abl_adj(ElDev d, const float *__restri
     auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
     const int zs = d.src_z[s], xs = d.src_x[s];
     const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float gmt = 0.0f;   // wavelet gradient of the moment-tensor injection, when a force term follows
     float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;     // forward velocities before the step
     float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
     float xz0 = 0.f, xz1 = 0.f, xz2 = 0.f, xz3 = 0.f, xz4 = 0.f;     // forward stresses after the step
@@ -1037,8 +1051,11 @@ __global__ __launch_bounds__(ENT) void el_abl_adj(ElDev d, const float *__restri
         const float lzz = (d.fsurf && k == 2) ? 0.0f : scur.ls[1] + lt_zz;
         const float lxz = scur.ls[2] + lt_xz;
         const bool own_k = out_col && k >= z0 && k < end_row;
-        if (own_k && k == zs && c == xs && gsrc)
-          gsrc[(long long)s * d.nt + t] = (mxx * lxx + mzz * lzz) + mxz * lxz;
+        if (own_k && k == zs && c == xs && gsrc) {
+          const float gm = (mxx * lxx + mzz * lzz) + mxz * lxz;
+          if (d.src_fz) gmt = gm;        // the force term of row zs follows two rows later
+          else gsrc[(long long)s * d.nt + t] = gm;
+        }
         if (own_k) {
           bst(lxx, rLo, vs, F(TXX) + o); bst(lzz, rLo, vs, F(TZZ) + o); bst(lxz, rLo, vs, F(TXZ) + o);
         }
@@ -1110,6 +1127,8 @@ __global__ __launch_bounds__(ENT) void el_abl_adj(ElDev d, const float *__restri
       const bool own_r = out_col && ((r >= z0 && r < end_row) || (first && r >= 0 && r < 2));
       if (own_r) {
         const unsigned o = rb(r);
+        if (d.src_fz && gsrc && r == zs && c == xs)   // vertical force entered vz before the taper
+          gsrc[(long long)s * d.nt + t] = gmt + d.src_fz[s] * uz0;
         bst(ux0 + lvx, rLo, vs, F(VX) + o);
         bst(uz0 + lvz, rLo, vs, F(VZ) + o);
         if (r >= z0) {
@@ -1202,6 +1221,7 @@ ElDev el_dev(const adfwi_elastic_desc *d, const adfwi_layout &L) {
   v.plane = (long long)d->nz * d->pitch;
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.pml_x = d->pml_x; v.pml_z = d->pml_z; v.src = d->src; v.src_m = d->src_m;
+  v.src_fz = d->src_fz;
   v.damp = d->damp;
   v.src_z = d->src_z; v.src_x = d->src_x; v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
   return v;

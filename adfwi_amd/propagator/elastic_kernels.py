@@ -56,11 +56,11 @@ def chunk_ends(nt: int, segments: int) -> List[int]:
 
 class ElasticGeometry:
     """Device-side description of one elastic problem: padded grid, sources (with their
-    moment-tensor factors), receivers, tiling."""
+    moment-tensor factors and, optionally, vertical-force weights), receivers, tiling."""
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, src_z, src_x, src_m: torch.Tensor,
                  rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0,
-                 damping: bool = False):
+                 damping: bool = False, src_fz: Optional[torch.Tensor] = None):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
@@ -84,6 +84,12 @@ class ElasticGeometry:
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
         self.src_m = src_m.detach().to(device=self.device, dtype=torch.float32).contiguous()
+        # vertical-force weight per shot (None: moment-tensor sources only, as in the reference)
+        self.src_fz = (None if src_fz is None else
+                       torch.as_tensor(src_fz).detach().to(device=self.device, dtype=torch.float32).reshape(-1)
+                       .contiguous())
+        if self.src_fz is not None and self.src_fz.numel() != self.S:
+            raise ValueError(f"src_fz needs one weight per shot ({self.S}), got {self.src_fz.numel()}")
         self.shots_per_wave = int(shots_per_wave or os.environ.get("ADFWI_EL_SHOTS_PER_WAVE", "0"))
         self.tile_z = int(tile_z or os.environ.get("ADFWI_EL_TILE_Z", "0"))
         self.layout = nat.Layout()
@@ -104,6 +110,7 @@ class ElasticGeometry:
         d.coef, d.pml_x, d.pml_z, d.src = ptr(coef), ptr(pml_x), ptr(pml_z), ptr(src)
         d.damp = ptr(damp)
         d.src_m = self.src_m.data_ptr()
+        d.src_fz = None if self.src_fz is None else self.src_fz.data_ptr()
         d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
         d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
         return d
@@ -248,13 +255,16 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    lamu, lam, bx: torch.Tensor, bz: torch.Tensor, CC: List[torch.Tensor],
                    fd_order: int = 4, n_segments: int = 1,
                    device: torch.device = torch.device("cpu"), dtype=torch.float32,
-                   geometry: Optional[ElasticGeometry] = None, ckpt_every: int = 0) -> Dict[str, torch.Tensor]:
+                   geometry: Optional[ElasticGeometry] = None, ckpt_every: int = 0,
+                   src_fz: Optional[torch.Tensor] = None) -> Dict[str, torch.Tensor]:
     """Elastic forward modelling with the reference's signature and result dictionary
     (elastic_kernels.py:917-1037), 4th order: the split-field PML scheme for abc_type 'pml' (any
     case, as the reference's driver tests it), the multiplicative-damping scheme with ``damp``
     otherwise.  ``n_segments`` keeps its meaning for the energy maps (summed over the last step of
     every time chunk); memory is governed by the sqrt(nt) wavefield checkpoints of the HIP
-    adjoint."""
+    adjoint.  ``src_fz`` (extension, no counterpart in the reference): per-shot vertical-force
+    weights f; the shot's wavelet w then also enters vz at the source point right after the
+    velocity update (f w / 2 into each PML split, f w in the damping scheme)."""
     nat.require_gpu(device)
     if dtype != torch.float32:
         raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
@@ -282,7 +292,8 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
             src_m = torch.stack([(-1 / 3) * MTd[:, 0, 0], (-1 / 3) * MTd[:, 2, 2], (-1 / 3) * MTd[:, 0, 2]], dim=1)
         geometry = ElasticGeometry(Z, X, nt, free_surface, (src_z + zoff).cpu().numpy(),
                                    (src_x + nabc).cpu().numpy(), src_m, (rcv_z + zoff).cpu().numpy(),
-                                   (rcv_x + nabc).cpu().numpy(), device, aniso=aniso, damping=not use_pml)
+                                   (rcv_x + nabc).cpu().numpy(), device, aniso=aniso, damping=not use_pml,
+                                   src_fz=src_fz)
     elif geometry.aniso != aniso:
         geometry.aniso = aniso
     if use_pml:

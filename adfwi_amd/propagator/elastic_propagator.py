@@ -51,6 +51,10 @@ class ElasticPropagator(torch.nn.Module):
         self.src_n = self.source.num
         self.wavelet = numpy2tensor(self.source.get_wavelet(), self.dtype).to(self.device)
         self.moment_tensor = numpy2tensor(self.source.get_moment_tensor(), self.dtype).to(self.device)
+        # vertical-force sources ("vf", an extension of the reference's moment-tensor sources)
+        types = np.array([str(t).lower() for t in self.source.get_type()])
+        self.force_weight = (torch.tensor((types == "vf").astype(np.float32), device=self.device)
+                             if np.any(types == "vf") else None)
         self.receiver = survey.receiver
         self.rcv_loc = self.receiver.get_loc()
         self.rcv_x = numpy2tensor(self.rcv_loc[:, 0], torch.long).to(self.device)
@@ -89,9 +93,10 @@ class ElasticPropagator(torch.nn.Module):
             else:         # -MT/2 into each split (elastic_kernels.py:358-372)
                 half = mt / 2
                 src_m = torch.stack([-half[:, 0, 0], -half[:, 2, 2], -half[:, 0, 2]], dim=1)
+            fz = None if self.force_weight is None else self.force_weight[torch.as_tensor(shots, device=self.device)]
             g = ElasticGeometry(Z, X, self.nt, self.free_surface, sl[:, 1] + zoff, sl[:, 0] + self.nabc, src_m,
                                 self.rcv_loc[:, 1] + zoff, self.rcv_loc[:, 0] + self.nabc, self.device, aniso=aniso,
-                                damping=damping)
+                                damping=damping, src_fz=fz)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)
@@ -117,4 +122,5 @@ class ElasticPropagator(torch.nn.Module):
                               src_x, src_z, len(shots), wavelet, mt, self.rcv_x, self.rcv_z, self.rcv_n,
                               self.abc_type, self.bcx, self.bcz, self.damp, model.lamu, model.lam,
                               model.bx, model.bz, CC, fd_order=fd_order, n_segments=checkpoint_segments,
-                              device=self.device, dtype=self.dtype, geometry=geom, ckpt_every=self.ckpt_every)
+                              device=self.device, dtype=self.dtype, geometry=geom, ckpt_every=self.ckpt_every,
+                              src_fz=None if self.force_weight is None else self.force_weight[sel])

@@ -33,8 +33,12 @@ class Source:
                 f"  Source z range: {z.min()} - {z.max()} (grids)\n")
 
     def _validate(self, src_wavelet, src_type, src_mt):
-        if src_type.lower() not in ["mt"]:
-            raise ValueError("Source type must be either mt")
+        # "vf" (vertical force, elastic propagator only) has no counterpart in the reference,
+        # which accepts moment-tensor sources alone
+        if src_type.lower() not in ["mt", "vf"]:
+            raise ValueError("Source type must be either mt or vf")
+        if src_type.lower() == "vf":
+            return
         if src_wavelet.shape[0] != self.nt:
             raise ValueError("Source wavelet must have the same length as the number of time samples")
         if src_mt is None:
@@ -53,7 +57,8 @@ class Source:
         self.loc_z.extend(numpy2list(np.asarray(src_z).reshape(-1)))
         self.type.extend([src_type] * n)
         self.wavelet.extend(np.ones((n, self.nt)) * src_wavelet)
-        self.moment_tensor.extend(np.ones((n, 3, 3)) * src_mt)
+        mt = np.zeros((3, 3)) if src_type.lower() == "vf" else src_mt
+        self.moment_tensor.extend(np.ones((n, 3, 3)) * mt)
         self.num += n
 
     def add_source(self, src_x: int, src_z: int, src_wavelet: np.ndarray,
@@ -64,7 +69,7 @@ class Source:
         self.loc_z.append(src_z)
         self.type.append(src_type)
         self.wavelet.append(src_wavelet)
-        self.moment_tensor.append(src_mt)
+        self.moment_tensor.append(np.zeros((3, 3)) if src_type.lower() == "vf" else src_mt)
         self.num += 1
 
     def get_loc(self):

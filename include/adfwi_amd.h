@@ -175,6 +175,10 @@ typedef struct adfwi_elastic_desc {
   const int32_t *src_z, *src_x;   /* [nshots] padded source indices (inside the window) */
   const int32_t *rcv_z, *rcv_x;   /* [nrec] padded receiver indices (inside the window) */
   const float *damp;       /* damping: [nz][pitch] multiplicative taper of vx, vz; PML: unused */
+  const float *src_fz;     /* [nshots] or NULL: vertical-force weight f of each shot's wavelet w,
+                              added to vz at the source point right after the velocity update
+                              (PML: f w / 2 into each vz split; damping: f w, before the taper).
+                              No counterpart in the reference, whose sources are moment tensors. */
 } adfwi_elastic_desc;
 
 /* Tiling and buffer sizes; layout.nfields = 10 (PML split fields) or 5 (damping), layout.reserved = shots per
