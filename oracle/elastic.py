@@ -1,11 +1,15 @@
-"""Oracle: the reference elastic (4th-order, PML) propagator path in eager PyTorch.
+"""Oracle: the reference elastic propagator path in eager PyTorch.
 
 Test infrastructure only.  Restates ADFWI/propagator/elastic_kernels.py:
-DiffCoef (:20-58), the 4th-order D operators (:66-84), pad_torchSingle
-(:175-216), step_forward_PML_4order (:222-423) and forward_kernel (:917-1037,
-PML branch), keeping the fp32 association of every expression, the split-field
-PML, the moment-tensor source (-MT/2 into both splits), the free-surface rows
-and the per-step overwrite of the energy maps.
+DiffCoef (:20-58), the 4th- and 6th-order D operators (:66-108), pad_torchSingle
+(:175-216), step_forward_PML_4order (:222-423), step_forward_PML_6order (:426-584),
+step_forward_ABL_4order (:585-777) and forward_kernel (:917-1037), keeping the
+fp32 association of every expression, the split-field PML, the moment-tensor
+source (-MT/2 into both splits), the free-surface rows and the per-step
+overwrite of the energy maps.  For fd_order 6, 8 and 10 the reference runs
+step_forward_PML_6order, whose three-term operators use the first three
+coefficients of DiffCoef(fd_order // 2) on the window [fd_order // 2, Z - fd_order // 2 - 1]
+with the free-surface rows at h = fd_order // 2 + 1.
 """
 from typing import Dict, List, Tuple
 
@@ -257,6 +261,91 @@ def _steps_tuple(*args):
     return tuple(_steps(*args))
 
 
+def _steps6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int, nn: int,
+            src_x, src_z, dt: float, src_v, mt, rcv_x, rcv_z, bcx, bcz, c11, c13, c15, c33, c35, c55, bx, bz,
+            fdc, txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, vx, vz):
+    """``src_v.shape[-1]`` steps of step_forward_PML_6order (elastic_kernels.py:426-584): the
+    order-4 step with three-term operators (:85-108), the window [nn, Z-nn-1] and the
+    free-surface rows at h = nn + 1."""
+    S, Z, X = vx.shape
+    nt = src_v.shape[-1]
+    R = rcv_x.shape[0]
+    txx_x, txx_z, tzz_x, tzz_z = txx_x.clone(), txx_z.clone(), tzz_x.clone(), tzz_z.clone()
+    txz_x, txz_z, txx, tzz, txz = txz_x.clone(), txz_z.clone(), txx.clone(), tzz.clone(), txz.clone()
+    vx_x, vx_z, vz_x, vz_z, vx, vz = vx_x.clone(), vx_z.clone(), vz_x.clone(), vz_z.clone(), vx.clone(), vz.clone()
+    recs = [torch.zeros((S, nt, R), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    ens = [torch.zeros((nz, nx), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    I, J = slice(nn, Z - nn), slice(nn, X - nn)
+    c0, c1, c2 = fdc[0], fdc[1], fdc[2]
+
+    def sh(a, dz_, dx_):   # a[:, i + dz_, j + dx_] over the window
+        return a[:, nn + dz_:Z - nn + dz_, nn + dx_:X - nn + dx_]
+
+    def dxf(a):
+        return c0 * (sh(a, 0, 1) - sh(a, 0, 0)) + c1 * (sh(a, 0, 2) - sh(a, 0, -1)) + c2 * (sh(a, 0, 3) - sh(a, 0, -2))
+
+    def dzf(a):
+        return c0 * (sh(a, 1, 0) - sh(a, 0, 0)) + c1 * (sh(a, 2, 0) - sh(a, -1, 0)) + c2 * (sh(a, 3, 0) - sh(a, -2, 0))
+
+    def dxb(a):
+        return c0 * (sh(a, 0, 0) - sh(a, 0, -1)) + c1 * (sh(a, 0, 1) - sh(a, 0, -2)) + c2 * (sh(a, 0, 2) - sh(a, 0, -3))
+
+    def dzb(a):
+        return c0 * (sh(a, 0, 0) - sh(a, -1, 0)) + c1 * (sh(a, 1, 0) - sh(a, -2, 0)) + c2 * (sh(a, 2, 0) - sh(a, -3, 0))
+    bxw, bzw = bcx[I, J], bcz[I, J]
+    pxd, pxn = 1 + 0.5 * dt * bxw, 1 - 0.5 * dt * bxw
+    pzd, pzn = 1 + 0.5 * dt * bzw, 1 - 0.5 * dt * bzw
+    dt_dx, dt_dz = dt / dx, dt / dz
+    pxi, pzi = 1.0 / pxd, 1.0 / pzd
+    shots = torch.arange(S, device=vx.device)
+    half = mt / 2
+    h = nn + 1
+    row0 = nn if free_surface else nn + nabc
+    C11, C13, C15, C33, C35, C55 = (a[I, J] for a in (c11, c13, c15, c33, c35, c55))
+    BX, BZ = bx[I, J], bz[I, J]
+    for t in range(nt):
+        dxb_vx, dxb_vz, dzb_vx, dzb_vz = dxb(vx), dxb(vz), dzb(vx), dzb(vz)
+        dxf_vx, dzf_vx, dxf_vz, dzf_vz = dxf(vx), dzf(vx), dxf(vz), dzf(vz)
+        txx_x[:, I, J] = (pxn * txx_x[:, I, J] + dt_dx * (C11 * dxb_vx + C15 * dxb_vz)) * pxi
+        txx_z[:, I, J] = (pzn * txx_z[:, I, J] + dt_dz * (C15 * dzb_vx + C13 * dzb_vz)) * pzi
+        tzz_x[:, I, J] = (pxn * tzz_x[:, I, J] + dt_dx * (C13 * dxb_vx + C35 * dxb_vz)) * pxi
+        tzz_z[:, I, J] = (pzn * tzz_z[:, I, J] + dt_dz * (C35 * dzb_vx + C33 * dzb_vz)) * pzi
+        txz_x[:, I, J] = (pxn * txz_x[:, I, J] + dt_dx * (C15 * dxf_vx + C55 * dxf_vz)) * pxi
+        txz_z[:, I, J] = (pzn * txz_z[:, I, J] + dt_dz * (C55 * dzf_vx + C35 * dzf_vz)) * pzi
+        sv = src_v[shots, t]
+        txx_x[shots, src_z, src_x] += -half[:, 0, 0] * sv
+        txx_z[shots, src_z, src_x] += -half[:, 0, 0] * sv
+        tzz_x[shots, src_z, src_x] += -half[:, 2, 2] * sv
+        tzz_z[shots, src_z, src_x] += -half[:, 2, 2] * sv
+        txz_x[shots, src_z, src_x] += -half[:, 0, 2] * sv
+        txz_z[shots, src_z, src_x] += -half[:, 0, 2] * sv
+        txx[:] = txx_x + txx_z
+        tzz[:] = tzz_x + tzz_z
+        txz[:] = txz_x + txz_z
+        if free_surface:
+            tzz[:, h - 1, :] = 0
+            tzz[:, h - 2, :] = -tzz[:, h, :]
+            txz[:, h - 2, :] = -txz[:, h - 1, :]
+            txz[:, h - 3, :] = -txz[:, h, :]
+        dxf_txx, dzb_txz, dxb_txz, dzf_tzz = dxf(txx), dzb(txz), dxb(txz), dzf(tzz)
+        vx_x[:, I, J] = (pxn * vx_x[:, I, J] + dt * BX * dxf_txx / dx) / pxd
+        vx_z[:, I, J] = (pzn * vx_z[:, I, J] + dt * BX * dzb_txz / dz) / pzd
+        vz_x[:, I, J] = (pxn * vz_x[:, I, J] + dt * BZ * dxb_txz / dx) / pxd
+        vz_z[:, I, J] = (pzn * vz_z[:, I, J] + dt * BZ * dzf_tzz / dz) / pzd
+        vx[:] = vx_x + vx_z
+        vz[:] = vz_x + vz_z
+        if free_surface:
+            J1 = slice(nn + 1, X - nn + 1)
+            vz[:, h - 2, J] = vz[:, h - 1, J]
+            vx[:, h - 2, J] = vz[:, h - 2, J1] - vz[:, h - 2, J] + vz[:, h - 1, J1] - vz[:, h - 1, J] + vx[:, h, J]
+            vz[:, h - 3, J] = vz[:, h - 2, J]
+        for k, f in enumerate((txx, tzz, txz, vx, vz)):
+            recs[k][:, t, :] = f[:, rcv_z, rcv_x]
+            ens[k] = torch.sum(f * f, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+    return (txx_x, txx_z, tzz_x, tzz_z, txz_x, txz_z, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, vx, vz,
+            *recs, *ens)
+
+
 def padded_fields(bx, bz, CC, bcx, bcz, nabc: int, free_surface: bool, fd_order: int = 4):
     """Pads the coefficient fields the PML kernel reads (elastic_kernels.py:935-953)."""
     fso = fd_order // 2
@@ -277,11 +366,12 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    lamu, lam, bx: torch.Tensor, bz: torch.Tensor, CC: List[torch.Tensor],
                    fd_order: int = 4, n_segments: int = 1,
                    device=torch.device("cpu"), dtype=torch.float32) -> Dict[str, torch.Tensor]:
-    """Elastic forward modelling, elastic_kernels.py:917-1037 semantics (order 4): the split-field
-    PML branch for abc_type 'pml' (any case, :948, :976), the damping (ABL) branch otherwise."""
-    if fd_order != 4:
-        raise NotImplementedError("oracle covers the 4th-order branches")
+    """Elastic forward modelling, elastic_kernels.py:917-1037 semantics: the split-field PML
+    branch for abc_type 'pml' (any case, :948, :976; order 4, or the three-term step for orders
+    6, 8, 10), the damping (ABL) branch otherwise (order 4)."""
     pml = abc_type.lower() == "pml"
+    if fd_order != 4 and not (pml and fd_order in (6, 8, 10)):
+        raise NotImplementedError("oracle covers the 4th-order branches and the PML branch of orders 6-10")
     fso = fd_order // 2
     X = nx + 2 * nabc
     Z = nz + (nabc + fso if free_surface else 2 * nabc + fso)
@@ -298,7 +388,11 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
     ens = [torch.zeros((nz, nx), dtype=dtype, device=device) for _ in range(5)]
     t0 = 0
     for chunk in torch.chunk(src_v, n_segments, dim=-1):
-        if pml:
+        if pml and fd_order != 4:
+            out = checkpoint(_steps6, free_surface, nx, nz, dx, dz, nabc, fso, sx, sz, dt, chunk, MT, rx, rz,
+                             f["bcx"], f["bcz"], f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
+                             f["bx"], f["bz"], fdc, *state, use_reentrant=True)
+        elif pml:
             out = checkpoint(_steps_tuple, free_surface, nx, nz, dx, dz, nabc, sx, sz, dt, chunk, MT, rx, rz,
                              f["bcx"], f["bcz"], f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
                              f["bx"], f["bz"], fdc, *state, use_reentrant=True)

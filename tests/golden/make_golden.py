@@ -92,7 +92,7 @@ def acoustic_case(ak, bc, free_surface, seed):
     return out
 
 
-def elastic_case(ek, bc, free_surface, vti, seed, abc="PML"):
+def elastic_case(ek, bc, free_surface, vti, seed, abc="PML", order=4):
     rng = np.random.default_rng(seed)
     nz, nx, nabc, dx, dt, nt = 16, 24, 6, 5.0, 4e-4, 120
     vp = smooth_field(rng, nz, nx, 1800.0, 3000.0)
@@ -122,7 +122,7 @@ def elastic_case(ek, bc, free_surface, vti, seed, abc="PML"):
     else:
         damp = np.zeros((1, 1))
     wts = rng.standard_normal((5, 2, nt, nx)).astype(np.float32)
-    out = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, free_surface=int(free_surface),
+    out = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, free_surface=int(free_surface), fd_order=order,
                c11=c11, c13=c13, c33=c33, c55=c55, bx=bx, bz=bz, src_x=src_x, src_z=src_z,
                rcv_x=rcv_x, rcv_z=rcv_z, wavelet=wav, mt=mt, bcx=bcx, bcz=bcz, weights=wts,
                vp=vp, vs=vs, rho=rho, eps=eps, delta=delta, abc=abc, damp=damp)
@@ -139,7 +139,7 @@ def elastic_case(ek, bc, free_surface, vti, seed, abc="PML"):
                                 abc, torch.tensor(bcx, dtype=torch.float32),
                                 torch.tensor(bcz, dtype=torch.float32),
                                 None if abc == "PML" else torch.tensor(damp, dtype=torch.float32),
-                                z, z, leaves["bx"], leaves["bz"], CC, fd_order=4, n_segments=seg)
+                                z, z, leaves["bx"], leaves["bz"], CC, fd_order=order, n_segments=seg)
         loss = sum((res[k] * torch.tensor(wts[i])).sum() for i, k in enumerate(names))
         loss.backward()
         for k in names:
@@ -177,8 +177,11 @@ def main():
         "elastic_vti_nofs": lambda: elastic_case(ek, bc, False, True, 22),
         "elastic_abl_gerjan_fs": lambda: elastic_case(ek, bc, True, False, 23, abc="gerjan"),
         "elastic_abl_sincos_nofs": lambda: elastic_case(ek, bc, False, True, 24, abc="sincos"),
+        "elastic_fd6_iso_fs": lambda: elastic_case(ek, bc, True, False, 25, order=6),
+        "elastic_fd6_vti_nofs": lambda: elastic_case(ek, bc, False, True, 26, order=6),
         "boundary": lambda: boundary_case(bc),
         "diffcoef": lambda: dict(fdc=ek.DiffCoef(2, "s").numpy()),
+        "diffcoef6": lambda: dict(fdc6=ek.DiffCoef(3, "s").numpy(), fdc8=ek.DiffCoef(4, "s").numpy()),
     }
     names = sys.argv[1:] or list(cases)   # write only the named cases when given
     for name in names:

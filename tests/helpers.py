@@ -69,7 +69,8 @@ def run_elastic_oracle(g, m, seg=1, want_grad=True):
                             torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
                             len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
                             torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z,
-                            leaves["bx"], leaves["bz"], elastic_cc(leaves, nz, nx), n_segments=seg)
+                            leaves["bx"], leaves["bz"], elastic_cc(leaves, nz, nx), n_segments=seg,
+                            fd_order=int(g["fd_order"]) if "fd_order" in g else 4)
     out = {k: res[k].detach().numpy() for k in ELASTIC_NAMES}
     out.update({"fw_" + k: res["forward_wavefield_" + k].numpy() for k in ELASTIC_NAMES})
     if want_grad:

@@ -22,7 +22,7 @@ def test_acoustic_oracle_matches_reference_vectors(name, seg):
 
 
 @pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs",
-                                  "elastic_abl_sincos_nofs"])
+                                  "elastic_abl_sincos_nofs", "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs"])
 @pytest.mark.parametrize("seg", [1, 3])
 def test_elastic_oracle_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
@@ -39,6 +39,11 @@ def test_fd_coefficients_closed_form():
     c = diff_coef(2).numpy()
     assert np.array_equal(c, golden("diffcoef")["fdc"])
     assert abs(float(c[0]) - 9 / 8) < 1e-7 and abs(float(c[1]) + 1 / 24) < 1e-7
+    # 6th order: 75/64, -25/384, 3/640 (staggered-grid Taylor coefficients); the reference
+    # solves the Vandermonde system in float32, so the closed form holds to a few 1e-6
+    c6 = diff_coef(3).numpy()
+    assert np.array_equal(c6, golden("diffcoef6")["fdc6"])
+    assert np.allclose(c6, [75 / 64, -25 / 384, 3 / 640], rtol=3e-5, atol=0)
 
 
 def test_oracle_boundary_profiles_match_reference_vectors():
