@@ -64,7 +64,8 @@ class ElasticDesc(ctypes.Structure):
                 ("shots_per_wave", _i32), ("tile_z", _i32), ("dt", _f32), ("dt_dx", _f32), ("dt_dz", _f32),
                 ("dx", _f32), ("dz", _f32), ("c0", _f32), ("c1", _f32), ("damping", _i32),
                 ("coef", _vp), ("pml_x", _vp), ("pml_z", _vp), ("src", _vp), ("src_m", _vp),
-                ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp), ("damp", _vp), ("src_fz", _vp)]
+                ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp), ("damp", _vp), ("src_fz", _vp),
+                ("nn", _i32), ("c2", _f32)]
 
 
 class ElasticFwdArgs(ctypes.Structure):

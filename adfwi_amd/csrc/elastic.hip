@@ -62,7 +62,8 @@ enum { C11 = 0, C13, C33, C55, BX, BZ, C15, C35, NCOEF };
 struct ElDev {
   int S, Z, X, pitch, fsurf, nt, nrec, spw;     // spw: shots per wavefront (runtime loop)
   int tz, nsx, nstrips;
-  float dt, dt_dx, dt_dz, dx, dz, c0, c1;
+  float dt, dt_dx, dt_dz, dx, dz, c0, c1, c2;
+  int nn;                                      // stencil half-order: 2, or 3..5 (three-term scheme)
   long long plane, field;
   const float *coef, *pml_x, *pml_z, *src, *src_m, *damp;
   const float *src_fz;   // [S] vertical-force weight of each shot's wavelet, or NULL
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
   }
 }
 
-// five records of step t from the state after step t (totals; free-surface tzz row 2 is zero)
+// five records of step t from the state after step t (totals; the free-surface tzz row NN is zero)
 __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restrict__ r_txx,
                           float *__restrict__ r_tzz, float *__restrict__ r_txz, float *__restrict__ r_vx,
                           float *__restrict__ r_vz, int t) {
@@ -293,7 +294,7 @@ __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restri
   const long long f = d.field;
   const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
   r_txx[ro] = st[TXX_X * f + o] + st[TXX_Z * f + o];
-  r_tzz[ro] = (d.fsurf && z == 2) ? 0.0f : st[TZZ_X * f + o] + st[TZZ_Z * f + o];
+  r_tzz[ro] = (d.fsurf && z == d.nn) ? 0.0f : st[TZZ_X * f + o] + st[TZZ_Z * f + o];
   r_txz[ro] = st[TXZ_X * f + o] + st[TXZ_Z * f + o];
   r_vx[ro] = st[VX_X * f + o] + st[VX_Z * f + o];
   r_vz[ro] = st[VZ_X * f + o] + st[VZ_Z * f + o];
@@ -644,7 +645,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
 }
 
 // adds the record adjoints of step t to both splits of each recorded field (free-surface tzz
-// on row 2 is identically zero there, so it receives nothing)
+// on row NN is identically zero there, so it receives nothing)
 __global__ void el_inject(ElDev d, float *__restrict__ lam, const float *__restrict__ g_txx,
                           const float *__restrict__ g_tzz, const float *__restrict__ g_txz,
                           const float *__restrict__ g_vx, const float *__restrict__ g_vz, int t) {
@@ -657,7 +658,7 @@ __global__ void el_inject(ElDev d, float *__restrict__ lam, const float *__restr
   const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
   const float *gs[5] = {g_txx, g_tzz, g_txz, g_vx, g_vz};
   for (int q = 0; q < 5; ++q) {
-    if (!gs[q] || (q == 1 && d.fsurf && z == 2)) continue;
+    if (!gs[q] || (q == 1 && d.fsurf && z == d.nn)) continue;
     const float v = gs[q][ro];
     atomicAdd(lam + (2 * q) * f + o, v);
     atomicAdd(lam + (2 * q + 1) * f + o, v);
@@ -1159,6 +1160,525 @@ __global__ __launch_bounds__(ENT) void el_abl_adj(ElDev d, const float *__restri
 }
 
 // ---------------------------------------------------------------------------
+// three-term (radius-3) PML scheme: fd_order 6, 8, 10
+// ---------------------------------------------------------------------------
+// Reference: step_forward_PML_6order (ADFWI/propagator/elastic_kernels.py:426-584) with the
+// operators Dxfm_6 .. Dzbm_6 (:85-108).  With NN = fd_order / 2 the step is the 4th-order one
+// with three-term differences c0 (a[i+1]-a[i]) + c1 (a[i+2]-a[i-1]) + c2 (a[i+3]-a[i-2]) (the
+// first three coefficients of DiffCoef(NN)), on the window rows [NN, Z-NN-1] x cols
+// [NN, X-NN-1], and the free-surface rows at h = NN+1: stress tzz[NN] = 0, tzz[NN-1] = -tzz[NN+1],
+// txz[NN-1] = -txz[NN], txz[NN-2] = -txz[NN+1]; velocity (cols J = [NN, X-NN-1])
+// vz[NN-1] = vz[NN], vx[NN-1] = vz[NN-1,j+1] - vz[NN-1,j] + vz[NN,j+1] - vz[NN,j] + vx[NN+1,j],
+// vz[NN-2] = vz[NN-1].
+//
+// Lanes: a wavefront owns columns x0-6 .. x0+57 and writes x0 .. x0+50: stress is valid on
+// lanes 3..59 (velocities 3 columns away, one more for the free-surface vx row), velocity on
+// lanes 6..56.  Rows: at iteration k the velocity totals of row k+3 enter a 7-row window, the
+// stress of row k is updated and the velocity of row k-3.  Strips are at least 8 rows, so that
+// only the first strip reaches the free-surface rows.
+constexpr int EHL3 = 6;
+constexpr int ESX3 = 51;
+
+__device__ __forceinline__ bool strip_of3(const ElDev &d, int &z0, int &x0) {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int xcd = b & 7, per = n >> 3, rem = n & 7;
+  const int strip = xcd * per + min(xcd, rem) + (b >> 3);
+  if (strip >= d.nstrips) return false;
+  const int sz = strip / d.nsx;
+  z0 = d.nn + sz * d.tz;
+  x0 = d.nn + (strip - sz * d.nsx) * ESX3;
+  return true;
+}
+
+template <int N>
+__device__ __forceinline__ void push(float (&w)[N], float v) {
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i) w[i] = w[i + 1];
+  w[N - 1] = v;
+}
+
+struct Fd3 {
+  float c0, c1, c2;
+  // x differences of a row value (every lane must take part in the shuffles)
+  __device__ __forceinline__ float dxf(float v, int l) const {
+    return c0 * (shfl(v, l + 1) - v) + c1 * (shfl(v, l + 2) - shfl(v, l - 1)) + c2 * (shfl(v, l + 3) - shfl(v, l - 2));
+  }
+  __device__ __forceinline__ float dxb(float v, int l) const {
+    return c0 * (v - shfl(v, l - 1)) + c1 * (shfl(v, l + 1) - shfl(v, l - 2)) + c2 * (shfl(v, l + 2) - shfl(v, l - 3));
+  }
+  // z differences about window entry i
+  template <int N>
+  __device__ __forceinline__ float dzf(const float (&w)[N], int i) const {
+    return c0 * (w[i + 1] - w[i]) + c1 * (w[i + 2] - w[i - 1]) + c2 * (w[i + 3] - w[i - 2]);
+  }
+  template <int N>
+  __device__ __forceinline__ float dzb(const float (&w)[N], int i) const {
+    return c0 * (w[i] - w[i - 1]) + c1 * (w[i + 1] - w[i - 2]) + c2 * (w[i + 2] - w[i - 3]);
+  }
+  // transposes
+  __device__ __forceinline__ float dxfT(float e, int l) const {
+    return c0 * (shfl(e, l - 1) - e) + c1 * (shfl(e, l - 2) - shfl(e, l + 1)) + c2 * (shfl(e, l - 3) - shfl(e, l + 2));
+  }
+  __device__ __forceinline__ float dxbT(float e, int l) const {
+    return c0 * (e - shfl(e, l + 1)) + c1 * (shfl(e, l - 1) - shfl(e, l + 2)) + c2 * (shfl(e, l - 2) - shfl(e, l + 3));
+  }
+  template <int N>
+  __device__ __forceinline__ float dzbT(const float (&w)[N], int i) const {
+    return c0 * (w[i] - w[i + 1]) + c1 * (w[i - 1] - w[i + 2]) + c2 * (w[i - 2] - w[i + 3]);
+  }
+  template <int N>
+  __device__ __forceinline__ float dzfT(const float (&w)[N], int i) const {
+    return c0 * (w[i - 1] - w[i]) + c1 * (w[i - 2] - w[i + 1]) + c2 * (w[i - 3] - w[i + 2]);
+  }
+};
+
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__restrict__ in, float *__restrict__ out,
+                                                    int t) {
+  int z0, x0;
+  if (!strip_of3(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nn = d.nn;
+  const int c = x0 - EHL3 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const int zw1 = d.Z - nn - 1, xw1 = d.X - nn - 1;   // last window row / column
+  const bool out_col = lane >= EHL3 && lane < EHL3 + ESX3 && c <= xw1;   // c >= nn by the strip origin
+  const bool cwin = c >= nn && c <= xw1;
+  const bool jj1 = c + 1 >= nn && c + 1 <= xw1;
+  const int zmax = d.Z - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rIn = rsrc(in, (unsigned)NFIELD * fieldb);
+  const rsrc_t rOut = rsrc(out, (unsigned)NFIELD * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const Fd3 D{d.c0, d.c1, d.c2};
+  const int cc = min(max(c, 0), d.X - 1);
+  const float pxn = d.pml_x[cc], pxd = d.pml_x[d.pitch + cc], pxi = d.pml_x[2 * d.pitch + cc];
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const int nk = end_row - z0 + 6;
+
+  const int s_first = (blockIdx.y * EWPB + wv) * d.spw;
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = s_first + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float wsrc = d.src[(long long)s * d.nt + t];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    // velocity totals rows k-3..k+3; stress totals txz/tzz rows k-6..k, txx rows k-3..k
+    float vx[7] = {}, vz[7] = {}, xz[7] = {}, zz[7] = {}, xx[4] = {};
+    for (int rr = z0 - 6; rr <= z0 - 1; ++rr) {
+      const unsigned o = rb(rr);
+      const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
+      const float b = bld(rIn, vc, F(VZ_X) + o) + bld(rIn, vc, F(VZ_Z) + o);
+      const bool ok = rr >= 0;
+      push(vx, ok ? a : 0.f);
+      push(vz, ok ? b : 0.f);
+    }
+    // loaded one iteration ahead: velocity splits of row k+3, stress splits and moduli of
+    // row k, old velocity splits and buoyancies of row k-3
+    struct Row {
+      float v[4], s[6], cm[ANISO ? 6 : 4], u[4], b[2];
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o3 = rb(k + 3), o0 = rb(k), om = rb(k - 3);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.v[q] = bld(rIn, vc, F(VX_X + q) + o3);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.s[q] = bld(rIn, vc, F(TXX_X + q) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+      if (k - 3 >= z0 && k - 3 < end_row) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R.u[q] = bld(rIn, vc, F(VX_X + q) + om);
+        R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
+      }
+    };
+    Row cur;
+    load(z0 - 3, cur);
+#pragma unroll 1
+    for (int kk = 0; kk < nk; ++kk) {
+      const int k = z0 - 3 + kk;
+      Row nxt;
+      load(k + 1, nxt);
+      {
+        const bool ok = k + 3 <= zmax;
+        push(vx, ok ? cur.v[0] + cur.v[1] : 0.f);
+        push(vz, ok ? cur.v[2] + cur.v[3] : 0.f);
+      }
+      // free surface of the previous step's velocities: window entries 1..4 = rows NN-2..NN+1
+      if (d.fsurf && k == nn) {
+        const float vzn = shfl(vz[3], lane + 1);            // vz row NN, column j+1
+        const float vz2m_n = jj1 ? vzn : 0.0f;              // new vz row NN-1 at column j+1
+        const float vz2m = cwin ? vz[3] : vz[2];
+        if (cwin) {
+          vx[2] = (((vz2m_n - vz2m) + vzn) - vz[3]) + vx[4];
+          vz[1] = vz2m;
+        }
+        vz[2] = vz2m;
+      }
+      // stress of row k (velocity row k = entry 3)
+      float sxx_x = cur.s[0], sxx_z = cur.s[1], szz_x = cur.s[2], szz_z = cur.s[3];
+      float sxz_x = cur.s[4], sxz_z = cur.s[5];
+      const float dxb_vx = D.dxb(vx[3], lane), dxf_vz = D.dxf(vz[3], lane);
+      float dxb_vz = 0.f, dxf_vx = 0.f;
+      if constexpr (ANISO) {
+        dxb_vz = D.dxb(vz[3], lane);
+        dxf_vx = D.dxf(vx[3], lane);
+      }
+      if (k >= nn && k <= zw1 && cwin) {
+        const float dzb_vz = D.dzb(vz, 3), dzf_vx = D.dzf(vx, 3);
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
+        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        float sx_xx, sz_xx, sx_zz, sz_zz, sx_xz, sz_xz;
+        if constexpr (ANISO) {
+          const float dzb_vx = D.dzb(vx, 3), dzf_vz = D.dzf(vz, 3);
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
+          sx_xx = C11v * dxb_vx + C15v * dxb_vz;
+          sz_xx = C15v * dzb_vx + C13v * dzb_vz;
+          sx_zz = C13v * dxb_vx + C35v * dxb_vz;
+          sz_zz = C35v * dzb_vx + C33v * dzb_vz;
+          sx_xz = C15v * dxf_vx + C55v * dxf_vz;
+          sz_xz = C55v * dzf_vx + C35v * dzf_vz;
+        } else {
+          sx_xx = C11v * dxb_vx;
+          sz_xx = C13v * dzb_vz;
+          sx_zz = C13v * dxb_vx;
+          sz_zz = C33v * dzb_vz;
+          sx_xz = C55v * dxf_vz;
+          sz_xz = C55v * dzf_vx;
+        }
+        sxx_x = (pxn * sxx_x + d.dt_dx * sx_xx) * pxi;
+        sxx_z = (pzn * sxx_z + d.dt_dz * sz_xx) * pzi;
+        szz_x = (pxn * szz_x + d.dt_dx * sx_zz) * pxi;
+        szz_z = (pzn * szz_z + d.dt_dz * sz_zz) * pzi;
+        sxz_x = (pxn * sxz_x + d.dt_dx * sx_xz) * pxi;
+        sxz_z = (pzn * sxz_z + d.dt_dz * sz_xz) * pzi;
+      }
+      if (k == zs && c == xs) {
+        sxx_x = sxx_x + mxx * wsrc; sxx_z = sxx_z + mxx * wsrc;
+        szz_x = szz_x + mzz * wsrc; szz_z = szz_z + mzz * wsrc;
+        sxz_x = sxz_x + mxz * wsrc; sxz_z = sxz_z + mxz * wsrc;
+      }
+      if (out_col && k >= z0 && k < end_row) {
+        const unsigned o = rb(k);
+        bst(sxx_x, rOut, vs, F(TXX_X) + o); bst(sxx_z, rOut, vs, F(TXX_Z) + o);
+        bst(szz_x, rOut, vs, F(TZZ_X) + o); bst(szz_z, rOut, vs, F(TZZ_Z) + o);
+        bst(sxz_x, rOut, vs, F(TXZ_X) + o); bst(sxz_z, rOut, vs, F(TXZ_Z) + o);
+      }
+      push(xx, sxx_x + sxx_z);
+      push(xz, sxz_x + sxz_z);
+      push(zz, szz_x + szz_z);
+      // stress free surface once row NN+1 exists (entries 3..6 = rows NN-2..NN+1)
+      if (d.fsurf && k == nn + 1) {
+        zz[5] = 0.0f;       // tzz[NN] = 0
+        zz[4] = -zz[6];     // tzz[NN-1] = -tzz[NN+1]
+        xz[4] = -xz[5];     // txz[NN-1] = -txz[NN]
+        xz[3] = -xz[6];     // txz[NN-2] = -txz[NN+1]
+      }
+      // velocity of row r = k-3 (txz/tzz rows r-3..r+3 = entries 0..6, txx row r = xx[0])
+      const int r = k - 3;
+      const float dxf_txx = D.dxf(xx[0], lane), dxb_txz = D.dxb(xz[3], lane);
+      if (out_col && r >= z0 && r < end_row) {
+        const unsigned o = rb(r);
+        const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
+        const float bxv = cur.b[0], bzv = cur.b[1];
+        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float nx_x = (pxn * cur.u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * cur.u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
+          const float hf = 0.5f * (d.src_fz[s] * wsrc);
+          nz_x = nz_x + hf;
+          nz_z = nz_z + hf;
+        }
+        bst(nx_x, rOut, vs, F(VX_X) + o); bst(nx_z, rOut, vs, F(VX_Z) + o);
+        bst(nz_x, rOut, vs, F(VZ_X) + o); bst(nz_z, rOut, vs, F(VZ_Z) + o);
+      }
+      cur = nxt;
+    }
+  }
+}
+
+// Adjoint of el3_fwd_step: the transposes listed for el_adj_step with the three-term
+// operators and the free-surface rows moved to NN-2 .. NN+1:
+//   stress free surface^T: Lt_zz[NN] = 0, Lt_zz[NN+1] -= Lt_zz[NN-1], Lt_xz[NN] -= Lt_xz[NN-1],
+//   Lt_xz[NN+1] -= Lt_xz[NN-2];
+//   velocity free surface^T (cols J): Lv_z[NN,j] += Lv_z[NN-1,j] + Lv_z[NN-2,j] - 2 Lv_x[NN-1,j]
+//   + 2 Lv_x[NN-1,j-1], Lv_x[NN+1,j] += Lv_x[NN-1,j].
+// Iteration k: forward velocity totals and e of row k+3 enter their windows, the stress
+// adjoints of row k are formed, then the velocity adjoints of row k-3.  Strips of ETZA rows.
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
+                                                    const float *__restrict__ fv, const float *__restrict__ fsn, int t,
+                                                    float *__restrict__ gacc, float *__restrict__ gsrc) {
+  constexpr int NGF = ANISO ? 8 : 6;
+  __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  int z0, x0;
+  if (!strip_of3(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nn = d.nn;
+  const int c = x0 - EHL3 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const int zw1 = d.Z - nn - 1, xw1 = d.X - nn - 1;
+  const bool out_col = lane >= EHL3 && lane < EHL3 + ESX3 && c <= xw1;
+  const bool cwin = c >= nn && c <= xw1;
+  const bool jj1 = c + 1 >= nn && c + 1 <= xw1;
+  const bool jm1 = c - 1 >= nn && c - 1 <= xw1;
+  const int zmax = d.Z - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rLi = rsrc(lin, (unsigned)NFIELD * fieldb);
+  const rsrc_t rLo = rsrc(lout, (unsigned)NFIELD * fieldb);
+  const rsrc_t rFv = rsrc(fv, (unsigned)NFIELD * fieldb);
+  const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const Fd3 D{d.c0, d.c1, d.c2};
+  const int cc = min(max(c, 0), d.X - 1);
+  const float pxn = d.pml_x[cc], pxd = d.pml_x[d.pitch + cc], pxi = d.pml_x[2 * d.pitch + cc];
+  const float sx_keep = pxn * pxi, sx_a = d.dt_dx * pxi;
+  const float vx_keep = pxn / pxd, vx_w = d.dt / (d.dx * pxd);
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const int nk = end_row - z0 + 6;
+  const int grp = blockIdx.y * EWPB + wv;
+  float *sl = slab + wv * (NGF * ETZA * WAVE) + lane;
+  for (int q = 0; q < NGF * ETZA; ++q) sl[q * WAVE] = 0.0f;
+  auto acc = [&](int f, int row, float v) { sl[(f * ETZA + (row - z0)) * WAVE] += v; };
+
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = grp * d.spw + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float gmt = 0.0f;   // wavelet gradient of the moment-tensor injection, when a force term follows
+    // forward velocity totals rows k-3..k+3; forward stress totals txz/tzz rows k-6..k, txx k-3..k
+    float vx[7] = {}, vz[7] = {}, xz[7] = {}, zz[7] = {}, xx[4] = {};
+    // e windows: e1 (vx_x), e3 (vz_x) rows k..k+3; e2 (vx_z), e4 (vz_z) rows k-3..k+3
+    float e1[4] = {}, e3[4] = {}, e2[7] = {}, e4[7] = {};
+    // stress-product windows: P1, Q3 (, P3, Q1) rows k-3..k; P4, Q2 (, P2, Q4) rows k-6..k
+    float p1[4] = {}, q3[4] = {}, p4[7] = {}, q2[7] = {};
+    float p3[ANISO ? 4 : 1] = {}, q1[ANISO ? 4 : 1] = {}, p2[ANISO ? 7 : 1] = {}, q4[ANISO ? 7 : 1] = {};
+    // free-surface rows kept for the transposes
+    float ltzz1 = 0.f, ltxz0 = 0.f, ltxz1 = 0.f, lvz0 = 0.f, lvz1 = 0.f, lvx1 = 0.f;
+
+    auto push_row = [&](int rr, float fvx, float fvz, float lvxx, float lvxz, float lvzx, float lvzz, float bxv,
+                        float bzv) {
+      const bool in_grid = rr >= 0 && rr <= zmax;
+      push(vx, in_grid ? fvx : 0.f);
+      push(vz, in_grid ? fvz : 0.f);
+      float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
+      if (rr >= nn && rr <= zw1 && cwin) {
+        const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
+        ea = lvxx * (vx_w * bxv);
+        eb = lvxz * (vz_w * bxv);
+        ec = lvzx * (vx_w * bzv);
+        ed = lvzz * (vz_w * bzv);
+      }
+      push(e1, ea);
+      push(e2, eb);
+      push(e3, ec);
+      push(e4, ed);
+    };
+    for (int rr = z0 - 6; rr <= z0 - 1; ++rr) {
+      const unsigned o = rb(rr);
+      push_row(rr, bld(rFv, vc, F(VX_X) + o) + bld(rFv, vc, F(VX_Z) + o),
+               bld(rFv, vc, F(VZ_X) + o) + bld(rFv, vc, F(VZ_Z) + o), bld(rLi, vc, F(VX_X) + o),
+               bld(rLi, vc, F(VX_Z) + o), bld(rLi, vc, F(VZ_X) + o), bld(rLi, vc, F(VZ_Z) + o),
+               bld(rC, vc, BX * planeb + o), bld(rC, vc, BZ * planeb + o));
+    }
+    // loaded one iteration ahead: row k+3 of the forward velocity splits, the velocity
+    // adjoints and the buoyancies; row k of the stress adjoints, the moduli and the forward
+    // stress splits; row k-3 of the velocity adjoints
+    struct Row {
+      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4], fs[6], lr[4];
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o3 = rb(k + 3), o0 = rb(k), om = rb(k - 3);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.fv[q] = bld(rFv, vc, F(VX_X + q) + o3);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) R.lv[q] = bld(rLi, vc, F(VX_X + q) + o3);
+      R.b[0] = bld(rC, vc, BX * planeb + o3); R.b[1] = bld(rC, vc, BZ * planeb + o3);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.ls[q] = bld(rLi, vc, F(TXX_X + q) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) R.fs[q] = bld(rFs, vc, F(TXX_X + q) + o0);
+      if (k - 3 >= z0 && k - 3 < end_row) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R.lr[q] = bld(rLi, vc, F(VX_X + q) + om);
+      }
+    };
+    Row cur;
+    load(z0 - 3, cur);
+#pragma unroll 1
+    for (int kk = 0; kk < nk; ++kk) {
+      const int k = z0 - 3 + kk;
+      Row nxt;
+      load(k + 1, nxt);
+      push_row(k + 3, cur.fv[0] + cur.fv[1], cur.fv[2] + cur.fv[3], cur.lv[0], cur.lv[1], cur.lv[2], cur.lv[3],
+               cur.b[0], cur.b[1]);
+      if (d.fsurf && k == nn) {   // forward velocity free surface (entries 1..4 = rows NN-2..NN+1)
+        const float vzn = shfl(vz[3], lane + 1);
+        const float vz2m_n = jj1 ? vzn : 0.0f;
+        const float vz2m = cwin ? vz[3] : vz[2];
+        if (cwin) {
+          vx[2] = (((vz2m_n - vz2m) + vzn) - vz[3]) + vx[4];
+          vz[1] = vz2m;
+        }
+        vz[2] = vz2m;
+      }
+      // a. stress-total adjoints of row k (e1/e3 row k = entry 0; e2/e4 row k = entry 3)
+      float lt_xx = D.dxfT(e1[0], lane);
+      float lt_xz = D.dzbT(e2, 3) + D.dxbT(e3[0], lane);
+      float lt_zz = D.dzfT(e4, 3);
+      // b. stress free surface transposed
+      if (d.fsurf) {
+        if (k == nn - 2) ltxz0 = lt_xz;
+        if (k == nn - 1) { ltzz1 = lt_zz; ltxz1 = lt_xz; }
+        if (k == nn) { lt_zz = 0.0f; lt_xz = lt_xz - ltxz1; }
+        if (k == nn + 1) { lt_zz = lt_zz - ltzz1; lt_xz = lt_xz - ltxz0; }
+      }
+      // forward velocity derivatives of row k (for the C gradients)
+      const float dxb_vx = D.dxb(vx[3], lane), dxf_vz = D.dxf(vz[3], lane);
+      float dxb_vz = 0.f, dxf_vx = 0.f;
+      if constexpr (ANISO) {
+        dxb_vz = D.dxb(vz[3], lane);
+        dxf_vx = D.dxf(vx[3], lane);
+      }
+      // c, d. stress splits of row k
+      float P1 = 0.f, Q2 = 0.f, Q3 = 0.f, P4 = 0.f, P2 = 0.f, P3 = 0.f, Q1 = 0.f, Q4 = 0.f;
+      if (k >= nn && k <= zw1 && cwin) {
+        const unsigned o = rb(k);
+        const float lxx_x = cur.ls[0] + lt_xx, lxx_z = cur.ls[1] + lt_xx;
+        const float lzz_x = cur.ls[2] + lt_zz, lzz_z = cur.ls[3] + lt_zz;
+        const float lxz_x = cur.ls[4] + lt_xz, lxz_z = cur.ls[5] + lt_xz;
+        const bool own_k = out_col && k >= z0 && k < end_row;
+        if (own_k && k == zs && c == xs && gsrc) {
+          const float gm = (mxx * (lxx_x + lxx_z) + mzz * (lzz_x + lzz_z)) + mxz * (lxz_x + lxz_z);
+          if (d.src_fz) gmt = gm;        // the force term of row zs follows three rows later
+          else gsrc[(long long)s * d.nt + t] = gm;
+        }
+        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float sz_keep = pzn * pzi, sz_a = d.dt_dz * pzi;
+        if (own_k) {
+          bst(sx_keep * lxx_x, rLo, vs, F(TXX_X) + o); bst(sz_keep * lxx_z, rLo, vs, F(TXX_Z) + o);
+          bst(sx_keep * lzz_x, rLo, vs, F(TZZ_X) + o); bst(sz_keep * lzz_z, rLo, vs, F(TZZ_Z) + o);
+          bst(sx_keep * lxz_x, rLo, vs, F(TXZ_X) + o); bst(sz_keep * lxz_z, rLo, vs, F(TXZ_Z) + o);
+        }
+        const float axx_x = sx_a * lxx_x, axx_z = sz_a * lxx_z;
+        const float azz_x = sx_a * lzz_x, azz_z = sz_a * lzz_z;
+        const float axz_x = sx_a * lxz_x, axz_z = sz_a * lxz_z;
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
+        P1 = C11v * axx_x + C13v * azz_x;
+        Q2 = C13v * axx_z + C33v * azz_z;
+        Q3 = C55v * axz_x;
+        P4 = C55v * axz_z;
+        if constexpr (ANISO) {
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
+          P2 = C15v * axx_z + C35v * azz_z;
+          P3 = C15v * axz_x;
+          Q1 = C15v * axx_x + C35v * azz_x;
+          Q4 = C35v * axz_z;
+        }
+        if (own_k) {
+          const float dzb_vz = D.dzb(vz, 3), dzf_vx = D.dzf(vx, 3);
+          if constexpr (ANISO) {
+            const float dzb_vx = D.dzb(vx, 3), dzf_vz = D.dzf(vz, 3);
+            acc(6, k, axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx);
+            acc(7, k, azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz);
+          }
+          acc(0, k, axx_x * dxb_vx);
+          acc(1, k, axx_z * dzb_vz + azz_x * dxb_vx);
+          acc(2, k, azz_z * dzb_vz);
+          acc(3, k, axz_x * dxf_vz + axz_z * dzf_vx);
+        }
+      }
+      push(p1, P1);
+      push(q3, Q3);
+      push(p4, P4);
+      push(q2, Q2);
+      if constexpr (ANISO) {
+        push(p3, P3);
+        push(q1, Q1);
+        push(p2, P2);
+        push(q4, Q4);
+      }
+      // forward stress totals of row k (after the step), free surface once row NN+1 exists
+      {
+        const bool in_grid = k >= 0 && k <= zmax;
+        push(xx, in_grid ? cur.fs[0] + cur.fs[1] : 0.f);
+        push(zz, in_grid ? cur.fs[2] + cur.fs[3] : 0.f);
+        push(xz, in_grid ? cur.fs[4] + cur.fs[5] : 0.f);
+      }
+      if (d.fsurf && k == nn + 1) {
+        zz[5] = 0.0f; zz[4] = -zz[6]; xz[4] = -xz[5]; xz[3] = -xz[6];
+      }
+      // e. velocity-total adjoints of row r = k-3 (P1/Q3 row r = entry 0; P4/Q2 row r = entry 3)
+      const int r = k - 3;
+      float lvx = D.dxbT(p1[0], lane) + D.dzfT(p4, 3);
+      float lvz = D.dzbT(q2, 3) + D.dxfT(q3[0], lane);
+      if constexpr (ANISO) {
+        lvx = lvx + (D.dzbT(p2, 3) + D.dxfT(p3[0], lane));
+        lvz = lvz + (D.dxbT(q1[0], lane) + D.dzfT(q4, 3));
+      }
+      if (d.fsurf) {
+        if (r == nn - 2) lvz0 = lvz;
+        if (r == nn - 1) { lvz1 = lvz; lvx1 = lvx; }
+        if (r == nn) {
+          const float lvx1m = shfl(lvx1, lane - 1);
+          if (cwin) lvz = lvz + (((lvz1 + lvz0) - 2.0f * lvx1) + (jm1 ? 2.0f * lvx1m : 0.0f));
+        }
+        if (r == nn + 1 && cwin) lvx = lvx + lvx1;
+      }
+      // forward stress derivatives of row r (for the b gradients)
+      const float dxf_txx = D.dxf(xx[0], lane), dxb_txz = D.dxb(xz[3], lane);
+      if (out_col && r >= z0 && r < end_row) {
+        const unsigned o = rb(r);
+        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
+        const float l1 = cur.lr[0], l2 = cur.lr[1], l3 = cur.lr[2], l4 = cur.lr[3];
+        if (d.src_fz && gsrc && r == zs && c == xs)
+          gsrc[(long long)s * d.nt + t] = gmt + 0.5f * (d.src_fz[s] * (l3 + l4));
+        bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);
+        bst(vx_keep * l3 + lvz, rLo, vs, F(VZ_X) + o); bst(vz_keep * l4 + lvz, rLo, vs, F(VZ_Z) + o);
+        const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
+        acc(4, r, l1 * (vx_w * dxf_txx) + l2 * (vz_w * dzb_txz));
+        acc(5, r, l3 * (vx_w * dxb_txz) + l4 * (vz_w * dzf_tzz));
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  if (!out_col) return;
+  const long long planef = (long long)d.Z * d.pitch;
+  float *g = gacc + (long long)blockIdx.y * NCOEF * planef + c;
+  const float *sb = slab + lane;
+  for (int f = wv; f < NGF; f += EWPB)
+    for (int row = z0; row < end_row; ++row) {
+      const int o = (f * ETZA + (row - z0)) * WAVE;
+      const float v = (sb[o] + sb[NGF * ETZA * WAVE + o]) + (sb[2 * NGF * ETZA * WAVE + o] + sb[3 * NGF * ETZA * WAVE + o]);
+      float *p = g + (long long)f * planef + (long long)row * d.pitch;
+      *p = *p + v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
@@ -1169,22 +1689,29 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
     return ADFWI_E_VERSION;
   }
   ADFWI_CHECK_ARG(d->nshots >= 1, "nshots must be >= 1 (got %d)", d->nshots);
-  ADFWI_CHECK_ARG(d->nz >= 9 && d->nx >= 9, "padded grid %d x %d too small (need >= 9 x 9)", d->nz, d->nx);
+  const int nn = d->nn == 0 ? 2 : d->nn;
+  ADFWI_CHECK_ARG(nn >= 2 && nn <= 5, "stencil half-order nn %d outside 2..5 (fd_order 4..10)", d->nn);
+  ADFWI_CHECK_ARG(nn == 2 || !d->damping, "the damping scheme is implemented at 4th order only (nn = 2, got %d)", nn);
+  ADFWI_CHECK_ARG(nn == 2 || d->tile_z == 0 || d->tile_z >= 8,
+                  "tile_z %d: the three-term scheme needs strips of at least 8 rows", d->tile_z);
+  ADFWI_CHECK_ARG(d->nz >= 2 * nn + 5 && d->nx >= 2 * nn + 5, "padded grid %d x %d too small (need >= %d x %d)",
+                  d->nz, d->nx, 2 * nn + 5, 2 * nn + 5);
   ADFWI_CHECK_ARG(d->pitch >= d->nx, "pitch %d < nx %d", d->pitch, d->nx);
   ADFWI_CHECK_ARG(d->nt >= 1 && d->nrec >= 0, "nt %d / nrec %d invalid", d->nt, d->nrec);
   ADFWI_CHECK_ARG(d->shots_per_wave >= 0 && (d->tile_z == 0 || (d->tile_z >= 4 && d->tile_z <= ETZ_MAX)),
                   "shots_per_wave %d / tile_z %d invalid (tile_z: 0 or 4..%d)", d->shots_per_wave, d->tile_z,
                   ETZ_MAX);
-  const int rows = d->nz - 4;   // window rows [2, Z-3]
-  L->zbase = 2;
-  L->tile_x = ESX;
-  L->ntiles_x = (d->nx - 4 + ESX - 1) / ESX;
+  const int rows = d->nz - 2 * nn;   // window rows [nn, Z-nn-1]
+  L->zbase = nn;
+  L->tile_x = nn == 2 ? ESX : ESX3;
+  L->ntiles_x = (d->nx - 2 * nn + L->tile_x - 1) / L->tile_x;
   // strips of 16 rows (shorter for small grids); shots per wavefront so that there are
   // about 4096 wavefronts, as few shot groups as that allows (fewer gradient planes)
   int tz = d->tile_z;
   if (tz == 0) {
-    tz = 4;
-    for (int cand = 16; cand >= 4; cand >>= 1)
+    const int tz_min = nn == 2 ? 4 : 8;
+    tz = tz_min;
+    for (int cand = 16; cand >= tz_min; cand >>= 1)
       if ((long long)L->ntiles_x * ((rows + cand - 1) / cand) * d->nshots >= 4096) { tz = cand; break; }
   }
   L->tile_z = tz;
@@ -1218,6 +1745,8 @@ ElDev el_dev(const adfwi_elastic_desc *d, const adfwi_layout &L) {
   v.nt = d->nt; v.nrec = d->nrec; v.spw = L.reserved; v.tz = L.tile_z; v.nsx = L.ntiles_x;
   v.nstrips = L.ntiles;
   v.dt = d->dt; v.dt_dx = d->dt_dx; v.dt_dz = d->dt_dz; v.dx = d->dx; v.dz = d->dz; v.c0 = d->c0; v.c1 = d->c1;
+  v.nn = d->nn == 0 ? 2 : d->nn;
+  v.c2 = v.nn > 2 ? d->c2 : 0.0f;
   v.plane = (long long)d->nz * d->pitch;
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.pml_x = d->pml_x; v.pml_z = d->pml_z; v.src = d->src; v.src_m = d->src_m;
@@ -1277,6 +1806,12 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
       else
         hipLaunchKernelGGL(el_abl_fwd<false>, grid, block, 0, st, v, cur, nxt, t);
       if ((rc = adfwi::check_launch("el_abl_fwd"))) return rc;
+    } else if (v.nn > 2) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el3_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
+      else
+        hipLaunchKernelGGL(el3_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
+      if ((rc = adfwi::check_launch("el3_fwd_step"))) return rc;
     } else {
       if (d->aniso)
         hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
@@ -1320,8 +1855,10 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   // transposes need rows 0..3 inside the first strip)
   // (damping scheme: strips of ETZA rows, so that no strip but the first reaches the
   // free-surface rows, whose velocity adjoints are state there)
-  v.tz = d->damping ? ETZA : min(max(L.tile_z, 4), ETZA);
-  v.nstrips = ((d->nz - 4 + v.tz - 1) / v.tz) * v.nsx;
+  // (three-term scheme: strips of ETZA rows, so that only the first strip reaches the
+  // free-surface rows NN-2 .. NN+1)
+  v.tz = (d->damping || v.nn > 2) ? ETZA : min(max(L.tile_z, 4), ETZA);
+  v.nstrips = ((d->nz - 2 * v.nn + v.tz - 1) / v.tz) * v.nsx;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid(v.nstrips, L.ngroups), block(ENT);
   const long long ng = (long long)d->nshots * d->nrec;
@@ -1348,6 +1885,12 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
       else
         hipLaunchKernelGGL(el_abl_adj<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
       if ((rc = adfwi::check_launch("el_abl_adj"))) return rc;
+    } else if (v.nn > 2) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el3_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      else
+        hipLaunchKernelGGL(el3_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      if ((rc = adfwi::check_launch("el3_adj_step"))) return rc;
     } else {
       if (d->aniso)
         hipLaunchKernelGGL(el_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);

@@ -60,7 +60,7 @@ class ElasticGeometry:
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, src_z, src_x, src_m: torch.Tensor,
                  rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0,
-                 damping: bool = False, src_fz: Optional[torch.Tensor] = None):
+                 damping: bool = False, src_fz: Optional[torch.Tensor] = None, nn: int = 2, c2: float = 0.0):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
@@ -68,6 +68,9 @@ class ElasticGeometry:
         self.aniso = bool(aniso)
         self.damping = bool(damping)          # multiplicative damping scheme (5 fields) instead of PML
         self.nfield = 5 if self.damping else NFIELD
+        # stencil half-order fd_order // 2: 2 for the 4th-order schemes, 3..5 for the three-term
+        # PML scheme (window [nn, Z-nn-1], free-surface rows nn-2 .. nn+1), c2 its third coefficient
+        self.nn, self.c2 = int(nn), float(c2)
         self.pitch = -(-self.X // PITCH_ALIGN) * PITCH_ALIGN
         src_z = np.asarray(src_z, dtype=np.int64).reshape(-1)
         src_x = np.asarray(src_x, dtype=np.int64).reshape(-1)
@@ -76,10 +79,11 @@ class ElasticGeometry:
         self.S, self.R = len(src_z), len(rcv_z)
         if self.S < 1:
             raise ValueError("at least one shot is required")
+        w = self.nn
         for name, zz, xx in (("source", src_z, src_x), ("receiver", rcv_z, rcv_x)):
-            if len(zz) and (zz.min() < 2 or zz.max() > self.Z - 3 or xx.min() < 2 or xx.max() > self.X - 3):
-                raise ValueError(f"{name} index outside the update window [2, {self.Z - 3}] x [2, {self.X - 3}] "
-                                 f"of the padded grid")
+            if len(zz) and (zz.min() < w or zz.max() > self.Z - w - 1 or xx.min() < w or xx.max() > self.X - w - 1):
+                raise ValueError(f"{name} index outside the update window [{w}, {self.Z - w - 1}] x "
+                                 f"[{w}, {self.X - w - 1}] of the padded grid")
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
@@ -106,6 +110,7 @@ class ElasticGeometry:
         d.shots_per_wave, d.tile_z = self.shots_per_wave, self.tile_z
         d.dt, d.dt_dx, d.dt_dz, d.dx, d.dz, d.c0, d.c1 = scalars
         d.damping = int(self.damping)
+        d.nn, d.c2 = self.nn, self.c2
         ptr = lambda t: None if t is None else t.data_ptr()
         d.coef, d.pml_x, d.pml_z, d.src = ptr(coef), ptr(pml_x), ptr(pml_z), ptr(src)
         d.damp = ptr(damp)
@@ -149,15 +154,15 @@ def pml_factors(bcx_p: torch.Tensor, bcz_p: torch.Tensor, dt: float, pitch: int)
     return px, pz
 
 
-def _energy(state: torch.Tensor, X: int, free_surface: bool):
-    """sum over shots of each total squared (elastic_kernels.py:414-418), free-surface tzz row 2 = 0.
+def _energy(state: torch.Tensor, X: int, free_surface: bool, nn: int = 2):
+    """sum over shots of each total squared (elastic_kernels.py:414-418), free-surface tzz row nn = 0.
     A 5-field state (damping scheme) holds the totals themselves."""
     out = []
     for k in range(5):
         tot = state[k] if state.shape[0] == 5 else state[2 * k] + state[2 * k + 1]
         if k == 1 and free_surface:
             tot = tot.clone()
-            tot[:, 2, :] = 0.0
+            tot[:, nn, :] = 0.0
         out.append(tot[:, :, :X].square().sum(0))
     return out
 
@@ -194,7 +199,7 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
             lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream), "adfwi_elastic_forward"))
         cur = work[0] if a.state_out == work[0].data_ptr() else work[1]
         if energy_ends is not None:
-            e = _energy(cur, geom.X, geom.free_surface)
+            e = _energy(cur, geom.X, geom.free_surface, geom.nn)
             energy = e if energy is None else [x + y for x, y in zip(energy, e)]
         t0 = te + 1
     return {"rec": rec, "energy": energy, "ckpt": ckpt, "final": cur}
@@ -258,8 +263,9 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    geometry: Optional[ElasticGeometry] = None, ckpt_every: int = 0,
                    src_fz: Optional[torch.Tensor] = None) -> Dict[str, torch.Tensor]:
     """Elastic forward modelling with the reference's signature and result dictionary
-    (elastic_kernels.py:917-1037), 4th order: the split-field PML scheme for abc_type 'pml' (any
-    case, as the reference's driver tests it), the multiplicative-damping scheme with ``damp``
+    (elastic_kernels.py:917-1037): the split-field PML scheme for abc_type 'pml' (any case, as the
+    reference's driver tests it) at fd_order 4, or the three-term scheme of fd_order 6, 8 and 10
+    (step_forward_PML_6order, :426-584); the 4th-order multiplicative-damping scheme with ``damp``
     otherwise.  ``n_segments`` keeps its meaning for the energy maps (summed over the last step of
     every time chunk); memory is governed by the sqrt(nt) wavefield checkpoints of the HIP
     adjoint.  ``src_fz`` (extension, no counterpart in the reference): per-shot vertical-force
@@ -268,9 +274,10 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
     nat.require_gpu(device)
     if dtype != torch.float32:
         raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
-    if fd_order != 4:
-        raise NotImplementedError(f"the HIP elastic propagator implements the 4th-order scheme (got fd_order={fd_order})")
     use_pml = abc_type.lower() == "pml"
+    if fd_order != 4 and not (use_pml and fd_order in (6, 8, 10)):
+        raise NotImplementedError(f"the HIP elastic propagator implements fd_order 4, and 6 / 8 / 10 with the PML "
+                                  f"(got fd_order={fd_order}, abc_type={abc_type!r})")
     if use_pml and (bcx is None or bcz is None):
         raise ValueError("the PML scheme needs the bcx / bcz damping profiles")
     if not use_pml and damp is None:
@@ -282,7 +289,8 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
     c11, c13, c15, c33, c35, c55 = (pad(CC[i]) for i in (0, 2, 4, 11, 13, 18))
     bxp, bzp = pad(bx), pad(bz)
     aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
-    if geometry is None or geometry.damping == use_pml:
+    fdc = DiffCoef(fso, "s")
+    if geometry is None or geometry.damping == use_pml or geometry.nn != fso:
         zoff = fso if free_surface else nabc + fso
         MTd = MT.to(device=device, dtype=torch.float32)
         if use_pml:
@@ -293,7 +301,7 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
         geometry = ElasticGeometry(Z, X, nt, free_surface, (src_z + zoff).cpu().numpy(),
                                    (src_x + nabc).cpu().numpy(), src_m, (rcv_z + zoff).cpu().numpy(),
                                    (rcv_x + nabc).cpu().numpy(), device, aniso=aniso, damping=not use_pml,
-                                   src_fz=src_fz)
+                                   src_fz=src_fz, nn=fso, c2=float(fdc[2]) if fso > 2 else 0.0)
     elif geometry.aniso != aniso:
         geometry.aniso = aniso
     if use_pml:
@@ -304,7 +312,6 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
         dmp = pad_torchSingle(torch.as_tensor(damp).to(device=device, dtype=torch.float32), 0, fso, free_surface,
                               device)
         absorb_a, absorb_b = geometry.pack_plane(dmp), None
-    fdc = DiffCoef(fso, "s")
     f32 = lambda v: float(np.float32(v))
     scalars = (f32(dt), f32(dt / dx), f32(dt / dz), f32(dx), f32(dz), float(fdc[0]), float(fdc[1]))
     if src_v.dim() == 1:

@@ -15,7 +15,7 @@ from .. import _native as nat
 from ..utils import numpy2tensor
 from .acoustic_propagator import _check_duck, shot_subset
 from .boundary_condition import bc_gerjan, bc_pml_xz, bc_sincos
-from .elastic_kernels import ElasticGeometry, forward_kernel
+from .elastic_kernels import DiffCoef, ElasticGeometry, forward_kernel
 
 _MODEL_ATTRS = ("ox", "oz", "dx", "dz", "nx", "nz", "abc_type", "nabc", "free_surface", "vp", "forward")
 
@@ -78,11 +78,11 @@ class ElasticPropagator(torch.nn.Module):
             damp = bc_sincos(self.nx, self.nz, self.dx, self.dz, pml=self.nabc, free_surface=self.free_surface)
             self.damp = numpy2tensor(damp, self.dtype).to(self.device)
 
-    def _geometry(self, shots, aniso, damping):
-        key = (tuple(int(s) for s in shots), bool(aniso), bool(damping))
+    def _geometry(self, shots, aniso, damping, fd_order=4):
+        key = (tuple(int(s) for s in shots), bool(aniso), bool(damping), int(fd_order))
         g = self._geoms.get(key)
         if g is None:
-            fso = 2
+            fso = fd_order // 2
             Z = self.nz + (self.nabc + fso if self.free_surface else 2 * self.nabc + fso)
             X = self.nx + 2 * self.nabc
             zoff = fso if self.free_surface else self.nabc + fso
@@ -96,7 +96,8 @@ class ElasticPropagator(torch.nn.Module):
             fz = None if self.force_weight is None else self.force_weight[torch.as_tensor(shots, device=self.device)]
             g = ElasticGeometry(Z, X, self.nt, self.free_surface, sl[:, 1] + zoff, sl[:, 0] + self.nabc, src_m,
                                 self.rcv_loc[:, 1] + zoff, self.rcv_loc[:, 0] + self.nabc, self.device, aniso=aniso,
-                                damping=damping, src_fz=fz)
+                                damping=damping, src_fz=fz, nn=fso,
+                                c2=float(DiffCoef(fso, "s")[2]) if fso > 2 else 0.0)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)
@@ -114,10 +115,11 @@ class ElasticPropagator(torch.nn.Module):
         CC = model.CC
         aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
         geom = None
-        if fd_order == 4 and (self.abc_type == "PML") == (self.abc_type.lower() == "pml"):
+        if (self.abc_type == "PML") == (self.abc_type.lower() == "pml") and (
+                fd_order == 4 or (fd_order in (6, 8, 10) and self.abc_type == "PML")):
             # the reference's propagator builds bcx/bcz only for 'PML' exactly (elastic_propagator.py
             # :219) while its driver takes the PML branch for any case of 'pml' (elastic_kernels.py:948)
-            geom = self._geometry(shots, aniso, damping=self.abc_type != "PML")
+            geom = self._geometry(shots, aniso, damping=self.abc_type != "PML", fd_order=fd_order)
         return forward_kernel(self.nx, self.nz, self.dx, self.dz, self.nt, self.dt, self.nabc, self.free_surface,
                               src_x, src_z, len(shots), wavelet, mt, self.rcv_x, self.rcv_z, self.rcv_n,
                               self.abc_type, self.bcx, self.bcz, self.damp, model.lamu, model.lam,

@@ -154,9 +154,9 @@ typedef struct adfwi_elastic_desc {
   int32_t abi_version;     /* ADFWI_ABI_VERSION */
   int32_t struct_size;     /* sizeof(adfwi_elastic_desc) */
   int32_t nshots;          /* S */
-  int32_t nz, nx;          /* padded grid (two free-surface rows on top included) */
+  int32_t nz, nx;          /* padded grid (the nn free-surface padding rows on top included) */
   int32_t pitch;           /* row pitch in floats (>= nx) */
-  int32_t free_surface;    /* 0 / 1: free-surface rows 0..2 as in the reference */
+  int32_t free_surface;    /* 0 / 1: free-surface rows nn-2 .. nn+1 as in the reference */
   int32_t nt, nrec;
   int32_t aniso;           /* 1 when C15 or C35 is non-zero (general 2-D anisotropy) */
   int32_t shots_per_wave;  /* 0 = library choice */
@@ -179,6 +179,11 @@ typedef struct adfwi_elastic_desc {
                               added to vz at the source point right after the velocity update
                               (PML: f w / 2 into each vz split; damping: f w, before the taper).
                               No counterpart in the reference, whose sources are moment tensors. */
+  int32_t nn;              /* stencil half-order NN = fd_order / 2: 0 or 2 for the 4th-order schemes;
+                              3, 4, 5 for the three-term PML scheme of fd_order 6, 8, 10
+                              (step_forward_PML_6order, elastic_kernels.py:426-584: window rows
+                              [NN, nz-NN-1], free-surface rows NN-2 .. NN+1, padding NN rows on top) */
+  float c2;                /* third coefficient of DiffCoef(NN, 's') (NN >= 3) */
 } adfwi_elastic_desc;
 
 /* Tiling and buffer sizes; layout.nfields = 10 (PML split fields) or 5 (damping), layout.reserved = shots per
@@ -200,7 +205,8 @@ typedef struct adfwi_elastic_fwd_args {
   const float *state_out;  /* OUT */
 } adfwi_elastic_fwd_args;
 
-/* Forward time stepping (replaces step_forward_PML_4order, elastic_kernels.py:222-423). */
+/* Forward time stepping (replaces step_forward_PML_4order, elastic_kernels.py:222-423,
+ * step_forward_PML_6order :426-584 and step_forward_ABL_4order :585-777). */
 int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a, adfwi_stream_t stream);
 
 typedef struct adfwi_elastic_adj_args {

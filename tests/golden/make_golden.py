@@ -17,8 +17,9 @@ the inputs and outputs of:
   five records, five energy maps and gradients with respect to C11, C13, C33,
   C55, bx, bz and the wavelet, isotropic with free surface and VTI without,
   segments 1 and 3, for the split-field PML branch and for the damping (ABL)
-  branch with Cerjan (free surface) and sin^2 (no free surface) tapers;
-  DiffCoef(2, 's') (:20-58);
+  branch with Cerjan (free surface) and sin^2 (no free surface) tapers; the
+  three-term PML scheme of fd_order 6, 8 and 10 (step_forward_PML_6order,
+  :426-584); DiffCoef(2, 's'), DiffCoef(3, 's') and DiffCoef(4, 's') (:20-58);
 * ADFWI/propagator/boundary_condition.py bc_pml, bc_pml_xz, bc_sincos,
   bc_gerjan (:15-120).
 
@@ -179,9 +180,12 @@ def main():
         "elastic_abl_sincos_nofs": lambda: elastic_case(ek, bc, False, True, 24, abc="sincos"),
         "elastic_fd6_iso_fs": lambda: elastic_case(ek, bc, True, False, 25, order=6),
         "elastic_fd6_vti_nofs": lambda: elastic_case(ek, bc, False, True, 26, order=6),
+        "elastic_fd8_iso_fs": lambda: elastic_case(ek, bc, True, False, 27, order=8),
+        "elastic_fd10_vti_nofs": lambda: elastic_case(ek, bc, False, True, 28, order=10),
         "boundary": lambda: boundary_case(bc),
         "diffcoef": lambda: dict(fdc=ek.DiffCoef(2, "s").numpy()),
-        "diffcoef6": lambda: dict(fdc6=ek.DiffCoef(3, "s").numpy(), fdc8=ek.DiffCoef(4, "s").numpy()),
+        "diffcoef6": lambda: dict(fdc6=ek.DiffCoef(3, "s").numpy(), fdc8=ek.DiffCoef(4, "s").numpy(),
+                                  fdc10=ek.DiffCoef(5, "s").numpy()),
     }
     names = sys.argv[1:] or list(cases)   # write only the named cases when given
     for name in names:

@@ -35,7 +35,8 @@ def run_acoustic_oracle(g, m, seg=1, dtype=torch.float32, want_grad=True, weight
 def elastic_inputs(name):
     g = golden(name)
     meta = dict(nz=int(g["nz"]), nx=int(g["nx"]), nabc=int(g["nabc"]), dx=float(g["dx"]),
-                dt=float(g["dt"]), nt=int(g["nt"]), fs=bool(g["free_surface"]))
+                dt=float(g["dt"]), nt=int(g["nt"]), fs=bool(g["free_surface"]),
+                fd=int(g["fd_order"]) if "fd_order" in g else 4)
     return g, meta
 
 
@@ -89,7 +90,8 @@ def _smooth(rng, nz, nx, lo, hi, passes=4):
     return lo + (hi - lo) * a
 
 
-def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=False, dx=5.0, dt=4e-4, abc="PML"):
+def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=False, dx=5.0, dt=4e-4, abc="PML",
+                        fd_order=4):
     """A golden-shaped elastic case on a larger grid (no reference outputs): smooth fields,
     staggered shapes as the reference's models build them, shots spread over the grid,
     receivers along one row.  aniso adds small C15/C35 fields."""
@@ -129,7 +131,7 @@ def elastic_random_case(nz, nx, nabc, nt, nshots, free_surface, seed, aniso=Fals
         g["damp"] = (ob.bc_gerjan(nx, nz, dx, dx, nabc, alpha=0.0053, free_surface=free_surface) if abc == "gerjan"
                      else ob.bc_sincos(nx, nz, dx, dx, nabc, free_surface=free_surface))
     g["weights"] = rng.standard_normal((5, nshots, nt, len(g["rcv_x"]))).astype(np.float32)
-    meta = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, fs=bool(free_surface))
+    meta = dict(nz=nz, nx=nx, nabc=nabc, dx=dx, dt=dt, nt=nt, fs=bool(free_surface), fd=int(fd_order))
     return g, meta
 
 

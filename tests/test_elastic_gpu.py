@@ -19,13 +19,15 @@ def _oracle_setup(g, m, *, shots_per_wave=0, tile_z=0, fdc=None):
     from adfwi_amd.propagator.elastic_kernels import ElasticGeometry, pml_factors
     from oracle import elastic as oe
     nz, nx, nabc, fs = m["nz"], m["nx"], m["nabc"], m["fs"]
+    nn = m["fd"] // 2
+    fdc = oe.diff_coef(nn) if fdc is None else fdc
     cc = elastic_cc_general(g, nz, nx)
     f = oe.padded_fields(torch.tensor(g["bx"]), torch.tensor(g["bz"]), cc,
                          torch.tensor(g["bcx"], dtype=torch.float32), torch.tensor(g["bcz"], dtype=torch.float32),
-                         nabc, fs)
-    Z = nz + (nabc + 2 if fs else 2 * nabc + 2)
+                         nabc, fs, m["fd"])
+    Z = nz + (nabc + nn if fs else 2 * nabc + nn)
     X = nx + 2 * nabc
-    zoff = 2 if fs else nabc + 2
+    zoff = nn if fs else nabc + nn
     mt = torch.tensor(g["mt"])
     damping = elastic_abc(g) != "PML"
     if damping:
@@ -36,15 +38,15 @@ def _oracle_setup(g, m, *, shots_per_wave=0, tile_z=0, fdc=None):
     aniso = "c15" in g
     geom = ElasticGeometry(Z, X, m["nt"], fs, g["src_z"] + zoff, g["src_x"] + nabc, src_m,
                            g["rcv_z"] + zoff, g["rcv_x"] + nabc, DEV, aniso=aniso,
-                           shots_per_wave=shots_per_wave, tile_z=tile_z, damping=damping)
+                           shots_per_wave=shots_per_wave, tile_z=tile_z, damping=damping, nn=nn,
+                           c2=float(fdc[2]) if nn > 2 else 0.0)
     fields = [f[k] for k in ("c11", "c13", "c33", "c55", "bx", "bz")]
     fields += [f["c15"], f["c35"]] if aniso else [None, None]
     coef = geom.pack_coef([None if t is None else t.to(DEV) for t in fields])
     if damping:
-        px, pz = geom.pack_plane(oe.pad_elastic(elastic_damp(g), 0, 2, fs).to(DEV)), None
+        px, pz = geom.pack_plane(oe.pad_elastic(elastic_damp(g), 0, nn, fs).to(DEV)), None
     else:
         px, pz = pml_factors(f["bcx"], f["bcz"], m["dt"], geom.pitch)
-    fdc = oe.diff_coef(2) if fdc is None else fdc
     f32 = lambda v: float(np.float32(v))
     scalars = (f32(m["dt"]), f32(m["dt"] / m["dx"]), f32(m["dt"] / m["dx"]), f32(m["dx"]), f32(m["dx"]),
                float(fdc[0]), float(fdc[1]))
@@ -84,7 +86,7 @@ def _run_oracle(g, m, seg=1, want_grad=False):
                             torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
                             len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
                             torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z,
-                            leaves["bx"], leaves["bz"], cc, n_segments=seg)
+                            leaves["bx"], leaves["bz"], cc, n_segments=seg, fd_order=m["fd"])
     out = {k: res[k].detach().numpy() for k in ELASTIC_NAMES}
     out.update({"fw_" + k: res["forward_wavefield_" + k].numpy() for k in ELASTIC_NAMES})
     if want_grad:
@@ -95,23 +97,37 @@ def _run_oracle(g, m, seg=1, want_grad=False):
     return out
 
 
-def _dropin(g, m, seg=1, want_grad=False, wavelet=None, **leaf_kw):
-    """The drop-in forward_kernel on the GPU with the golden case's inputs."""
-    from adfwi_amd.propagator.elastic_kernels import forward_kernel
+def _reference_coefficients(fd):
+    """The reference's DiffCoef(fd // 2, 's') as its host solved it (a float32 inverse of an
+    ill-conditioned Vandermonde matrix, whose last bits depend on the host's LAPACK: at fd_order
+    10 another host's solve moves the records by ~1e-5)."""
+    from conftest import golden
+    return torch.tensor(golden("diffcoef")["fdc"] if fd == 4 else golden("diffcoef6")[f"fdc{fd}"])
+
+
+def _dropin(g, m, seg=1, want_grad=False, wavelet=None, fdc=None, **leaf_kw):
+    """The drop-in forward_kernel on the GPU with the case's inputs; fdc: FD coefficients to
+    feed it in place of its own DiffCoef solve (the reference's, for comparisons with the
+    reference's vectors; the product's DiffCoef is checked against them on the CPU)."""
+    from unittest import mock
+    from adfwi_amd.propagator import elastic_kernels as ek
     nz, nx = m["nz"], m["nx"]
     leaves, cc = _leaves(g, DEV, want_grad)
     w = torch.tensor(g["wavelet"] if wavelet is None else wavelet, device=DEV, requires_grad=want_grad)
     z = torch.zeros((nz, nx), device=DEV)
-    res = forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
-                         torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
-                         torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
-                         len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
-                         torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z, leaves["bx"], leaves["bz"],
-                         cc, fd_order=4, n_segments=seg, device=DEV, **leaf_kw)
+    own = ek.DiffCoef
+    with mock.patch.object(ek, "DiffCoef", lambda order, scheme: own(order, scheme) if fdc is None else fdc.clone()):
+        res = ek.forward_kernel(nx, nz, m["dx"], m["dx"], m["nt"], m["dt"], m["nabc"], m["fs"],
+                                torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), len(g["src_x"]), w,
+                                torch.tensor(g["mt"]), torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]),
+                                len(g["rcv_x"]), elastic_abc(g), torch.tensor(g["bcx"], dtype=torch.float32),
+                                torch.tensor(g["bcz"], dtype=torch.float32), elastic_damp(g), z, z, leaves["bx"],
+                                leaves["bz"], cc, fd_order=m["fd"], n_segments=seg, device=DEV, **leaf_kw)
     return res, leaves, w
 
 
-ALL_CASES = ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs", "elastic_abl_sincos_nofs"]
+ALL_CASES = ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs", "elastic_abl_sincos_nofs",
+             "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs", "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs"]
 
 
 @pytest.mark.parametrize("name", ALL_CASES)
@@ -120,9 +136,8 @@ def test_kernel_bit_exact_given_oracle_coefficients(name):
     fp32 association, stage order, split-field PML and free-surface rows.  The FD coefficients
     are the reference's own (a float32 matrix solve whose last bit can depend on the host's
     LAPACK), taken from the fixture."""
-    from conftest import golden
     g, m = elastic_inputs(name)
-    fdc = torch.tensor(golden("diffcoef")["fdc"])
+    fdc = _reference_coefficients(m["fd"])
     geom, coef, px, pz, src, scalars = _oracle_setup(g, m, fdc=fdc)
     res = _run(geom, coef, px, pz, src, scalars, record=True)
     torch.cuda.synchronize()
@@ -135,7 +150,7 @@ def test_kernel_bit_exact_given_oracle_coefficients(name):
 @pytest.mark.parametrize("seg", [1, 3])
 def test_dropin_forward_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
-    res, _, _ = _dropin(g, m, seg=seg)
+    res, _, _ = _dropin(g, m, seg=seg, fdc=_reference_coefficients(m["fd"]))
     torch.cuda.synchronize()
     for k in ELASTIC_NAMES:
         got = res[k].detach().cpu().numpy()
@@ -149,12 +164,16 @@ def test_dropin_forward_matches_reference_vectors(name, seg):
 @pytest.mark.parametrize("fs,aniso,spw,tz,abc", [(True, False, 1, 4, "PML"), (False, False, 2, 8, "PML"),
                                                   (True, True, 1, 16, "PML"), (False, True, 0, 0, "PML"),
                                                   (True, False, 1, 4, "gerjan"), (False, True, 2, 8, "sincos"),
-                                                  (True, True, 0, 0, "gerjan")])
+                                                  (True, True, 0, 0, "gerjan"),
+                                                  (True, False, 1, 8, "PML6"), (False, True, 2, 16, "PML6"),
+                                                  (True, True, 0, 0, "PML8"), (False, False, 1, 8, "PML10")])
 def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz, abc):
-    """A grid of several 55-column strips and row tiles, several shots per wavefront: strip
-    edges, tile edges, PML corners, both absorbing schemes and the anisotropic C15/C35 terms
-    against the oracle."""
-    g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso, abc=abc)
+    """A grid of several 55-column (51 at orders 6-10) strips and row tiles, several shots per
+    wavefront: strip edges, tile edges, PML corners, both absorbing schemes, the three-term
+    scheme of orders 6, 8, 10 and the anisotropic C15/C35 terms against the oracle."""
+    fd = int(abc[3:]) if abc.startswith("PML") and len(abc) > 3 else 4
+    g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso,
+                               abc=abc[:3] if fd != 4 else abc, fd_order=fd)
     ref = _run_oracle(g, m)
     geom, coef, px, pz, src, scalars = _oracle_setup(g, m, shots_per_wave=spw, tile_z=tz)
     assert geom.layout.ntiles_x >= 3 and geom.layout.ntiles_z >= 2 or tz == 0
@@ -175,7 +194,7 @@ def test_dropin_gradients_match_reference_vectors(name, seg):
     """Hand-written adjoint vs the reference's autograd gradients of a seeded linear functional
     of all five records (C11, C13, C33, C55, bx, bz on their own staggered shapes, wavelet)."""
     g, m = elastic_inputs(name)
-    res, leaves, w = _dropin(g, m, seg=seg, want_grad=True)
+    res, leaves, w = _dropin(g, m, seg=seg, want_grad=True, fdc=_reference_coefficients(m["fd"]))
     _loss(res, g["weights"]).backward()
     torch.cuda.synchronize()
     for k in ("c11", "c13", "c33", "c55", "bx", "bz"):
@@ -187,11 +206,16 @@ def test_dropin_gradients_match_reference_vectors(name, seg):
 
 
 @pytest.mark.parametrize("fs,aniso,abc", [(True, False, "PML"), (False, True, "PML"), (True, True, "PML"),
-                                          (True, False, "gerjan"), (False, True, "sincos"), (True, True, "sincos")])
+                                          (True, False, "gerjan"), (False, True, "sincos"), (True, True, "sincos"),
+                                          (True, False, "PML6"), (False, True, "PML6"), (True, True, "PML8"),
+                                          (True, False, "PML10")])
 def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
     """Several strips and row tiles, several shot groups, anisotropic C15/C35 gradients too, both
-    absorbing schemes, against the oracle's autograd gradients."""
-    g, m = elastic_random_case(44, 150, 8, 60, 9, fs, seed=21 + int(fs) + 2 * int(aniso), aniso=aniso, abc=abc)
+    absorbing schemes and the three-term scheme of orders 6, 8, 10, against the oracle's autograd
+    gradients."""
+    fd = int(abc[3:]) if abc.startswith("PML") and len(abc) > 3 else 4
+    g, m = elastic_random_case(44, 150, 8, 60, 9, fs, seed=21 + int(fs) + 2 * int(aniso), aniso=aniso,
+                               abc=abc[:3] if fd != 4 else abc, fd_order=fd)
     ref = _run_oracle(g, m, want_grad=True)
     res, leaves, w = _dropin(g, m, want_grad=True)
     _loss(res, g["weights"]).backward()
@@ -202,11 +226,13 @@ def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
     assert rel_l2(w.grad.cpu().numpy(), ref["grad_wavelet"]) <= 1e-4
 
 
-@pytest.mark.parametrize("abc", ["PML", "gerjan"])
+@pytest.mark.parametrize("abc", ["PML", "gerjan", "PML6"])
 def test_adjoint_dot_product_in_the_wavelet(abc):
     """Records are linear in the wavelet: <J dw, dd> = <dw, J^T dd> with J^T dd the wavelet
     gradient of the hand-written adjoint (free surface and anisotropy on, several tiles)."""
-    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True, abc=abc)
+    fd = 6 if abc == "PML6" else 4
+    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True, abc=abc[:3] if fd != 4 else abc,
+                               fd_order=fd)
     rng = np.random.default_rng(3)
     dw = rng.standard_normal(g["wavelet"].shape).astype(np.float32)
     dd = rng.standard_normal(g["weights"].shape).astype(np.float32)
@@ -230,7 +256,7 @@ def test_checkpoint_interval_does_not_change_gradients():
             assert rel_l2(b, a) <= 1e-6
 
 
-@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs"])
+@pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs", "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs"])
 def test_elastic_propagator_class_matches_reference_vectors(name):
     """ElasticPropagator(model, survey).forward() with the reference's model parameterisation
     (IsotropicElasticModel / AnisotropicElasticModel from vp, vs, rho, eps, delta) reproduces the
@@ -256,7 +282,7 @@ def test_elastic_propagator_class_matches_reference_vectors(name):
                                       **common)
     prop = ElasticPropagator(model, Survey(src, rcv), device=DEV)
     with torch.no_grad():
-        res = prop.forward()
+        res = prop.forward(fd_order=m["fd"])
     for k in ELASTIC_NAMES:
         got = res[k].cpu().numpy()
         assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))

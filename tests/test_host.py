@@ -39,6 +39,17 @@ def test_product_profiles_equal_reference_vectors():
 
 
 # ------------------------------------------------------------------ models
+def test_fd_coefficients_equal_the_reference_solve():
+    """The product's DiffCoef(n, 's') reproduces the reference's float32 solve on this host
+    (n = 2..5, the stencils of fd_order 4..10) and the staggered-grid closed forms."""
+    from adfwi_amd.propagator.elastic_kernels import DiffCoef
+    assert np.array_equal(DiffCoef(2, "s").numpy(), golden("diffcoef")["fdc"])
+    for n in (3, 4, 5):
+        assert np.array_equal(DiffCoef(n, "s").numpy(), golden("diffcoef6")[f"fdc{2 * n}"]), n
+    assert np.allclose(DiffCoef(2, "s").numpy(), [9 / 8, -1 / 24], rtol=1e-6)
+    assert np.allclose(DiffCoef(3, "s").numpy(), [75 / 64, -25 / 384, 3 / 640], rtol=3e-5)
+
+
 def test_wavelet_closed_form():
     from adfwi_amd.utils import wavelet
     t, w = wavelet(500, 1e-3, 10.0)

@@ -22,7 +22,8 @@ def test_acoustic_oracle_matches_reference_vectors(name, seg):
 
 
 @pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs",
-                                  "elastic_abl_sincos_nofs", "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs"])
+                                  "elastic_abl_sincos_nofs", "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs",
+                                  "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs"])
 @pytest.mark.parametrize("seg", [1, 3])
 def test_elastic_oracle_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
@@ -44,6 +45,11 @@ def test_fd_coefficients_closed_form():
     c6 = diff_coef(3).numpy()
     assert np.array_equal(c6, golden("diffcoef6")["fdc6"])
     assert np.allclose(c6, [75 / 64, -25 / 384, 3 / 640], rtol=3e-5, atol=0)
+    # 8th order: 1225/1024, -245/3072, 49/5120, -5/7168
+    c8 = diff_coef(4).numpy()
+    assert np.array_equal(c8, golden("diffcoef6")["fdc8"])
+    assert np.allclose(c8, [1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168], rtol=3e-4, atol=0)
+    assert np.array_equal(diff_coef(5).numpy(), golden("diffcoef6")["fdc10"])
 
 
 def test_oracle_boundary_profiles_match_reference_vectors():
