@@ -1278,13 +1278,12 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
       push(vx, ok ? a : 0.f);
       push(vz, ok ? b : 0.f);
     }
-    // loaded one iteration ahead: velocity splits of row k+3, stress splits and moduli of
-    // row k, old velocity splits and buoyancies of row k-3
+    // loaded one iteration ahead: velocity splits of row k+3, stress splits and moduli of row k
     struct Row {
-      float v[4], s[6], cm[ANISO ? 6 : 4], u[4], b[2];
+      float v[4], s[6], cm[ANISO ? 6 : 4];
     };
     auto load = [&](int k, Row &R) {
-      const unsigned o3 = rb(k + 3), o0 = rb(k), om = rb(k - 3);
+      const unsigned o3 = rb(k + 3), o0 = rb(k);
 #pragma unroll
       for (int q = 0; q < 4; ++q) R.v[q] = bld(rIn, vc, F(VX_X + q) + o3);
 #pragma unroll
@@ -1294,17 +1293,20 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
       if constexpr (ANISO) {
         R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
       }
-      if (k - 3 >= z0 && k - 3 < end_row) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) R.u[q] = bld(rIn, vc, F(VX_X + q) + om);
-        R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
-      }
     };
     Row cur;
     load(z0 - 3, cur);
 #pragma unroll 1
     for (int kk = 0; kk < nk; ++kk) {
       const int k = z0 - 3 + kk;
+      // old velocity splits and buoyancies of row k-3, consumed at the end of the iteration
+      float u[4] = {}, bb[2] = {};
+      if (k - 3 >= z0 && k - 3 < end_row) {
+        const unsigned om = rb(k - 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[q] = bld(rIn, vc, F(VX_X + q) + om);
+        bb[0] = bld(rC, vc, BX * planeb + om); bb[1] = bld(rC, vc, BZ * planeb + om);
+      }
       Row nxt;
       load(k + 1, nxt);
       {
@@ -1388,12 +1390,12 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
       if (out_col && r >= z0 && r < end_row) {
         const unsigned o = rb(r);
         const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
-        const float bxv = cur.b[0], bzv = cur.b[1];
+        const float bxv = bb[0], bzv = bb[1];
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
-        const float nx_x = (pxn * cur.u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
-        const float nx_z = (pzn * cur.u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
-        float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
-        float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        const float nx_x = (pxn * u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        float nz_x = (pxn * u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        float nz_z = (pzn * u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
         if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
           const float hf = 0.5f * (d.src_fz[s] * wsrc);
           nz_x = nz_x + hf;
@@ -1498,13 +1500,14 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
                bld(rC, vc, BX * planeb + o), bld(rC, vc, BZ * planeb + o));
     }
     // loaded one iteration ahead: row k+3 of the forward velocity splits, the velocity
-    // adjoints and the buoyancies; row k of the stress adjoints, the moduli and the forward
-    // stress splits; row k-3 of the velocity adjoints
+    // adjoints and the buoyancies; row k of the stress adjoints and the moduli.  The forward
+    // stress splits of row k and the velocity adjoints of row k-3, consumed late in the
+    // iteration, are loaded at its start (fewer live registers: three waves per SIMD)
     struct Row {
-      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4], fs[6], lr[4];
+      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4];
     };
     auto load = [&](int k, Row &R) {
-      const unsigned o3 = rb(k + 3), o0 = rb(k), om = rb(k - 3);
+      const unsigned o3 = rb(k + 3), o0 = rb(k);
 #pragma unroll
       for (int q = 0; q < 4; ++q) R.fv[q] = bld(rFv, vc, F(VX_X + q) + o3);
 #pragma unroll
@@ -1517,18 +1520,22 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
       if constexpr (ANISO) {
         R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
       }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) R.fs[q] = bld(rFs, vc, F(TXX_X + q) + o0);
-      if (k - 3 >= z0 && k - 3 < end_row) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) R.lr[q] = bld(rLi, vc, F(VX_X + q) + om);
-      }
     };
     Row cur;
     load(z0 - 3, cur);
 #pragma unroll 1
     for (int kk = 0; kk < nk; ++kk) {
       const int k = z0 - 3 + kk;
+      float fs[6], lr[4] = {};
+      {
+        const unsigned o0 = rb(k), om = rb(k - 3);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) fs[q] = bld(rFs, vc, F(TXX_X + q) + o0);
+        if (k - 3 >= z0 && k - 3 < end_row) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) lr[q] = bld(rLi, vc, F(VX_X + q) + om);
+        }
+      }
       Row nxt;
       load(k + 1, nxt);
       push_row(k + 3, cur.fv[0] + cur.fv[1], cur.fv[2] + cur.fv[3], cur.lv[0], cur.lv[1], cur.lv[2], cur.lv[3],
@@ -1622,9 +1629,9 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
       // forward stress totals of row k (after the step), free surface once row NN+1 exists
       {
         const bool in_grid = k >= 0 && k <= zmax;
-        push(xx, in_grid ? cur.fs[0] + cur.fs[1] : 0.f);
-        push(zz, in_grid ? cur.fs[2] + cur.fs[3] : 0.f);
-        push(xz, in_grid ? cur.fs[4] + cur.fs[5] : 0.f);
+        push(xx, in_grid ? fs[0] + fs[1] : 0.f);
+        push(zz, in_grid ? fs[2] + fs[3] : 0.f);
+        push(xz, in_grid ? fs[4] + fs[5] : 0.f);
       }
       if (d.fsurf && k == nn + 1) {
         zz[5] = 0.0f; zz[4] = -zz[6]; xz[4] = -xz[5]; xz[3] = -xz[6];
@@ -1652,7 +1659,7 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
         const unsigned o = rb(r);
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
-        const float l1 = cur.lr[0], l2 = cur.lr[1], l3 = cur.lr[2], l4 = cur.lr[3];
+        const float l1 = lr[0], l2 = lr[1], l3 = lr[2], l4 = lr[3];
         if (d.src_fz && gsrc && r == zs && c == xs)
           gsrc[(long long)s * d.nt + t] = gmt + 0.5f * (d.src_fz[s] * (l3 + l4));
         bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);

@@ -54,7 +54,7 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[to
         fa.state_in = fwd0.data_ptr()
         fa.trace = trace.data_ptr()
         fa.t0, fa.nsteps = t0, n
-        TIMELINE.call("recompute", "el_abl_fwd" if geom.damping else "el_fwd_step", n, dev, lambda: nat.check(
+        TIMELINE.call("recompute", geom.kernel_names[0], n, dev, lambda: nat.check(
             lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(fa), stream), "adfwi_elastic_forward (recompute)"))
         aa = nat.ElasticAdjArgs()
         aa.lam_in = cur.data_ptr()
@@ -65,7 +65,7 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[to
             aa.grad_rec[i] = None if g[i] is None else g[i].data_ptr()
         aa.grad_coef = gacc.data_ptr()
         aa.grad_src = gsrc.data_ptr()
-        TIMELINE.call("adjoint", "el_abl_adj" if geom.damping else "el_adj_step", n, dev, lambda: nat.check(
+        TIMELINE.call("adjoint", geom.kernel_names[1], n, dev, lambda: nat.check(
             lib.adfwi_elastic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream), "adfwi_elastic_adjoint"))
         cur = lam[0] if aa.lam_out == lam[0].data_ptr() else lam[1]
     planes = gacc.view(-1, NCOEF, Z, pitch).sum(0)

@@ -71,6 +71,9 @@ class ElasticGeometry:
         # stencil half-order fd_order // 2: 2 for the 4th-order schemes, 3..5 for the three-term
         # PML scheme (window [nn, Z-nn-1], free-surface rows nn-2 .. nn+1), c2 its third coefficient
         self.nn, self.c2 = int(nn), float(c2)
+        # (forward, adjoint) step kernels, as the timeline and profiles name them
+        self.kernel_names = (("el_abl_fwd", "el_abl_adj") if self.damping else
+                             ("el3_fwd_step", "el3_adj_step") if self.nn > 2 else ("el_fwd_step", "el_adj_step"))
         self.pitch = -(-self.X // PITCH_ALIGN) * PITCH_ALIGN
         src_z = np.asarray(src_z, dtype=np.int64).reshape(-1)
         src_x = np.asarray(src_x, dtype=np.int64).reshape(-1)
@@ -195,7 +198,7 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
         if rec is not None:
             for i in range(5):
                 a.rec[i] = rec[i].data_ptr()
-        TIMELINE.call("forward", "el_abl_fwd" if geom.damping else "el_fwd_step", a.nsteps, dev, lambda: nat.check(
+        TIMELINE.call("forward", geom.kernel_names[0], a.nsteps, dev, lambda: nat.check(
             lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream), "adfwi_elastic_forward"))
         cur = work[0] if a.state_out == work[0].data_ptr() else work[1]
         if energy_ends is not None:

@@ -60,7 +60,12 @@ class Workload:
     @property
     def padded_elastic(self):
         """Padded 4th-order elastic grid (Z, X): two free-surface rows on top."""
-        top = self.nabc + 2 if self.free_surface else 2 * self.nabc + 2
+        return self.padded_elastic_order(4)
+
+    def padded_elastic_order(self, fd_order: int):
+        """Padded elastic grid (Z, X) of an fd_order: fd_order // 2 free-surface rows on top."""
+        nn = fd_order // 2
+        top = self.nabc + nn if self.free_surface else 2 * self.nabc + nn
         return self.nz + top, self.nx + 2 * self.nabc
 
     def wavelets(self):

@@ -83,7 +83,7 @@ def build_problem(wl, dev):
     return prop, init, obs
 
 
-def build_elastic_problem(wl, dev):
+def build_elastic_problem(wl, dev, fd_order=4):
     """Elastic problem of an elastic workload: vertical-force Ricker shots; the true model gives
     the observed vx / vz, the initial model (each field smoothed, sigma 10 cells) carries the
     gradients; rho is a leaf."""
@@ -112,15 +112,15 @@ def build_elastic_problem(wl, dev):
     true, init = mk(lambda a: a, False), mk(smooth, True)
     prop = ElasticPropagator(init, survey, device=dev)
     with torch.no_grad():
-        res = prop.forward(model=true)
+        res = prop.forward(model=true, fd_order=fd_order)
         obs = {k: res[k].detach().clone() for k in ("vx", "vz")}
     return prop, init, obs
 
 
-def grad_step(prop, model, obs, dt):
+def grad_step(prop, model, obs, dt, fd_order=4):
     for p in model.parameters():
         p.grad = None
-    rec = prop.forward()
+    rec = prop.forward(fd_order=fd_order) if fd_order != 4 else prop.forward()
     if isinstance(obs, dict):
         loss = sum(l2_misfit(rec[k], obs[k], dt) for k in obs)
     else:
@@ -181,7 +181,7 @@ def cpu_baseline(seconds_budget=24.0):
             "cpu": _cpu_name()}
 
 
-def elastic_cpu_baseline(wl, seconds_budget=24.0):
+def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4):
     """The CPU oracle (eager torch fp32, the reference's op order, autograd) on a bounded sample of
     the elastic workload: all shots cut to `nb` shots and a few time steps, scaled per shot-step."""
     from adfwi_amd.propagator.boundary_condition import bc_pml_xz
@@ -214,7 +214,7 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0):
                                 torch.tensor(np.tile(np.eye(3, dtype=np.float32), (nb, 1, 1))),
                                 torch.tensor(wl.rcv_x), torch.tensor(wl.rcv_z), len(wl.rcv_x), "PML",
                                 torch.tensor(bcx, dtype=torch.float32), torch.tensor(bcz, dtype=torch.float32),
-                                None, z, z, bx, bz, CC)
+                                None, z, z, bx, bz, CC, fd_order=fd_order)
         (res["vx"].square().sum() + res["vz"].square().sum()).backward()
     t0 = time.perf_counter()
     one(10)
@@ -228,7 +228,8 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0):
     t = float(np.median(times)) * (wl.nt / nt_run) * (wl.nshots / nb)
     return {"value": wl.nshots / t, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
             "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward+backward, full "
-                       f"{nz}x{nx} grid, {nb} of {wl.nshots} shots, {nt_run} of {wl.nt} steps, median of 3, "
+                       f"{nz}x{nx} grid, fd_order {fd_order}, {nb} of {wl.nshots} shots, {nt_run} of {wl.nt} steps, "
+                       f"median of 3, "
                        f"scaled per shot and step; explosive moment-tensor shots (the reference has no "
                        f"vertical-force source; the stepping cost is the same)"),
             "cpu": _cpu_name()}
@@ -283,6 +284,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="acoustic_large")
     ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs only)")
+    ap.add_argument("--fd-order", type=int, default=4, choices=(4, 6, 8, 10),
+                    help="elastic workloads: FD order (6-10: the three-term PML scheme)")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (PMC counter calibration)")
@@ -308,11 +311,14 @@ def main():
     kw = {"nt": args.nt} if args.nt else {}
     wl = getattr(workloads, args.workload)(**kw)
     elastic = wl.name.startswith("elastic")
-    log(f"workload {wl.name}: grid {wl.nz}x{wl.nx}, nt {wl.nt}, {wl.nshots} shots")
-    prop, model, obs = (build_elastic_problem if elastic else build_problem)(wl, dev)
+    fd = args.fd_order if elastic else 4
+    if fd != 4 and not elastic:
+        sys.exit("--fd-order applies to the elastic workloads")
+    log(f"workload {wl.name}: grid {wl.nz}x{wl.nx}, nt {wl.nt}, {wl.nshots} shots, fd_order {fd}")
+    prop, model, obs = build_elastic_problem(wl, dev, fd) if elastic else build_problem(wl, dev)
     log("observed data modelled")
     for i in range(args.warmup):
-        grad_step(prop, model, obs, wl.dt)
+        grad_step(prop, model, obs, wl.dt, fd)
         torch.cuda.synchronize()
         log(f"warmup step {i} done")
     torch.cuda.synchronize()
@@ -320,13 +326,13 @@ def main():
     ak.TIMELINE.enabled = True
     t0 = time.perf_counter()
     for i in range(args.steps):
-        loss = grad_step(prop, model, obs, wl.dt)
+        loss = grad_step(prop, model, obs, wl.dt, fd)
         log(f"timed step {i} issued")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ak.TIMELINE.enabled = False
     tl = ak.TIMELINE.summary()
-    Z, X = wl.padded_elastic if elastic else wl.padded
+    Z, X = wl.padded_elastic_order(fd) if elastic else wl.padded
     N = Z * X
     S = wl.nshots
     value = S * args.steps / elapsed
@@ -338,7 +344,7 @@ def main():
             byts = (steps_by.get("forward", 0) * BYTES_FWD + steps_by.get("recompute", 0) * BYTES_RECOMPUTE) * N * S
         elif name in ("ac_adj_step", "ac_adj_pair"):
             byts = d["steps"] * BYTES_ADJ * N * S
-        elif name == "el_fwd_step":
+        elif name in ("el_fwd_step", "el3_fwd_step"):
             byts = d["steps"] * EL_BYTES_FWD * N * S
         else:
             byts = d["steps"] * EL_BYTES_ADJ * N * S
@@ -376,6 +382,7 @@ def main():
     }
     log(f"timed region {elapsed:.2f} s, {value:.2f} shot-gradients/s")
     if elastic:
+        out["config"]["fd_order"] = fd
         out["config"]["free_surface"] = wl.free_surface
         out["config"]["components"] = ["vx", "vz"]
     if not args.no_baselines and not elastic:
@@ -385,7 +392,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         log("CPU baseline done")
     elif not args.no_baselines:
-        out["cpu_baseline"] = elastic_cpu_baseline(wl)
+        out["cpu_baseline"] = elastic_cpu_baseline(wl, fd_order=fd)
         log("CPU baseline done")
     else:
         out["cpu_baseline"] = None
