@@ -3,11 +3,13 @@
 Test infrastructure only.  Restates ADFWI/propagator/elastic_kernels.py:
 DiffCoef (:20-58), the 4th- and 6th-order D operators (:66-108), pad_torchSingle
 (:175-216), step_forward_PML_4order (:222-423), step_forward_PML_6order (:426-584),
-step_forward_ABL_4order (:585-777) and forward_kernel (:917-1037), keeping the
+step_forward_ABL_4order (:585-777), step_forward_ABL_6order (:781-912) and
+forward_kernel (:917-1037), keeping the
 fp32 association of every expression, the split-field PML, the moment-tensor
 source (-MT/2 into both splits), the free-surface rows and the per-step
 overwrite of the energy maps.  For fd_order 6, 8 and 10 the reference runs
-step_forward_PML_6order, whose three-term operators use the first three
+step_forward_PML_6order (step_forward_ABL_6order for the damping scheme), whose
+three-term operators use the first three
 coefficients of DiffCoef(fd_order // 2) on the window [fd_order // 2, Z - fd_order // 2 - 1]
 with the free-surface rows at h = fd_order // 2 + 1.
 """
@@ -346,6 +348,72 @@ def _steps6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: in
             *recs, *ens)
 
 
+def _steps_abl6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int, nn: int,
+                src_x, src_z, dt: float, src_v, mt, rcv_x, rcv_z, damp, c11, c13, c15, c33, c35, c55, bx, bz,
+                fdc, txx, tzz, txz, vx, vz):
+    """``src_v.shape[-1]`` steps of step_forward_ABL_6order (elastic_kernels.py:781-912): the
+    damping step of :585-777 with three-term operators, the window [nn, Z-nn-1] and the
+    free-surface rows at h = nn + 1, kept in the state."""
+    S, Z, X = vx.shape
+    nt = src_v.shape[-1]
+    R = rcv_x.shape[0]
+    txx, tzz, txz, vx, vz = txx.clone(), tzz.clone(), txz.clone(), vx.clone(), vz.clone()
+    recs = [torch.zeros((S, nt, R), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    ens = [torch.zeros((nz, nx), dtype=vx.dtype, device=vx.device) for _ in range(5)]
+    I, J = slice(nn, Z - nn), slice(nn, X - nn)
+    c0, c1, c2 = fdc[0], fdc[1], fdc[2]
+
+    def sh(a, dz_, dx_):
+        return a[:, nn + dz_:Z - nn + dz_, nn + dx_:X - nn + dx_]
+
+    def dxf(a):
+        return c0 * (sh(a, 0, 1) - sh(a, 0, 0)) + c1 * (sh(a, 0, 2) - sh(a, 0, -1)) + c2 * (sh(a, 0, 3) - sh(a, 0, -2))
+
+    def dzf(a):
+        return c0 * (sh(a, 1, 0) - sh(a, 0, 0)) + c1 * (sh(a, 2, 0) - sh(a, -1, 0)) + c2 * (sh(a, 3, 0) - sh(a, -2, 0))
+
+    def dxb(a):
+        return c0 * (sh(a, 0, 0) - sh(a, 0, -1)) + c1 * (sh(a, 0, 1) - sh(a, 0, -2)) + c2 * (sh(a, 0, 2) - sh(a, 0, -3))
+
+    def dzb(a):
+        return c0 * (sh(a, 0, 0) - sh(a, -1, 0)) + c1 * (sh(a, 1, 0) - sh(a, -2, 0)) + c2 * (sh(a, 2, 0) - sh(a, -3, 0))
+    shots = torch.arange(S, device=vx.device)
+    scaling = -1.0 / 3
+    h = nn + 1
+    row0 = nn if free_surface else nn + nabc
+    C11, C13, C15, C33, C35, C55 = (a[I, J] for a in (c11, c13, c15, c33, c35, c55))
+    BX, BZ = bx[I, J], bz[I, J]
+    for t in range(nt):
+        dxb_vx, dxb_vz, dzb_vx, dzb_vz = dxb(vx), dxb(vz), dzb(vx), dzb(vz)
+        dxf_vx, dzf_vx, dxf_vz, dzf_vz = dxf(vx), dzf(vx), dxf(vz), dzf(vz)
+        txx[:, I, J] = txx[:, I, J] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dx + (C15 * dzb_vx + C13 * dzb_vz) / dz)
+        tzz[:, I, J] = tzz[:, I, J] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dx + (C35 * dzb_vx + C33 * dzb_vz) / dz)
+        txz[:, I, J] = txz[:, I, J] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dx + (C55 * dzf_vx + C35 * dzf_vz) / dz)
+        sv = src_v[shots, t]
+        txx[shots, src_z, src_x] += scaling * mt[:, 0, 0] * sv
+        tzz[shots, src_z, src_x] += scaling * mt[:, 2, 2] * sv
+        txz[shots, src_z, src_x] += scaling * mt[:, 0, 2] * sv
+        if free_surface:
+            tzz[:, h - 1, :] = 0
+            tzz[:, h - 2, :] = -tzz[:, h, :]
+            txz[:, h - 2, :] = -txz[:, h - 1, :]
+            txz[:, h - 3, :] = -txz[:, h, :]
+        dxf_txx, dzb_txz, dxb_txz, dzf_tzz = dxf(txx), dzb(txz), dxb(txz), dzf(tzz)
+        vx[:, I, J] += dt * BX * (dxf_txx / dx + dzb_txz / dz)
+        vz[:, I, J] += dt * BZ * (dxb_txz / dx + dzf_tzz / dz)
+        if free_surface:
+            J1 = slice(nn + 1, X - nn + 1)
+            vz[:, h - 2, J] = vz[:, h - 1, J]
+            vx[:, h - 2, J] = vz[:, h - 2, J1] - vz[:, h - 2, J] + vz[:, h - 1, J1] - vz[:, h - 1, J] + vx[:, h, J]
+            vz[:, h - 3, J] = vz[:, h - 2, J]
+        vx = vx * damp
+        vz = vz * damp
+        for k, f in enumerate((txx, tzz, txz, vx, vz)):
+            recs[k][:, t, :] = f[:, rcv_z, rcv_x]
+            ens[k] = torch.sum(f * f, dim=0)[row0:row0 + nz, nabc:nabc + nx].detach()
+    return (txx, tzz, txz, vx, vz, *recs, *ens)
+
+
 def padded_fields(bx, bz, CC, bcx, bcz, nabc: int, free_surface: bool, fd_order: int = 4):
     """Pads the coefficient fields the PML kernel reads (elastic_kernels.py:935-953)."""
     fso = fd_order // 2
@@ -367,11 +435,11 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    fd_order: int = 4, n_segments: int = 1,
                    device=torch.device("cpu"), dtype=torch.float32) -> Dict[str, torch.Tensor]:
     """Elastic forward modelling, elastic_kernels.py:917-1037 semantics: the split-field PML
-    branch for abc_type 'pml' (any case, :948, :976; order 4, or the three-term step for orders
-    6, 8, 10), the damping (ABL) branch otherwise (order 4)."""
+    branch for abc_type 'pml' (any case, :948, :976), the damping (ABL) branch otherwise; order 4,
+    or the three-term steps (:426-584, :781-912) for orders 6, 8, 10."""
     pml = abc_type.lower() == "pml"
-    if fd_order != 4 and not (pml and fd_order in (6, 8, 10)):
-        raise NotImplementedError("oracle covers the 4th-order branches and the PML branch of orders 6-10")
+    if fd_order not in (4, 6, 8, 10):
+        raise NotImplementedError("oracle covers fd_order 4, 6, 8 and 10")
     fso = fd_order // 2
     X = nx + 2 * nabc
     Z = nz + (nabc + fso if free_surface else 2 * nabc + fso)
@@ -391,6 +459,10 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
         if pml and fd_order != 4:
             out = checkpoint(_steps6, free_surface, nx, nz, dx, dz, nabc, fso, sx, sz, dt, chunk, MT, rx, rz,
                              f["bcx"], f["bcz"], f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
+                             f["bx"], f["bz"], fdc, *state, use_reentrant=True)
+        elif fd_order != 4:
+            out = checkpoint(_steps_abl6, free_surface, nx, nz, dx, dz, nabc, fso, sx, sz, dt, chunk, MT, rx, rz,
+                             dmp, f["c11"], f["c13"], f["c15"], f["c33"], f["c35"], f["c55"],
                              f["bx"], f["bz"], fdc, *state, use_reentrant=True)
         elif pml:
             out = checkpoint(_steps_tuple, free_surface, nx, nz, dx, dz, nabc, sx, sz, dt, chunk, MT, rx, rz,

@@ -18,8 +18,8 @@ the inputs and outputs of:
   C55, bx, bz and the wavelet, isotropic with free surface and VTI without,
   segments 1 and 3, for the split-field PML branch and for the damping (ABL)
   branch with Cerjan (free surface) and sin^2 (no free surface) tapers; the
-  three-term PML scheme of fd_order 6, 8 and 10 (step_forward_PML_6order,
-  :426-584); DiffCoef(2, 's'), DiffCoef(3, 's') and DiffCoef(4, 's') (:20-58);
+  three-term schemes of fd_order 6, 8 and 10 (step_forward_PML_6order,
+  :426-584, and step_forward_ABL_6order, :781-912); DiffCoef(2, 's'), DiffCoef(3, 's') and DiffCoef(4, 's') (:20-58);
 * ADFWI/propagator/boundary_condition.py bc_pml, bc_pml_xz, bc_sincos,
   bc_gerjan (:15-120).
 
@@ -182,6 +182,8 @@ def main():
         "elastic_fd6_vti_nofs": lambda: elastic_case(ek, bc, False, True, 26, order=6),
         "elastic_fd8_iso_fs": lambda: elastic_case(ek, bc, True, False, 27, order=8),
         "elastic_fd10_vti_nofs": lambda: elastic_case(ek, bc, False, True, 28, order=10),
+        "elastic_fd6_abl_gerjan_fs": lambda: elastic_case(ek, bc, True, False, 29, abc="gerjan", order=6),
+        "elastic_fd8_abl_sincos_nofs": lambda: elastic_case(ek, bc, False, True, 30, abc="sincos", order=8),
         "boundary": lambda: boundary_case(bc),
         "diffcoef": lambda: dict(fdc=ek.DiffCoef(2, "s").numpy()),
         "diffcoef6": lambda: dict(fdc6=ek.DiffCoef(3, "s").numpy(), fdc8=ek.DiffCoef(4, "s").numpy(),

@@ -23,7 +23,8 @@ def test_acoustic_oracle_matches_reference_vectors(name, seg):
 
 @pytest.mark.parametrize("name", ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs",
                                   "elastic_abl_sincos_nofs", "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs",
-                                  "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs"])
+                                  "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs", "elastic_fd6_abl_gerjan_fs",
+                                  "elastic_fd8_abl_sincos_nofs"])
 @pytest.mark.parametrize("seg", [1, 3])
 def test_elastic_oracle_matches_reference_vectors(name, seg):
     g, m = elastic_inputs(name)
