@@ -864,7 +864,7 @@ __global__ void el_gather5(ElDev d, const float *__restrict__ st, float *__restr
   r_vz[ro] = st[VZ * f + o];
 }
 
-// record adjoints of step t into the five-field adjoint state (tzz on the free-surface row 2
+// record adjoints of step t into the five-field adjoint state (tzz on the free-surface row NN
 // is identically zero there, so it receives nothing)
 __global__ void el_inject5(ElDev d, float *__restrict__ lam, const float *__restrict__ g_txx,
                            const float *__restrict__ g_tzz, const float *__restrict__ g_txz,
@@ -877,7 +877,7 @@ __global__ void el_inject5(ElDev d, float *__restrict__ lam, const float *__rest
   const long long ro = ((long long)s * d.nt + t) * d.nrec + r;
   const float *gs[5] = {g_txx, g_tzz, g_txz, g_vx, g_vz};
   for (int q = 0; q < 5; ++q) {
-    if (!gs[q] || (q == TZZ && d.fsurf && z == 2)) continue;
+    if (!gs[q] || (q == TZZ && d.fsurf && z == d.nn)) continue;
     atomicAdd(lam + q * d.field + o, gs[q][ro]);
   }
 }
@@ -1685,6 +1685,424 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
     }
 }
 
+// Three-term damping scheme (reference: step_forward_ABL_6order, elastic_kernels.py:781-912):
+// the stages of el_abl_fwd with the three-term differences, the window [NN, Z-NN-1] x
+// [NN, X-NN-1] and the free-surface rows kept in the state: tzz[NN] = 0, tzz[NN-1] = -tzz[NN+1],
+// txz[NN-1] = -txz[NN], txz[NN-2] = -txz[NN+1]; on the columns J, from the new undamped rows NN
+// and NN+1, vz[NN-1] = vz[NN], vx[NN-1] = vz[NN-1,j+1] - vz[NN-1,j] + vz[NN,j+1] - vz[NN,j] +
+// vx[NN+1,j], vz[NN-2] = vz[NN-1]; then vx, vz *= damp.  Lanes and rows march as in
+// el3_fwd_step; the first strip also writes the free-surface rows NN-2 and NN-1.
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el3_abl_fwd(ElDev d, const float *__restrict__ in, float *__restrict__ out,
+                                                   int t) {
+  int z0, x0;
+  if (!strip_of3(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nn = d.nn;
+  const int c = x0 - EHL3 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const int zw1 = d.Z - nn - 1, xw1 = d.X - nn - 1;
+  const bool out_col = lane >= EHL3 && lane < EHL3 + ESX3 && c <= xw1;
+  const bool cwin = c >= nn && c <= xw1;
+  const bool jj1 = c + 1 >= nn && c + 1 <= xw1;
+  const int zmax = d.Z - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rIn = rsrc(in, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rOut = rsrc(out, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const rsrc_t rD = rsrc(d.damp, planeb);
+  const Fd3 D{d.c0, d.c1, d.c2};
+  const float dt = d.dt, dx = d.dx, dz = d.dz;
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const int nk = end_row - z0 + 6;
+  const bool first = d.fsurf && z0 == nn;         // the strip that writes the free-surface rows
+
+  const int s_first = (blockIdx.y * EWPB + wv) * d.spw;
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = s_first + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float wsrc = d.src[(long long)s * d.nt + t];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    // velocities (state before the step) rows k-3..k+3; new stresses txz/tzz rows k-6..k, txx k-3..k
+    float vx[7] = {}, vz[7] = {}, xz[7] = {}, zz[7] = {}, xx[4] = {};
+    float nvzh = 0.f;                            // new vz of row NN before damping (free surface)
+    for (int rr = z0 - 6; rr <= z0 - 1; ++rr) {
+      const unsigned o = rb(rr);
+      const bool ok = rr >= 0;
+      const float a = bld(rIn, vc, F(VX) + o), b = bld(rIn, vc, F(VZ) + o);
+      push(vx, ok ? a : 0.f);
+      push(vz, ok ? b : 0.f);
+    }
+    // loaded one iteration ahead: velocities of row k+3, stresses and moduli of row k
+    struct Row {
+      float v[2], s[3], cm[ANISO ? 6 : 4];
+    };
+    auto load = [&](int k, Row &R) {
+      const unsigned o3 = rb(k + 3), o0 = rb(k);
+      R.v[0] = bld(rIn, vc, F(VX) + o3); R.v[1] = bld(rIn, vc, F(VZ) + o3);
+      R.s[0] = bld(rIn, vc, F(TXX) + o0); R.s[1] = bld(rIn, vc, F(TZZ) + o0); R.s[2] = bld(rIn, vc, F(TXZ) + o0);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o0); R.cm[1] = bld(rC, vc, C13 * planeb + o0);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o0); R.cm[3] = bld(rC, vc, C55 * planeb + o0);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
+      }
+    };
+    Row cur;
+    load(z0 - 3, cur);
+#pragma unroll 1
+    for (int kk = 0; kk < nk; ++kk) {
+      const int k = z0 - 3 + kk;
+      // buoyancies and taper of row k-3, consumed at the end of the iteration
+      float bxr = 0.f, bzr = 0.f, dmr = 0.f;
+      {
+        const unsigned om = rb(k - 3);
+        bxr = bld(rC, vc, BX * planeb + om); bzr = bld(rC, vc, BZ * planeb + om);
+        dmr = bld(rD, vc, om);
+      }
+      Row nxt;
+      load(k + 1, nxt);
+      {
+        const bool ok = k + 3 <= zmax;
+        push(vx, ok ? cur.v[0] : 0.f);
+        push(vz, ok ? cur.v[1] : 0.f);
+      }
+      // 1. stress of row k (velocity row k = entry 3)
+      float sxx = cur.s[0], szz = cur.s[1], sxz = cur.s[2];
+      const float dxb_vx = D.dxb(vx[3], lane), dxf_vz = D.dxf(vz[3], lane);
+      float dxb_vz = 0.f, dxf_vx = 0.f;
+      if constexpr (ANISO) {
+        dxb_vz = D.dxb(vz[3], lane);
+        dxf_vx = D.dxf(vx[3], lane);
+      }
+      if (k >= nn && k <= zw1 && cwin) {
+        const float dzb_vz = D.dzb(vz, 3), dzf_vx = D.dzf(vx, 3);
+        const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
+        if constexpr (ANISO) {
+          const float dzb_vx = D.dzb(vx, 3), dzf_vz = D.dzf(vz, 3);
+          const float C15v = cur.cm[4], C35v = cur.cm[5];
+          sxx = sxx + dt * ((C11v * dxb_vx + C15v * dxb_vz) / dx + (C15v * dzb_vx + C13v * dzb_vz) / dz);
+          szz = szz + dt * ((C13v * dxb_vx + C35v * dxb_vz) / dx + (C35v * dzb_vx + C33v * dzb_vz) / dz);
+          sxz = sxz + dt * ((C15v * dxf_vx + C55v * dxf_vz) / dx + (C55v * dzf_vx + C35v * dzf_vz) / dz);
+        } else {
+          sxx = sxx + dt * ((C11v * dxb_vx) / dx + (C13v * dzb_vz) / dz);
+          szz = szz + dt * ((C13v * dxb_vx) / dx + (C33v * dzb_vz) / dz);
+          sxz = sxz + dt * ((C55v * dxf_vz) / dx + (C55v * dzf_vx) / dz);
+        }
+      }
+      if (k == zs && c == xs) {
+        sxx = sxx + mxx * wsrc;
+        szz = szz + mzz * wsrc;
+        sxz = sxz + mxz * wsrc;
+      }
+      if (d.fsurf && k == nn) szz = 0.0f;        // 2. tzz[NN] = 0
+      if (out_col && k >= z0 && k < end_row) {
+        const unsigned o = rb(k);
+        bst(sxx, rOut, vs, F(TXX) + o); bst(szz, rOut, vs, F(TZZ) + o); bst(sxz, rOut, vs, F(TXZ) + o);
+      }
+      push(xx, sxx);
+      push(xz, sxz);
+      push(zz, szz);
+      if (d.fsurf && k == nn + 1) {              // entries 3..6 = rows NN-2..NN+1
+        zz[4] = -zz[6];
+        xz[4] = -xz[5];
+        xz[3] = -xz[6];
+        if (out_col) {
+          bst(zz[4], rOut, vs, F(TZZ) + rb(nn - 1));
+          bst(xz[4], rOut, vs, F(TXZ) + rb(nn - 1));
+          bst(xz[3], rOut, vs, F(TXZ) + rb(nn - 2));
+        }
+      }
+      // 3. velocity of row r = k-3 (old values: window entry 0)
+      const int r = k - 3;
+      const float dxf_txx = D.dxf(xx[0], lane), dxb_txz = D.dxb(xz[3], lane);
+      float nvx = 0.0f, nvz = 0.0f;
+      if (r >= nn && r <= zw1 && cwin) {
+        const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
+        nvx = vx[0] + (dt * bxr) * (dxf_txx / dx + dzb_txz / dz);
+        nvz = vz[0] + (dt * bzr) * (dxb_txz / dx + dzf_tzz / dz);
+      }
+      if (d.src_fz && r == zs && c == xs) nvz = nvz + d.src_fz[s] * wsrc;   // vertical force
+      if (out_col && r >= z0 && r < end_row) {
+        const unsigned o = rb(r);
+        bst(nvx * dmr, rOut, vs, F(VX) + o);
+        bst(nvz * dmr, rOut, vs, F(VZ) + o);
+      }
+      // 4. free-surface velocity rows from the new rows NN and NN+1
+      if (first && r == nn) nvzh = nvz;
+      if (first && r == nn + 1) {
+        const float vzn = shfl(nvzh, lane + 1);
+        const float vz1m_n = jj1 ? vzn : 0.0f;
+        const float vz1m = cwin ? nvzh : 0.0f;
+        if (out_col) {
+          const float d1 = bld(rD, vc, rb(nn - 1)), d0 = bld(rD, vc, rb(nn - 2));
+          const float nvx1 = (((vz1m_n - vz1m) + vzn) - nvzh) + nvx;
+          bst(nvx1 * d1, rOut, vs, F(VX) + rb(nn - 1));
+          bst(vz1m * d1, rOut, vs, F(VZ) + rb(nn - 1));
+          bst(vz1m * d0, rOut, vs, F(VZ) + rb(nn - 2));
+        }
+      }
+      cur = nxt;
+    }
+  }
+}
+
+// Adjoint of el3_abl_fwd: the stages of el_abl_adj with the three-term transposes and the
+// free-surface rows NN-2 .. NN+1:
+//   5'. L2(v) = damp * L(v);
+//   4'. on J: Lvz[NN] += L2vz[NN-1] + L2vz[NN-2] - 2 L2vx[NN-1,j] + 2 L2vx[NN-1,j-1],
+//       Lvx[NN+1] += L2vx[NN-1]; the overwritten rows NN-2, NN-1 get zero;
+//   3'. e = L1(v) dt b / d feeds the stress adjoints; b gradients;
+//   2'. stress free surface transposed; the stored tzz[NN] = 0 is a constant, so L1(tzz)[NN] = 0;
+//   1'. L(s before) = L1(s); L(v before) = L1(v) + D^T(C dt/d L1(s)); C and source gradients.
+// The first strip also writes the velocity adjoints of the free-surface rows NN-2 and NN-1
+// (state in this scheme).  Strips of ETZA rows.
+template <bool ANISO>
+__global__ __launch_bounds__(ENT) void el3_abl_adj(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
+                                                   const float *__restrict__ fv, const float *__restrict__ fsn, int t,
+                                                   float *__restrict__ gacc, float *__restrict__ gsrc) {
+  constexpr int NGF = ANISO ? 8 : 6;
+  __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  int z0, x0;
+  if (!strip_of3(d, z0, x0)) return;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nn = d.nn;
+  const int c = x0 - EHL3 + lane;
+  const unsigned vc = 4u * (unsigned)min(max(c, 0), d.X - 1);
+  const unsigned vs = 4u * (unsigned)max(c, 0);
+  const int zw1 = d.Z - nn - 1, xw1 = d.X - nn - 1;
+  const bool out_col = lane >= EHL3 && lane < EHL3 + ESX3 && c <= xw1;
+  const bool cwin = c >= nn && c <= xw1;
+  const bool jm1 = c - 1 >= nn && c - 1 <= xw1;
+  const int zmax = d.Z - 1;
+  const unsigned rowb = 4u * (unsigned)d.pitch, planeb = 4u * (unsigned)d.plane, fieldb = 4u * (unsigned)d.field;
+  auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
+  const rsrc_t rLi = rsrc(lin, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rLo = rsrc(lout, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rFv = rsrc(fv, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD5 * fieldb);
+  const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  const rsrc_t rD = rsrc(d.damp, planeb);
+  const Fd3 D{d.c0, d.c1, d.c2};
+  const float dt = d.dt, dx = d.dx, dz = d.dz;
+  const float dtdx = d.dt_dx, dtdz = d.dt_dz;
+  const int end_row = min(z0 + d.tz, zw1 + 1);
+  const int nk = end_row - z0 + 6;
+  const bool first = d.fsurf && z0 == nn;
+  const int grp = blockIdx.y * EWPB + wv;
+  float *sl = slab + wv * (NGF * ETZA * WAVE) + lane;
+  for (int q = 0; q < NGF * ETZA; ++q) sl[q * WAVE] = 0.0f;
+  auto acc = [&](int f, int row, float v) { sl[(f * ETZA + (row - z0)) * WAVE] += v; };
+
+  for (int si = 0; si < d.spw; ++si) {
+    const int s = grp * d.spw + si;
+    if (s >= d.S) break;
+    const unsigned so = (unsigned)s * planeb;
+    auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
+    const int zs = d.src_z[s], xs = d.src_x[s];
+    const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    float gmt = 0.0f;   // wavelet gradient of the moment-tensor injection, when a force term follows
+    // forward velocities before the step rows k-3..k+3; forward stresses after the step
+    // txz/tzz rows k-6..k, txx rows k-3..k; L1(v) rows k-3..k+3
+    float vx[7] = {}, vz[7] = {}, xz[7] = {}, zz[7] = {}, xx[4] = {}, ux[7] = {}, uz[7] = {};
+    float e1[4] = {}, e3[4] = {}, e2[7] = {}, e4[7] = {};
+    float p1[4] = {}, q3[4] = {}, p4[7] = {}, q2[7] = {};
+    float p3[ANISO ? 4 : 1] = {}, q1[ANISO ? 4 : 1] = {}, p2[ANISO ? 7 : 1] = {}, q4[ANISO ? 7 : 1] = {};
+    float ltzz1 = 0.f, ltxz0 = 0.f, ltxz1 = 0.f;     // stress adjoints of rows NN-2, NN-1
+    float wz0 = 0.f, wz1 = 0.f, wx1 = 0.f;           // damped velocity adjoints of rows NN-2, NN-1
+
+    struct VRow {
+      float fv[2], lv[2], b[2], dm;
+    };
+    auto vload = [&](int rr, VRow &R) {
+      const unsigned o = rb(rr);
+      R.fv[0] = bld(rFv, vc, F(VX) + o); R.fv[1] = bld(rFv, vc, F(VZ) + o);
+      R.lv[0] = bld(rLi, vc, F(VX) + o); R.lv[1] = bld(rLi, vc, F(VZ) + o);
+      R.b[0] = bld(rC, vc, BX * planeb + o); R.b[1] = bld(rC, vc, BZ * planeb + o);
+      R.dm = bld(rD, vc, o);
+    };
+    auto vpush = [&](int rr, const VRow &R) {
+      const bool in_grid = rr >= 0 && rr <= zmax;
+      float lx = in_grid ? R.lv[0] * R.dm : 0.f, lz = in_grid ? R.lv[1] * R.dm : 0.f;   // 5'
+      if (d.fsurf) {                                                                     // 4'
+        if (rr == nn - 2) { wz0 = lz; if (cwin) lz = 0.f; }
+        if (rr == nn - 1) { wz1 = lz; wx1 = lx; if (cwin) { lz = 0.f; lx = 0.f; } }
+        if (rr == nn) {
+          const float wx1m = shfl(wx1, lane - 1);
+          if (cwin) lz = lz + (((wz1 + wz0) - 2.0f * wx1) + (jm1 ? 2.0f * wx1m : 0.0f));
+        }
+        if (rr == nn + 1 && cwin) lx = lx + wx1;
+      }
+      float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
+      if (rr >= nn && rr <= zw1 && cwin) {
+        ea = lx * ((dt * R.b[0]) / dx);
+        eb = lx * ((dt * R.b[0]) / dz);
+        ec = lz * ((dt * R.b[1]) / dx);
+        ed = lz * ((dt * R.b[1]) / dz);
+      }
+      push(vx, in_grid ? R.fv[0] : 0.f);
+      push(vz, in_grid ? R.fv[1] : 0.f);
+      push(ux, lx);
+      push(uz, lz);
+      push(e1, ea);
+      push(e2, eb);
+      push(e3, ec);
+      push(e4, ed);
+    };
+    for (int rr = z0 - 6; rr <= z0 - 1; ++rr) {
+      VRow R;
+      vload(rr, R);
+      vpush(rr, R);
+    }
+    struct SRow {
+      float ls[3], cm[ANISO ? 6 : 4], fs[3];
+    };
+    auto sload = [&](int k, SRow &R) {
+      const unsigned o = rb(k);
+      R.ls[0] = bld(rLi, vc, F(TXX) + o); R.ls[1] = bld(rLi, vc, F(TZZ) + o); R.ls[2] = bld(rLi, vc, F(TXZ) + o);
+      R.cm[0] = bld(rC, vc, C11 * planeb + o); R.cm[1] = bld(rC, vc, C13 * planeb + o);
+      R.cm[2] = bld(rC, vc, C33 * planeb + o); R.cm[3] = bld(rC, vc, C55 * planeb + o);
+      if constexpr (ANISO) {
+        R.cm[4] = bld(rC, vc, C15 * planeb + o); R.cm[5] = bld(rC, vc, C35 * planeb + o);
+      }
+      R.fs[0] = bld(rFs, vc, F(TXX) + o); R.fs[1] = bld(rFs, vc, F(TZZ) + o); R.fs[2] = bld(rFs, vc, F(TXZ) + o);
+    };
+    VRow vcur;
+    SRow scur;
+    vload(z0, vcur);
+    sload(z0 - 3, scur);
+#pragma unroll 1
+    for (int kk = 0; kk < nk; ++kk) {
+      const int k = z0 - 3 + kk;
+      VRow vnxt;
+      SRow snxt;
+      vload(k + 4, vnxt);
+      sload(k + 1, snxt);
+      vpush(k + 3, vcur);
+      // 3'. stress adjoints of row k from e (e1/e3 row k = entry 0; e2/e4 row k = entry 3)
+      float lt_xx = D.dxfT(e1[0], lane);
+      float lt_xz = D.dzbT(e2, 3) + D.dxbT(e3[0], lane);
+      float lt_zz = D.dzfT(e4, 3);
+      // 2'. stress free surface transposed
+      if (d.fsurf) {
+        if (k == nn - 2) ltxz0 = lt_xz;
+        if (k == nn - 1) { ltzz1 = lt_zz; ltxz1 = lt_xz; }
+        if (k == nn) lt_xz = lt_xz - ltxz1;
+        if (k == nn + 1) { lt_zz = lt_zz - ltzz1; lt_xz = lt_xz - ltxz0; }
+      }
+      const float dxb_vx = D.dxb(vx[3], lane), dxf_vz = D.dxf(vz[3], lane);
+      float dxb_vz = 0.f, dxf_vx = 0.f;
+      if constexpr (ANISO) {
+        dxb_vz = D.dxb(vz[3], lane);
+        dxf_vx = D.dxf(vx[3], lane);
+      }
+      // 1'. stress row k
+      float P1 = 0.f, Q2 = 0.f, Q3 = 0.f, P4 = 0.f, P2 = 0.f, P3 = 0.f, Q1 = 0.f, Q4 = 0.f;
+      if (k >= nn && k <= zw1 && cwin) {
+        const unsigned o = rb(k);
+        const float lxx = scur.ls[0] + lt_xx;
+        const float lzz = (d.fsurf && k == nn) ? 0.0f : scur.ls[1] + lt_zz;
+        const float lxz = scur.ls[2] + lt_xz;
+        const bool own_k = out_col && k >= z0 && k < end_row;
+        if (own_k && k == zs && c == xs && gsrc) {
+          const float gm = (mxx * lxx + mzz * lzz) + mxz * lxz;
+          if (d.src_fz) gmt = gm;        // the force term of row zs follows three rows later
+          else gsrc[(long long)s * d.nt + t] = gm;
+        }
+        if (own_k) {
+          bst(lxx, rLo, vs, F(TXX) + o); bst(lzz, rLo, vs, F(TZZ) + o); bst(lxz, rLo, vs, F(TXZ) + o);
+        }
+        const float axx_x = dtdx * lxx, axx_z = dtdz * lxx;
+        const float azz_x = dtdx * lzz, azz_z = dtdz * lzz;
+        const float axz_x = dtdx * lxz, axz_z = dtdz * lxz;
+        const float C11v = scur.cm[0], C13v = scur.cm[1], C33v = scur.cm[2], C55v = scur.cm[3];
+        P1 = C11v * axx_x + C13v * azz_x;
+        Q2 = C13v * axx_z + C33v * azz_z;
+        Q3 = C55v * axz_x;
+        P4 = C55v * axz_z;
+        if constexpr (ANISO) {
+          const float C15v = scur.cm[4], C35v = scur.cm[5];
+          P2 = C15v * axx_z + C35v * azz_z;
+          P3 = C15v * axz_x;
+          Q1 = C15v * axx_x + C35v * azz_x;
+          Q4 = C35v * axz_z;
+        }
+        if (own_k) {
+          const float dzb_vz = D.dzb(vz, 3), dzf_vx = D.dzf(vx, 3);
+          if constexpr (ANISO) {
+            const float dzb_vx = D.dzb(vx, 3), dzf_vz = D.dzf(vz, 3);
+            acc(6, k, axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx);
+            acc(7, k, azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz);
+          }
+          acc(0, k, axx_x * dxb_vx);
+          acc(1, k, axx_z * dzb_vz + azz_x * dxb_vx);
+          acc(2, k, azz_z * dzb_vz);
+          acc(3, k, axz_x * dxf_vz + axz_z * dzf_vx);
+        }
+      }
+      push(p1, P1);
+      push(q3, Q3);
+      push(p4, P4);
+      push(q2, Q2);
+      if constexpr (ANISO) {
+        push(p3, P3);
+        push(q1, Q1);
+        push(p2, P2);
+        push(q4, Q4);
+      }
+      // forward stresses of row k after the step (stored with the free surface applied)
+      {
+        const bool in_grid = k >= 0 && k <= zmax;
+        push(xx, in_grid ? scur.fs[0] : 0.f);
+        push(zz, in_grid ? scur.fs[1] : 0.f);
+        push(xz, in_grid ? scur.fs[2] : 0.f);
+      }
+      // velocity adjoints of row r = k-3: L1(v) + D^T(products)
+      const int r = k - 3;
+      float lvx = D.dxbT(p1[0], lane) + D.dzfT(p4, 3);
+      float lvz = D.dzbT(q2, 3) + D.dxfT(q3[0], lane);
+      if constexpr (ANISO) {
+        lvx = lvx + (D.dzbT(p2, 3) + D.dxfT(p3[0], lane));
+        lvz = lvz + (D.dxbT(q1[0], lane) + D.dzfT(q4, 3));
+      }
+      const float dxf_txx = D.dxf(xx[0], lane), dxb_txz = D.dxb(xz[3], lane);
+      const bool own_r = out_col && ((r >= z0 && r < end_row) || (first && r >= nn - 2 && r < nn));
+      if (own_r) {
+        const unsigned o = rb(r);
+        if (d.src_fz && gsrc && r == zs && c == xs)   // vertical force entered vz before the taper
+          gsrc[(long long)s * d.nt + t] = gmt + d.src_fz[s] * uz[0];
+        bst(ux[0] + lvx, rLo, vs, F(VX) + o);
+        bst(uz[0] + lvz, rLo, vs, F(VZ) + o);
+        if (r >= z0) {
+          const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
+          acc(4, r, ux[0] * (dt * (dxf_txx / dx + dzb_txz / dz)));
+          acc(5, r, uz[0] * (dt * (dxb_txz / dx + dzf_tzz / dz)));
+        }
+      }
+      vcur = vnxt;
+      scur = snxt;
+    }
+  }
+  __syncthreads();
+  if (!out_col) return;
+  const long long planef = (long long)d.Z * d.pitch;
+  float *g = gacc + (long long)blockIdx.y * NCOEF * planef + c;
+  const float *sb = slab + lane;
+  for (int f = wv; f < NGF; f += EWPB)
+    for (int row = z0; row < end_row; ++row) {
+      const int o = (f * ETZA + (row - z0)) * WAVE;
+      const float v = (sb[o] + sb[NGF * ETZA * WAVE + o]) + (sb[2 * NGF * ETZA * WAVE + o] + sb[3 * NGF * ETZA * WAVE + o]);
+      float *p = g + (long long)f * planef + (long long)row * d.pitch;
+      *p = *p + v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1698,7 +2116,6 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
   ADFWI_CHECK_ARG(d->nshots >= 1, "nshots must be >= 1 (got %d)", d->nshots);
   const int nn = d->nn == 0 ? 2 : d->nn;
   ADFWI_CHECK_ARG(nn >= 2 && nn <= 5, "stencil half-order nn %d outside 2..5 (fd_order 4..10)", d->nn);
-  ADFWI_CHECK_ARG(nn == 2 || !d->damping, "the damping scheme is implemented at 4th order only (nn = 2, got %d)", nn);
   ADFWI_CHECK_ARG(nn == 2 || d->tile_z == 0 || d->tile_z >= 8,
                   "tile_z %d: the three-term scheme needs strips of at least 8 rows", d->tile_z);
   ADFWI_CHECK_ARG(d->nz >= 2 * nn + 5 && d->nx >= 2 * nn + 5, "padded grid %d x %d too small (need >= %d x %d)",
@@ -1807,7 +2224,13 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
     } else {
       nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
     }
-    if (d->damping) {
+    if (d->damping && v.nn > 2) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el3_abl_fwd<true>, grid, block, 0, st, v, cur, nxt, t);
+      else
+        hipLaunchKernelGGL(el3_abl_fwd<false>, grid, block, 0, st, v, cur, nxt, t);
+      if ((rc = adfwi::check_launch("el3_abl_fwd"))) return rc;
+    } else if (d->damping) {
       if (d->aniso)
         hipLaunchKernelGGL(el_abl_fwd<true>, grid, block, 0, st, v, cur, nxt, t);
       else
@@ -1886,7 +2309,13 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
     const float *fv = i == 0 ? a->fwd0 : a->trace + (long long)(i - 1) * L.state_floats;
     const float *fsn = a->trace + (long long)i * L.state_floats;
     float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
-    if (d->damping) {
+    if (d->damping && v.nn > 2) {
+      if (d->aniso)
+        hipLaunchKernelGGL(el3_abl_adj<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      else
+        hipLaunchKernelGGL(el3_abl_adj<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      if ((rc = adfwi::check_launch("el3_abl_adj"))) return rc;
+    } else if (d->damping) {
       if (d->aniso)
         hipLaunchKernelGGL(el_abl_adj<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
       else

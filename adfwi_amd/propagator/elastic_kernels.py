@@ -267,9 +267,9 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
                    src_fz: Optional[torch.Tensor] = None) -> Dict[str, torch.Tensor]:
     """Elastic forward modelling with the reference's signature and result dictionary
     (elastic_kernels.py:917-1037): the split-field PML scheme for abc_type 'pml' (any case, as the
-    reference's driver tests it) at fd_order 4, or the three-term scheme of fd_order 6, 8 and 10
-    (step_forward_PML_6order, :426-584); the 4th-order multiplicative-damping scheme with ``damp``
-    otherwise.  ``n_segments`` keeps its meaning for the energy maps (summed over the last step of
+    reference's driver tests it), the multiplicative-damping scheme with ``damp`` otherwise; at
+    fd_order 4, or the three-term schemes of fd_order 6, 8 and 10 (step_forward_PML_6order,
+    :426-584, and step_forward_ABL_6order, :781-912).  ``n_segments`` keeps its meaning for the energy maps (summed over the last step of
     every time chunk); memory is governed by the sqrt(nt) wavefield checkpoints of the HIP
     adjoint.  ``src_fz`` (extension, no counterpart in the reference): per-shot vertical-force
     weights f; the shot's wavelet w then also enters vz at the source point right after the
@@ -278,9 +278,8 @@ def forward_kernel(nx: int, nz: int, dx: float, dz: float, nt: int, dt: float,
     if dtype != torch.float32:
         raise NotImplementedError("the HIP propagator computes in float32 (dtype=torch.float32)")
     use_pml = abc_type.lower() == "pml"
-    if fd_order != 4 and not (use_pml and fd_order in (6, 8, 10)):
-        raise NotImplementedError(f"the HIP elastic propagator implements fd_order 4, and 6 / 8 / 10 with the PML "
-                                  f"(got fd_order={fd_order}, abc_type={abc_type!r})")
+    if fd_order not in (4, 6, 8, 10):
+        raise NotImplementedError(f"the HIP elastic propagator implements fd_order 4, 6, 8 and 10 (got {fd_order})")
     if use_pml and (bcx is None or bcz is None):
         raise ValueError("the PML scheme needs the bcx / bcz damping profiles")
     if not use_pml and damp is None:

@@ -115,8 +115,7 @@ class ElasticPropagator(torch.nn.Module):
         CC = model.CC
         aniso = bool((CC[4] != 0).any() or (CC[13] != 0).any())
         geom = None
-        if (self.abc_type == "PML") == (self.abc_type.lower() == "pml") and (
-                fd_order == 4 or (fd_order in (6, 8, 10) and self.abc_type == "PML")):
+        if (self.abc_type == "PML") == (self.abc_type.lower() == "pml") and fd_order in (4, 6, 8, 10):
             # the reference's propagator builds bcx/bcz only for 'PML' exactly (elastic_propagator.py
             # :219) while its driver takes the PML branch for any case of 'pml' (elastic_kernels.py:948)
             geom = self._geometry(shots, aniso, damping=self.abc_type != "PML", fd_order=fd_order)

@@ -127,7 +127,8 @@ def _dropin(g, m, seg=1, want_grad=False, wavelet=None, fdc=None, **leaf_kw):
 
 
 ALL_CASES = ["elastic_iso_fs", "elastic_vti_nofs", "elastic_abl_gerjan_fs", "elastic_abl_sincos_nofs",
-             "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs", "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs"]
+             "elastic_fd6_iso_fs", "elastic_fd6_vti_nofs", "elastic_fd8_iso_fs", "elastic_fd10_vti_nofs",
+             "elastic_fd6_abl_gerjan_fs", "elastic_fd8_abl_sincos_nofs"]
 
 
 @pytest.mark.parametrize("name", ALL_CASES)
@@ -166,14 +167,16 @@ def test_dropin_forward_matches_reference_vectors(name, seg):
                                                   (True, False, 1, 4, "gerjan"), (False, True, 2, 8, "sincos"),
                                                   (True, True, 0, 0, "gerjan"),
                                                   (True, False, 1, 8, "PML6"), (False, True, 2, 16, "PML6"),
-                                                  (True, True, 0, 0, "PML8"), (False, False, 1, 8, "PML10")])
+                                                  (True, True, 0, 0, "PML8"), (False, False, 1, 8, "PML10"),
+                                                  (True, False, 1, 8, "gerjan6"), (False, True, 2, 16, "sincos8"),
+                                                  (True, True, 0, 0, "sincos10")])
 def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz, abc):
     """A grid of several 55-column (51 at orders 6-10) strips and row tiles, several shots per
     wavefront: strip edges, tile edges, PML corners, both absorbing schemes, the three-term
     scheme of orders 6, 8, 10 and the anisotropic C15/C35 terms against the oracle."""
-    fd = int(abc[3:]) if abc.startswith("PML") and len(abc) > 3 else 4
+    abc, fd = _split_abc(abc)
     g, m = elastic_random_case(44, 150, 8, 50, 9, fs, seed=11 + int(fs) + 2 * int(aniso), aniso=aniso,
-                               abc=abc[:3] if fd != 4 else abc, fd_order=fd)
+                               abc=abc, fd_order=fd)
     ref = _run_oracle(g, m)
     geom, coef, px, pz, src, scalars = _oracle_setup(g, m, shots_per_wave=spw, tile_z=tz)
     assert geom.layout.ntiles_x >= 3 and geom.layout.ntiles_z >= 2 or tz == 0
@@ -182,6 +185,12 @@ def test_multi_strip_grid_matches_oracle(fs, aniso, spw, tz, abc):
     for i, k in enumerate(ELASTIC_NAMES):
         got = res["rec"][i].cpu().numpy()
         assert np.array_equal(got, ref[k]), (k, rel_l2(got, ref[k]))
+
+
+def _split_abc(spec):
+    """'PML6' -> ('PML', 6), 'gerjan' -> ('gerjan', 4): absorbing scheme and FD order of a case."""
+    name = spec.rstrip("0123456789")
+    return name, int(spec[len(name):] or 4)
 
 
 def _loss(res, weights):
@@ -208,14 +217,15 @@ def test_dropin_gradients_match_reference_vectors(name, seg):
 @pytest.mark.parametrize("fs,aniso,abc", [(True, False, "PML"), (False, True, "PML"), (True, True, "PML"),
                                           (True, False, "gerjan"), (False, True, "sincos"), (True, True, "sincos"),
                                           (True, False, "PML6"), (False, True, "PML6"), (True, True, "PML8"),
-                                          (True, False, "PML10")])
+                                          (True, False, "PML10"), (True, False, "gerjan6"), (False, True, "sincos8"),
+                                          (True, True, "gerjan10")])
 def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
     """Several strips and row tiles, several shot groups, anisotropic C15/C35 gradients too, both
     absorbing schemes and the three-term scheme of orders 6, 8, 10, against the oracle's autograd
     gradients."""
-    fd = int(abc[3:]) if abc.startswith("PML") and len(abc) > 3 else 4
+    abc, fd = _split_abc(abc)
     g, m = elastic_random_case(44, 150, 8, 60, 9, fs, seed=21 + int(fs) + 2 * int(aniso), aniso=aniso,
-                               abc=abc[:3] if fd != 4 else abc, fd_order=fd)
+                               abc=abc, fd_order=fd)
     ref = _run_oracle(g, m, want_grad=True)
     res, leaves, w = _dropin(g, m, want_grad=True)
     _loss(res, g["weights"]).backward()
@@ -226,13 +236,12 @@ def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
     assert rel_l2(w.grad.cpu().numpy(), ref["grad_wavelet"]) <= 1e-4
 
 
-@pytest.mark.parametrize("abc", ["PML", "gerjan", "PML6"])
+@pytest.mark.parametrize("abc", ["PML", "gerjan", "PML6", "gerjan6"])
 def test_adjoint_dot_product_in_the_wavelet(abc):
     """Records are linear in the wavelet: <J dw, dd> = <dw, J^T dd> with J^T dd the wavelet
     gradient of the hand-written adjoint (free surface and anisotropy on, several tiles)."""
-    fd = 6 if abc == "PML6" else 4
-    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True, abc=abc[:3] if fd != 4 else abc,
-                               fd_order=fd)
+    abc, fd = _split_abc(abc)
+    g, m = elastic_random_case(40, 130, 8, 80, 5, True, seed=5, aniso=True, abc=abc, fd_order=fd)
     rng = np.random.default_rng(3)
     dw = rng.standard_normal(g["wavelet"].shape).astype(np.float32)
     dd = rng.standard_normal(g["weights"].shape).astype(np.float32)
@@ -288,13 +297,15 @@ def test_elastic_propagator_class_matches_reference_vectors(name):
         assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))
 
 
-def test_elastic_propagator_damping_scheme_matches_reference_vectors():
+@pytest.mark.parametrize("name", ["elastic_abl_gerjan_fs", "elastic_fd6_abl_gerjan_fs"])
+def test_elastic_propagator_damping_scheme_matches_reference_vectors(name):
     """ElasticPropagator with abc_type 'gerjan' (Cerjan taper from the model's alpha, the
-    damping scheme of the reference) reproduces the reference's records of that case."""
+    damping scheme of the reference) reproduces the reference's records of that case, at 4th
+    and 6th order."""
     from adfwi_amd.model import IsotropicElasticModel
     from adfwi_amd.propagator import ElasticPropagator
     from adfwi_amd.survey import Receiver, Source, Survey
-    g, m = elastic_inputs("elastic_abl_gerjan_fs")
+    g, m = elastic_inputs(name)
     nz, nx = m["nz"], m["nx"]
     src = Source(m["nt"], m["dt"], 60.0)
     for s_ in range(len(g["src_x"])):
@@ -308,7 +319,7 @@ def test_elastic_propagator_damping_scheme_matches_reference_vectors():
     prop = ElasticPropagator(model, Survey(src, rcv), device=DEV)
     assert np.allclose(prop.damp.cpu().numpy(), g["damp"].astype(np.float32))
     with torch.no_grad():
-        res = prop.forward()
+        res = prop.forward(fd_order=m["fd"])
     for k in ELASTIC_NAMES:
         got = res[k].cpu().numpy()
         assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))
