@@ -1162,6 +1162,7 @@ void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &i
   if (!imd && !imp) launch_pair_f<PF_REC>(w, g, b, st, io);
   else if (!rec && !imp) launch_pair_f<PF_IMGD>(w, g, b, st, io);
   else if (!rec) launch_pair_f<PF_IMGD | PF_IMGP>(w, g, b, st, io);
+  else if (!imp) launch_pair_f<PF_REC | PF_IMGD>(w, g, b, st, io);
   else launch_pair_f<PF_REC | PF_IMGD | PF_IMGP>(w, g, b, st, io);
 }
 

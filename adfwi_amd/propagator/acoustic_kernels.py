@@ -113,7 +113,8 @@ class AcousticGeometry:
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, nabc: int,
                  src_z: np.ndarray, src_x: np.ndarray, rcv_z: np.ndarray, rcv_x: np.ndarray,
-                 device, shots_per_block: int = 0, graph_steps: Optional[int] = None):
+                 device, shots_per_block: int = 0, graph_steps: Optional[int] = None,
+                 resident_images: Optional[int] = None):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
@@ -142,6 +143,11 @@ class AcousticGeometry:
         self.tile_z = int(os.environ.get("ADFWI_TILE_Z", "0"))
         # forward runs without checkpoints replay HIP graphs of this many steps (0: direct launches)
         self.graph_steps = int(os.environ.get("ADFWI_GRAPH_STEPS", "0") if graph_steps is None else graph_steps)
+        # checkpoint segments (the last ones) whose stencil images the forward pass keeps for the
+        # backward instead of recomputing them there: None = as many as the memory budget holds
+        # (resident_segments), 0 = none; ADFWI_RESIDENT_IMAGES sets it when the argument is None
+        env = os.environ.get("ADFWI_RESIDENT_IMAGES")
+        self.resident_images = (int(env) if env is not None and resident_images is None else resident_images)
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
@@ -204,13 +210,41 @@ class AcousticGeometry:
         return coef
 
 
+def resident_segments(geom: AcousticGeometry, k: int, need_a1: bool, need_a2: bool) -> int:
+    """How many of the last checkpoint segments keep their stencil (and pressure) images from the
+    forward pass, so that the backward skips their recomputation: at most geom.resident_images
+    (0 = none); with None, as many as fit in three quarters of the free device memory less 4 GiB
+    (ADFWI_IMAGE_BUDGET_BYTES overrides the budget).  What the backward allocates besides (two
+    adjoint states, one segment's recompute images, the gradient planes) stays well inside the
+    remaining quarter."""
+    planes = int(bool(need_a1)) + int(bool(need_a2))
+    nseg = -(-geom.nt // k)
+    cap = geom.resident_images
+    if planes == 0 or cap == 0:
+        return 0
+    seg_bytes = planes * k * geom.S * geom.Z * geom.pitch * 4
+    if cap is None:
+        budget = int(os.environ.get("ADFWI_IMAGE_BUDGET_BYTES", "0"))
+        if not budget:
+            free, _ = torch.cuda.mem_get_info(geom.device)
+            budget = max(0, free * 3 // 4 - (4 << 30))
+        cap = budget // seg_bytes
+    return int(max(0, min(cap, nseg)))
+
+
 def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *, record: bool = True,
-                energy_u_from: Optional[int] = None, ckpt_every: int = 0):
+                energy_u_from: Optional[int] = None, ckpt_every: int = 0, resident: int = 0,
+                need_a1: bool = True, need_a2: bool = False):
     """Runs all nt forward steps from a zero state.
 
     Returns dict(rec=[3 x (S, nt, R)] or None, energy=(3, Z, X) or None: sum over steps and shots of
     p^2, sum over steps >= energy_u_from of u^2, and sum over shots of w^2 after the last step;
-    ckpt=(n, 3, S, Z, pitch) or None).
+    ckpt=(n, 3, S, Z, pitch) or None, images={segment: (image, image_p)}).
+
+    resident > 0 (with ckpt_every): the last `resident` checkpoint segments are run with the
+    stencil image (need_a1) and the pressure image (need_a2) written to buffers of their own,
+    which run_backward then uses instead of recomputing those segments; their checkpoints are
+    not written.
     """
     lib = nat.load()
     dev = geom.device
@@ -218,9 +252,11 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
     L = geom.layout
     work = geom.state(2)
     ckpt = None
-    if ckpt_every:
-        nseg = -(-nt // ckpt_every)
-        ckpt = geom.state(max(nseg - 1, 1))
+    nseg = -(-nt // ckpt_every) if ckpt_every else 1
+    resident = int(resident) if ckpt_every and (need_a1 or need_a2) else 0
+    head = nt if not resident else (nseg - resident) * ckpt_every    # steps before the resident segments
+    if ckpt_every and head > 0:
+        ckpt = geom.state(max(nseg - 1, 1) if not resident else max(nseg - resident, 1))
     rec = None
     if record and R:
         rec = [torch.zeros((S, nt, R), device=dev, dtype=torch.float32) for _ in range(3)]
@@ -228,27 +264,44 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
     if energy_u_from is not None:
         energy = torch.zeros((L.ngroups, 2, Z, geom.pitch), device=dev, dtype=torch.float32)
     d = geom.desc(coef, src)
-    a = nat.AcousticFwdArgs()
-    a.state_in = work[0].data_ptr()          # zero initial state
-    a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
-    a.ckpt = None if ckpt is None else ckpt.data_ptr()
-    a.ckpt_every = int(ckpt_every)
-    a.t0, a.nsteps = 0, nt
-    a.energy_u_from = int(energy_u_from) if energy_u_from is not None else nt
-    if rec is not None:
-        a.rec_p, a.rec_u, a.rec_w = rec[0].data_ptr(), rec[1].data_ptr(), rec[2].data_ptr()
-    a.energy = None if energy is None else energy.data_ptr()
-    a.rcv_map = None if geom.rcv_map is None else geom.rcv_map.data_ptr()
-    a.fuse_steps = geom.fuse_steps
     fused = bool(geom.fuse_steps) and (rec is None or geom.rcv_map is not None)
-    pairs, singles = forward_launches(0, nt, nt, int(ckpt_every), fused)
-    TIMELINE.call("forward", "ac_fwd_pair" if fused else "ac_fwd_step", pairs + singles, dev, lambda: nat.check(
-        lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(a), nat.stream_handle(dev)),
-        "adfwi_acoustic_forward"), steps=nt)
-    out = {"rec": rec, "ckpt": ckpt, "energy": None}
+    images = {}
+
+    def call(state_in, t0, nsteps, ckpt_buf, img, img_p, kind):
+        a = nat.AcousticFwdArgs()
+        a.state_in = state_in
+        a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
+        a.ckpt = None if ckpt_buf is None else ckpt_buf.data_ptr()
+        a.ckpt_every = int(ckpt_every) if ckpt_buf is not None else 0
+        a.t0, a.nsteps = t0, nsteps
+        a.energy_u_from = int(energy_u_from) if energy_u_from is not None else nt
+        if rec is not None:
+            a.rec_p, a.rec_u, a.rec_w = rec[0].data_ptr(), rec[1].data_ptr(), rec[2].data_ptr()
+        a.energy = None if energy is None else energy.data_ptr()
+        a.image = None if img is None else img.data_ptr()
+        a.image_p = None if img_p is None else img_p.data_ptr()
+        a.rcv_map = None if geom.rcv_map is None else geom.rcv_map.data_ptr()
+        a.fuse_steps = geom.fuse_steps
+        pairs, singles = forward_launches(t0, nsteps, nt, int(a.ckpt_every), fused)
+        TIMELINE.call(kind, "ac_fwd_pair" if fused else "ac_fwd_step", pairs + singles, dev, lambda: nat.check(
+            lib.adfwi_acoustic_forward(ctypes.byref(d), ctypes.byref(a), nat.stream_handle(dev)),
+            "adfwi_acoustic_forward"), steps=nsteps)
+        return a.state_out
+
+    state = work[0].data_ptr()          # zero initial state
+    if head > 0:
+        state = call(state, 0, head, ckpt, None, None, "forward")
+    for seg in range(nseg - resident, nseg):
+        t_lo = seg * ckpt_every
+        n = min(nt, t_lo + ckpt_every) - t_lo
+        shape = (ckpt_every, S, Z, geom.pitch)
+        img = torch.zeros(shape, device=dev, dtype=torch.float32) if need_a1 else None
+        img_p = torch.zeros(shape, device=dev, dtype=torch.float32) if need_a2 else None
+        state = call(state, t_lo, n, None, img, img_p, "forward+images")
+        images[seg] = (img, img_p)
+    out = {"rec": rec, "ckpt": ckpt, "energy": None, "images": images}
     if energy is not None:
-        final = work[0] if a.state_out == work[0].data_ptr() else (
-            work[1] if a.state_out == work[1].data_ptr() else None)
+        final = work[0] if state == work[0].data_ptr() else (work[1] if state == work[1].data_ptr() else None)
         if final is None:   # last step wrote a checkpoint slot (only when nt is a multiple of k, never)
             raise RuntimeError("final state not in a work buffer")
         e = energy.sum(0)[:, :, :X]
@@ -259,10 +312,12 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
 
 def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, ckpt: Optional[torch.Tensor],
                  ckpt_every: int, g_rec, need_a1: bool, need_a2: bool, need_src: bool,
-                 group_planes: bool = False):
+                 group_planes: bool = False, images: Optional[dict] = None):
     """Hand-written adjoint over all steps with segment recomputation from the checkpoints.
 
     g_rec: tuple of three (S, nt, R) adjoint sources (or None each).
+    images: {segment: (image, image_p)} kept by run_forward(resident=...); those segments are not
+    recomputed, and each buffer is released once its segment is done.
     Returns (grad_a1 (Z, X) or None, grad_a2 or None, grad_src (S, nt) or None).
     group_planes: grad_a1 / grad_a2 are returned unsummed, (G, Z, X) with one plane per group of
     `geom.gradient_group` consecutive shots (their sum is the gradient of all shots).
@@ -275,8 +330,10 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
     nseg = -(-nt // k)
     rw = geom.state(2)    # recompute ping-pong
     lw = geom.state(2)    # adjoint ping-pong
-    image = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
-    image_p = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
+    images = {} if images is None else images
+    recompute_any = len(images) < nseg
+    image = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 and recompute_any else None
+    image_p = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 and recompute_any else None
     nacc = int(L.accum_floats) // (Z * geom.pitch)   # accumulator planes (finest shot grouping)
     ga1 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
     ga2 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
@@ -291,7 +348,12 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
         t_lo = seg * k
         t_hi = min(nt, (seg + 1) * k) - 1
         n = t_hi - t_lo + 1
-        if need_image:
+        kept = images.pop(seg, None)
+        if kept is not None:
+            seg_image, seg_image_p = kept
+        else:
+            seg_image, seg_image_p = image, image_p
+        if need_image and kept is None:
             if seg == 0:
                 rw[0].zero_()
                 sin = rw[0]
@@ -314,8 +376,8 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
         aa.lam_in = lam.data_ptr()
         aa.work[0], aa.work[1] = lw[0].data_ptr(), lw[1].data_ptr()
         aa.t_hi, aa.nsteps, aa.image_t0 = t_hi, n, t_lo
-        aa.image = None if image is None else image.data_ptr()
-        aa.image_p = None if image_p is None else image_p.data_ptr()
+        aa.image = None if seg_image is None or not need_a1 else seg_image.data_ptr()
+        aa.image_p = None if seg_image_p is None or not need_a2 else seg_image_p.data_ptr()
         aa.grad_rec_p, aa.grad_rec_u, aa.grad_rec_w = gp
         aa.grad_a1 = None if ga1 is None else ga1.data_ptr()
         aa.grad_a2 = None if ga2 is None else ga2.data_ptr()
@@ -327,6 +389,7 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
                       lambda: nat.check(lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream),
                                         "adfwi_acoustic_adjoint"), steps=n)
         lam = lw[0] if aa.lam_out == lw[0].data_ptr() else lw[1]
+        kept = seg_image = seg_image_p = None     # a kept segment's images are free for reuse now
     if group_planes:
         return (None if ga1 is None else ga1[:, :, :X], None if ga2 is None else ga2[:, :, :X], gsrc)
     out_a1 = ga1.sum(0)[:, :X] if ga1 is not None else None
@@ -351,12 +414,16 @@ class AcousticWave(torch.autograd.Function):
         src = src.detach().to(torch.float32).contiguous()
         need_grad = any(ctx.needs_input_grad[:3])  # grad mode is off inside forward()
         k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
+        need_a1, need_a2 = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         if not need_grad and geom.graph_steps and geom.R:
             from .graphs import graphed_forward
             res = graphed_forward(geom, coef, src, energy_u_from, geom.graph_steps)
         else:
+            resident = resident_segments(geom, k, need_a1, need_a2) if need_grad else 0
             res = run_forward(geom, coef, src, record=True, energy_u_from=energy_u_from,
-                              ckpt_every=k if need_grad else 0)
+                              ckpt_every=k if need_grad else 0, resident=resident, need_a1=need_a1,
+                              need_a2=need_a2)
+        ctx.images = res.get("images") if need_grad else None
         rec = res["rec"]
         if rec is None:
             rec = [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(3)]
@@ -372,10 +439,11 @@ class AcousticWave(torch.autograd.Function):
     def backward(ctx, g_p, g_u, g_w, *_):
         coef, src = ctx.saved_tensors
         need_a1, need_a2, need_src = ctx.needs_input_grad[:3]
+        images, ctx.images = ctx.images, None
         if g_p is None and g_u is None and g_w is None:
             return (None,) * 9
         ga1, ga2, gsrc = run_backward(ctx.geom, coef, src, ctx.ckpt, ctx.k, (g_p, g_u, g_w),
-                                      need_a1, need_a2, need_src)
+                                      need_a1, need_a2, need_src, images=images)
         ctx.ckpt = None
         return ga1, ga2, gsrc, None, None, None, None, None, None
 

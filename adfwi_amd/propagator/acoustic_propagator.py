@@ -83,6 +83,9 @@ class AcousticPropagator(torch.nn.Module):
         # forward runs without gradient replay HIP graphs of this many steps (None: the
         # ADFWI_GRAPH_STEPS environment variable, default 0 = direct launches)
         self.graph_steps = None
+        # checkpoint segments whose stencil images the forward keeps for the backward (None: as
+        # many as the free device memory allows; 0: recompute every segment)
+        self.resident_images = None
 
     def boundary_condition(self, vmax=None):
         """Damping profile (acoustic_propagator.py:102-118); PML vmax frozen at construction."""
@@ -106,7 +109,8 @@ class AcousticPropagator(torch.nn.Module):
             nabc = self.nabc
             g = AcousticGeometry(self.nz + 2 * nabc, self.nx + 2 * nabc, self.nt, self.free_surface, nabc,
                                  sl[:, 1] + nabc, sl[:, 0] + nabc, self.rcv_loc[:, 1] + nabc,
-                                 self.rcv_loc[:, 0] + nabc, self.device, graph_steps=self.graph_steps)
+                                 self.rcv_loc[:, 0] + nabc, self.device, graph_steps=self.graph_steps,
+                                 resident_images=self.resident_images)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)

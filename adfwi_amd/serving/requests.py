@@ -187,6 +187,7 @@ def build_problem(req: Request, device, shots=None, graph_steps: int = 0):
                               nabc=req.nabc, free_surface=False, device=device)
         prop = AcousticPropagator(model, survey, device=device)
         prop.graph_steps = int(graph_steps)
+        prop.resident_images = 0     # the admission footprint counts checkpoints, not image buffers
     else:
         vp, vs = elastic_layers(rng, nz, nx, 6, 1800, 4500, 200.0)
         model = IsotropicElasticModel(0, 0, nx, nz, req.dx, req.dx, vp=vp, vs=vs, rho=gardner(vp),

@@ -341,7 +341,8 @@ def main():
     for name, d in tl.items():
         steps_by = d["steps_by_call"]
         if name in ("ac_fwd_step", "ac_fwd_pair"):
-            byts = (steps_by.get("forward", 0) * BYTES_FWD + steps_by.get("recompute", 0) * BYTES_RECOMPUTE) * N * S
+            byts = (steps_by.get("forward", 0) * BYTES_FWD + (steps_by.get("recompute", 0)
+                    + steps_by.get("forward+images", 0)) * BYTES_RECOMPUTE) * N * S
         elif name in ("ac_adj_step", "ac_adj_pair"):
             byts = d["steps"] * BYTES_ADJ * N * S
         elif name in ("el_fwd_step", "el3_fwd_step"):
@@ -349,7 +350,8 @@ def main():
         else:
             byts = d["steps"] * EL_BYTES_ADJ * N * S
         avg_ms = d["ms"] / max(d["launches"], 1)
-        kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "steps": d["steps"], "avg_launch_ms": avg_ms,
+        kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "steps": d["steps"],
+                      "steps_by_call": d["steps_by_call"], "avg_launch_ms": avg_ms,
                       "achieved_gbs": byts / (d["ms"] / 1e3) / 1e9 if d["ms"] else 0.0,
                       "bytes_per_launch": byts / max(d["launches"], 1)}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])

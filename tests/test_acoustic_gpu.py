@@ -267,3 +267,43 @@ def test_propagator_other_absorbing_boundaries(abc):
     ref["p"].square().sum().backward()
     assert rel_l2(out["p"].detach().cpu().numpy(), ref["p"].detach().numpy()) <= 1e-5
     assert rel_l2(model.vp.grad.cpu().numpy(), v.grad.numpy()) <= 1e-4
+
+
+@pytest.mark.parametrize("rho_grad", [False, True])
+def test_resident_images_equal_recomputed_segments(monkeypatch, rho_grad):
+    """Keeping the last segments' stencil (and pressure) images from the forward pass instead of
+    recomputing those segments in the backward changes nothing: records and energy maps are
+    bit-identical with none, some and all segments resident (nt 61 in 8-step segments, the last
+    one odd), the vp / rho / wavelet gradients identical to rel-L2 1e-12 (the image-writing
+    forward variant and the recompute variant can differ in the last bits of values near the
+    subnormal range)."""
+    from adfwi_amd.propagator import acoustic_kernels as ak
+    g, m = acoustic_inputs("acoustic_fs")
+    nt = 61
+    out = {}
+    for resident in ("0", "3", "100", None):
+        if resident is None:
+            monkeypatch.delenv("ADFWI_RESIDENT_IMAGES", raising=False)
+        else:
+            monkeypatch.setenv("ADFWI_RESIDENT_IMAGES", resident)
+        v = torch.tensor(g["vp"], device=DEV, requires_grad=True)
+        r = torch.tensor(g["rho"], device=DEV, requires_grad=rho_grad)
+        w = torch.tensor(g["wavelet"][:, :nt], device=DEV, requires_grad=True)
+        res = ak.forward_kernel(m["nx"], m["nz"], m["dx"], m["dx"], nt, m["dt"], m["nabc"], m["fs"],
+                                torch.tensor(g["src_x"]), torch.tensor(g["src_z"]), 2, w,
+                                torch.tensor(g["rcv_x"]), torch.tensor(g["rcv_z"]), len(g["rcv_x"]),
+                                torch.tensor(g["damp"], dtype=torch.float32, device=DEV), v, r,
+                                device=DEV, ckpt_every=8)
+        loss = sum((res[k] * torch.tensor(g["weights"][i][:, :nt], device=DEV)).sum() for i, k in enumerate("puw"))
+        loss.backward()
+        o = {k: res[k].detach().cpu().numpy() for k in ("p", "u", "w", "forward_wavefield_p")}
+        o.update(vp=v.grad.cpu().numpy(), wavelet=w.grad.cpu().numpy())
+        if rho_grad:
+            o["rho"] = r.grad.cpu().numpy()
+        out[resident] = o
+    for resident in ("3", "100", None):
+        for k, a in out["0"].items():
+            if k in ("p", "u", "w", "forward_wavefield_p"):
+                assert np.array_equal(out[resident][k], a), (resident, k, rel_l2(out[resident][k], a))
+            else:
+                assert rel_l2(out[resident][k], a) <= 1e-12, (resident, k, rel_l2(out[resident][k], a))
