@@ -210,6 +210,18 @@ class AcousticGeometry:
         return coef
 
 
+def resident_budget_bytes(device) -> int:
+    """Device memory that segments kept resident for the backward may take: three quarters of
+    the free memory less 4 GiB, or ADFWI_IMAGE_BUDGET_BYTES.  Free memory counts what torch's
+    caching allocator holds unused (the previous iteration's resident buffers, typically)."""
+    budget = int(os.environ.get("ADFWI_IMAGE_BUDGET_BYTES", "0"))
+    if not budget:
+        free, _ = torch.cuda.mem_get_info(device)
+        free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+        budget = max(0, free * 3 // 4 - (4 << 30))
+    return budget
+
+
 def resident_segments(geom: AcousticGeometry, k: int, need_a1: bool, need_a2: bool) -> int:
     """How many of the last checkpoint segments keep their stencil (and pressure) images from the
     forward pass, so that the backward skips their recomputation: at most geom.resident_images
@@ -224,11 +236,7 @@ def resident_segments(geom: AcousticGeometry, k: int, need_a1: bool, need_a2: bo
         return 0
     seg_bytes = planes * k * geom.S * geom.Z * geom.pitch * 4
     if cap is None:
-        budget = int(os.environ.get("ADFWI_IMAGE_BUDGET_BYTES", "0"))
-        if not budget:
-            free, _ = torch.cuda.mem_get_info(geom.device)
-            budget = max(0, free * 3 // 4 - (4 << 30))
-        cap = budget // seg_bytes
+        cap = resident_budget_bytes(geom.device) // seg_bytes
     return int(max(0, min(cap, nseg)))
 
 

@@ -17,11 +17,14 @@ NCOEF = 8
 
 def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[torch.Tensor],
                  pml_z: Optional[torch.Tensor], scalars, ckpt: Optional[torch.Tensor], k: int,
-                 g_rec: Sequence[Optional[torch.Tensor]], needs: Sequence[bool], damp: Optional[torch.Tensor] = None):
+                 g_rec: Sequence[Optional[torch.Tensor]], needs: Sequence[bool], damp: Optional[torch.Tensor] = None,
+                 traces: Optional[dict] = None):
     """Adjoint of all nt steps.
 
     ckpt: the forward's checkpoints (slot j = state after step (j+1)k-1), g_rec: five (S, nt, R)
-    record adjoints (txx, tzz, txz, vx, vz; None = zero).  Returns (list of 8 (Z, pitch)
+    record adjoints (txx, tzz, txz, vx, vz; None = zero).  traces: {segment: per-step states}
+    kept by run_forward(resident=...); those segments are not recomputed, and each is released
+    once the segment before it no longer needs its last state.  Returns (list of 8 (Z, pitch)
     gradients of the padded coefficient planes C11, C13, C33, C55, bx, bz, C15, C35 -- None
     where not needed -- and the (S, nt) wavelet gradient or None).
     """
@@ -30,10 +33,12 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[to
     S, Z, nt, pitch = geom.S, geom.Z, geom.nt, geom.pitch
     L = geom.layout
     nseg = -(-nt // k)
-    if nseg > 1 and (ckpt is None or ckpt.shape[0] < nseg - 1):
+    traces = {} if traces is None else traces
+    first_resident = min(traces) if traces else nseg
+    if first_resident > 1 and (ckpt is None or ckpt.shape[0] < first_resident - 1):
         raise RuntimeError("the forward pass kept no checkpoints for the adjoint")
     lam = geom.state(2)
-    trace = geom.state(min(k, nt))
+    trace = geom.state(min(k, nt)) if len(traces) < nseg else None
     zero = None
     gacc = torch.zeros(int(L.accum_floats), device=dev, dtype=torch.float32)
     gsrc = torch.zeros((S, nt), device=dev, dtype=torch.float32)
@@ -48,19 +53,25 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[to
             if zero is None:
                 zero = geom.state(1)[0]
             fwd0 = zero
+        elif seg - 1 in traces:
+            fwd0 = traces[seg - 1][-1]     # the last state of the resident segment before
         else:
             fwd0 = ckpt[seg - 1]
-        fa = nat.ElasticFwdArgs()
-        fa.state_in = fwd0.data_ptr()
-        fa.trace = trace.data_ptr()
-        fa.t0, fa.nsteps = t0, n
-        TIMELINE.call("recompute", geom.kernel_names[0], n, dev, lambda: nat.check(
-            lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(fa), stream), "adfwi_elastic_forward (recompute)"))
+        seg_trace = traces.get(seg)
+        if seg_trace is None:
+            seg_trace = trace
+            fa = nat.ElasticFwdArgs()
+            fa.state_in = fwd0.data_ptr()
+            fa.trace = trace.data_ptr()
+            fa.t0, fa.nsteps = t0, n
+            TIMELINE.call("recompute", geom.kernel_names[0], n, dev, lambda: nat.check(
+                lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(fa), stream),
+                "adfwi_elastic_forward (recompute)"))
         aa = nat.ElasticAdjArgs()
         aa.lam_in = cur.data_ptr()
         aa.work[0], aa.work[1] = lam[0].data_ptr(), lam[1].data_ptr()
         aa.t_hi, aa.nsteps = t0 + n, n
-        aa.fwd0, aa.trace = fwd0.data_ptr(), trace.data_ptr()
+        aa.fwd0, aa.trace = fwd0.data_ptr(), seg_trace.data_ptr()
         for i in range(5):
             aa.grad_rec[i] = None if g[i] is None else g[i].data_ptr()
         aa.grad_coef = gacc.data_ptr()
@@ -68,6 +79,9 @@ def run_backward(geom, coef: torch.Tensor, src: torch.Tensor, pml_x: Optional[to
         TIMELINE.call("adjoint", geom.kernel_names[1], n, dev, lambda: nat.check(
             lib.adfwi_elastic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream), "adfwi_elastic_adjoint"))
         cur = lam[0] if aa.lam_out == lam[0].data_ptr() else lam[1]
+        traces.pop(seg, None)              # done; segment seg + 1, which read its last state, too
+        seg_trace = fwd0 = None
+    traces.clear()
     planes = gacc.view(-1, NCOEF, Z, pitch).sum(0)
     gcoef: List[Optional[torch.Tensor]] = [planes[i] if (i < len(needs) and needs[i]) else None
                                            for i in range(NCOEF)]

@@ -17,7 +17,7 @@ import torch
 import torch.nn.functional as F
 
 from .. import _native as nat
-from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval
+from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval, resident_budget_bytes
 
 NAMES = ("txx", "tzz", "txz", "vx", "vz")
 NFIELD = 10
@@ -60,7 +60,8 @@ class ElasticGeometry:
 
     def __init__(self, Z: int, X: int, nt: int, free_surface: bool, src_z, src_x, src_m: torch.Tensor,
                  rcv_z, rcv_x, device, aniso: bool = False, shots_per_wave: int = 0, tile_z: int = 0,
-                 damping: bool = False, src_fz: Optional[torch.Tensor] = None, nn: int = 2, c2: float = 0.0):
+                 damping: bool = False, src_fz: Optional[torch.Tensor] = None, nn: int = 2, c2: float = 0.0,
+                 resident_traces: Optional[int] = None):
         nat.require_gpu(device)
         self.device = torch.device(device)
         self.Z, self.X, self.nt = int(Z), int(X), int(nt)
@@ -71,6 +72,11 @@ class ElasticGeometry:
         # stencil half-order fd_order // 2: 2 for the 4th-order schemes, 3..5 for the three-term
         # PML scheme (window [nn, Z-nn-1], free-surface rows nn-2 .. nn+1), c2 its third coefficient
         self.nn, self.c2 = int(nn), float(c2)
+        # checkpoint segments (the last ones) whose per-step states the forward pass keeps for the
+        # backward instead of recomputing them there: None = as many as the memory budget holds,
+        # 0 = none; ADFWI_RESIDENT_IMAGES sets it when the argument is None
+        env = os.environ.get("ADFWI_RESIDENT_IMAGES")
+        self.resident_traces = int(env) if env is not None and resident_traces is None else resident_traces
         # (forward, adjoint) step kernels, as the timeline and profiles name them
         self.kernel_names = (("el_abl_fwd", "el_abl_adj") if self.damping else
                              ("el3_fwd_step", "el3_adj_step") if self.nn > 2 else ("el_fwd_step", "el_adj_step"))
@@ -170,42 +176,87 @@ def _energy(state: torch.Tensor, X: int, free_surface: bool, nn: int = 2):
     return out
 
 
+def resident_trace_segments(geom: ElasticGeometry, k: int) -> int:
+    """How many of the last checkpoint segments keep their per-step states from the forward pass
+    (the backward then skips their recomputation): at most geom.resident_traces (0 = none; None =
+    as many as resident_budget_bytes allows)."""
+    nseg = -(-geom.nt // k)
+    cap = geom.resident_traces
+    if cap == 0:
+        return 0
+    if cap is None:
+        cap = resident_budget_bytes(geom.device) // (k * int(geom.layout.state_floats) * 4)
+    return int(max(0, min(cap, nseg)))
+
+
+def _state_at(ptr: int, work, bufs) -> torch.Tensor:
+    """The state tensor a forward call left its last step in (a work buffer, a checkpoint slot
+    or a trace slot)."""
+    for w in work:
+        if w.data_ptr() == ptr:
+            return w
+    for buf in bufs:
+        step = buf[0].numel() * buf.element_size()
+        off = ptr - buf.data_ptr()
+        if 0 <= off < buf.shape[0] * step and off % step == 0:
+            return buf[off // step]
+    raise RuntimeError("forward state pointer outside the work, checkpoint and trace buffers")
+
+
 def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, record=True, energy_ends=None,
-                ckpt_every: int = 0, damp=None):
+                ckpt_every: int = 0, damp=None, resident: int = 0):
     """All nt steps from a zero state.  Returns dict(rec=[5 x (S, nt, R)] or None,
-    energy=[5 x (Z, X)] summed over the states after the steps in energy_ends, ckpt)."""
+    energy=[5 x (Z, X)] summed over the states after the steps in energy_ends, ckpt,
+    traces={segment: per-step states}).
+
+    resident > 0 (with ckpt_every): the last `resident` checkpoint segments write the state after
+    each of their steps to a trace buffer of their own, which run_backward uses instead of
+    recomputing the segment; their checkpoints are not written."""
     lib = nat.load()
     dev = geom.device
     S, nt, R = geom.S, geom.nt, geom.R
     work = geom.state(2)
+    k = int(ckpt_every)
+    nseg = -(-nt // k) if k else 1
+    resident = int(resident) if k else 0
+    head = (nseg - resident) * k if resident else nt      # steps before the resident segments
     ckpt = None
-    if ckpt_every:
-        ckpt = geom.state(max(-(-nt // ckpt_every) - 1, 1))
+    if k and head > 0:
+        ckpt = geom.state(max(nseg - 1, 1) if not resident else max(nseg - resident, 1))
+    traces = {seg: geom.state(min(nt, (seg + 1) * k) - seg * k) for seg in range(nseg - resident, nseg)}
     rec = [torch.zeros((S, nt, R), device=dev) for _ in range(5)] if record and R else None
     d = geom.desc(coef, pml_x, pml_z, src, scalars, damp)
     ends = sorted(set(energy_ends or [])) or [nt - 1]
+    # calls end at the energy-map steps and at the edges of the resident segments
+    stops = set(ends) | {head - 1} | {min(nt, (seg + 1) * k) - 1 for seg in traces}
     energy = None
     cur = work[0]
     t0 = 0
     stream = nat.stream_handle(dev)
-    for te in ends:
+    bufs = ([] if ckpt is None else [ckpt]) + list(traces.values())
+    for te in sorted(s for s in stops if s >= 0):
         a = nat.ElasticFwdArgs()
         a.state_in = cur.data_ptr()
         a.work[0], a.work[1] = work[0].data_ptr(), work[1].data_ptr()
-        a.ckpt = None if ckpt is None else ckpt.data_ptr()
-        a.ckpt_every = int(ckpt_every)
+        if t0 >= head:            # inside a resident segment: every state goes to its trace
+            seg = t0 // k
+            a.trace = traces[seg][t0 - seg * k].data_ptr()
+        else:
+            a.ckpt = None if ckpt is None else ckpt.data_ptr()
+            a.ckpt_every = k if ckpt is not None else 0
         a.t0, a.nsteps = t0, te + 1 - t0
         if rec is not None:
             for i in range(5):
                 a.rec[i] = rec[i].data_ptr()
-        TIMELINE.call("forward", geom.kernel_names[0], a.nsteps, dev, lambda: nat.check(
-            lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream), "adfwi_elastic_forward"))
-        cur = work[0] if a.state_out == work[0].data_ptr() else work[1]
-        if energy_ends is not None:
+        TIMELINE.call("forward" if t0 < head else "forward+trace", geom.kernel_names[0], a.nsteps, dev,
+                      lambda: nat.check(lib.adfwi_elastic_forward(ctypes.byref(d), ctypes.byref(a), stream),
+                                        "adfwi_elastic_forward"))
+        cur = _state_at(a.state_out, work, bufs)
+        if energy_ends is not None and te in ends:
             e = _energy(cur, geom.X, geom.free_surface, geom.nn)
             energy = e if energy is None else [x + y for x, y in zip(energy, e)]
         t0 = te + 1
-    return {"rec": rec, "energy": energy, "ckpt": ckpt, "final": cur}
+    return {"rec": rec, "energy": energy, "ckpt": ckpt, "final": cur, "traces": traces}
 
 
 class ElasticWave(torch.autograd.Function):
@@ -225,8 +276,10 @@ class ElasticWave(torch.autograd.Function):
         need_grad = any(ctx.needs_input_grad[:9])
         k = int(ckpt_every) if ckpt_every else checkpoint_interval(geom.nt)
         absorb = (None, None, pml_x) if geom.damping else (pml_x, pml_z, None)
+        resident = resident_trace_segments(geom, k) if need_grad else 0
         res = run_forward(geom, coef, absorb[0], absorb[1], src, scalars, record=True, energy_ends=energy_ends,
-                          ckpt_every=k if need_grad else 0, damp=absorb[2])
+                          ckpt_every=k if need_grad else 0, damp=absorb[2], resident=resident)
+        ctx.traces = res["traces"] if need_grad else None
         rec = res["rec"] or [torch.zeros((geom.S, geom.nt, geom.R), device=geom.device) for _ in range(5)]
         ctx.geom, ctx.k, ctx.scalars = geom, k, scalars
         ctx.ckpt = res["ckpt"] if need_grad else None
@@ -238,13 +291,14 @@ class ElasticWave(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grads):
+        traces, ctx.traces = ctx.traces, None
         if all(g is None for g in grads[:5]):
             return (None,) * 15
         from .elastic_adjoint import run_backward
         coef, src, pml_x, pml_z = ctx.saved_tensors
         absorb = (None, None, pml_x) if ctx.geom.damping else (pml_x, pml_z, None)
         gcoef, gsrc = run_backward(ctx.geom, coef, src, absorb[0], absorb[1], ctx.scalars, ctx.ckpt, ctx.k, grads[:5],
-                                   ctx.needs_input_grad[:9], damp=absorb[2])
+                                   ctx.needs_input_grad[:9], damp=absorb[2], traces=traces)
         ctx.ckpt = None
         out = []
         for i, shp in enumerate(ctx.shapes):

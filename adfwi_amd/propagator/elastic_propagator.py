@@ -62,6 +62,9 @@ class ElasticPropagator(torch.nn.Module):
         self.rcv_n = self.receiver.num
         self._geoms = OrderedDict()
         self.ckpt_every = 0
+        # checkpoint segments whose per-step states the forward keeps for the backward (None: as
+        # many as the free device memory allows; 0: recompute every segment)
+        self.resident_images = None
 
     def boundary_condition(self):
         """Damping profiles (elastic_propagator.py:99-114); 'PML' is case-sensitive as there."""
@@ -97,7 +100,8 @@ class ElasticPropagator(torch.nn.Module):
             g = ElasticGeometry(Z, X, self.nt, self.free_surface, sl[:, 1] + zoff, sl[:, 0] + self.nabc, src_m,
                                 self.rcv_loc[:, 1] + zoff, self.rcv_loc[:, 0] + self.nabc, self.device, aniso=aniso,
                                 damping=damping, src_fz=fz, nn=fso,
-                                c2=float(DiffCoef(fso, "s")[2]) if fso > 2 else 0.0)
+                                c2=float(DiffCoef(fso, "s")[2]) if fso > 2 else 0.0,
+                                resident_traces=self.resident_images)
             self._geoms[key] = g
             while len(self._geoms) > 16:
                 self._geoms.popitem(last=False)

@@ -194,6 +194,7 @@ def build_problem(req: Request, device, shots=None, graph_steps: int = 0):
                                       vp_grad=grad, vs_grad=grad, rho_grad=grad, free_surface=True,
                                       nabc=req.nabc, auto_update_rho=False, device=device)
         prop = ElasticPropagator(model, survey, device=device)
+        prop.resident_images = 0
     return prop, model
 
 

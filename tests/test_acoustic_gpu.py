@@ -274,9 +274,8 @@ def test_resident_images_equal_recomputed_segments(monkeypatch, rho_grad):
     """Keeping the last segments' stencil (and pressure) images from the forward pass instead of
     recomputing those segments in the backward changes nothing: records and energy maps are
     bit-identical with none, some and all segments resident (nt 61 in 8-step segments, the last
-    one odd), the vp / rho / wavelet gradients identical to rel-L2 1e-12 (the image-writing
-    forward variant and the recompute variant can differ in the last bits of values near the
-    subnormal range)."""
+    one odd), the vp / rho / wavelet gradients identical to rel-L2 1e-12 (torch's backward of the
+    edge padding sums the pad cells with atomic adds, whose order varies from run to run)."""
     from adfwi_amd.propagator import acoustic_kernels as ak
     g, m = acoustic_inputs("acoustic_fs")
     nt = 61

@@ -323,3 +323,47 @@ def test_elastic_propagator_damping_scheme_matches_reference_vectors(name):
     for k in ELASTIC_NAMES:
         got = res[k].cpu().numpy()
         assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))
+
+
+@pytest.mark.parametrize("abc", ["PML", "gerjan6"])
+def test_resident_traces_equal_recomputed_segments(monkeypatch, abc):
+    """Keeping the last segments' per-step states from the forward pass instead of recomputing
+    them in the backward changes nothing (the same kernel writes them): records and energy maps
+    are bit-identical with none, some and all segments resident, also when the energy-map chunks
+    of checkpoint_segments=3 split a resident segment; gradients agree to rel-L2 1e-12 (torch's
+    backward of the edge padding sums the pad cells with atomic adds, whose order varies from run
+    to run)."""
+    abc, fd = _split_abc(abc)
+    g, m = elastic_random_case(30, 90, 6, 45, 3, True, seed=8, aniso=True, abc=abc, fd_order=fd)
+    out = {}
+    for resident in ("0", "2", "100", None):
+        if resident is None:
+            monkeypatch.delenv("ADFWI_RESIDENT_IMAGES", raising=False)
+        else:
+            monkeypatch.setenv("ADFWI_RESIDENT_IMAGES", resident)
+        res, leaves, w = _dropin(g, m, seg=3, want_grad=True, ckpt_every=7)
+        _loss(res, g["weights"]).backward()
+        o = {k: res[k].detach().cpu().numpy() for k in ELASTIC_NAMES}
+        o.update({"fw_" + k: res["forward_wavefield_" + k].cpu().numpy() for k in ELASTIC_NAMES})
+        o.update({"grad_" + k: v.grad.cpu().numpy() for k, v in leaves.items()})
+        o["grad_wavelet"] = w.grad.cpu().numpy()
+        out[resident] = o
+    for resident in ("2", "100", None):
+        for k, a in out["0"].items():
+            if k.startswith("grad_"):
+                assert rel_l2(out[resident][k], a) <= 1e-12, (resident, k, rel_l2(out[resident][k], a))
+            else:
+                assert np.array_equal(out[resident][k], a), (resident, k, rel_l2(out[resident][k], a))
+
+
+def test_energy_chunk_ending_on_a_checkpoint_step(monkeypatch):
+    """checkpoint_segments=3 over nt 45 ends its chunks after steps 14 and 29, whose states are
+    checkpoints when the checkpoint interval is 5: the next chunk must start from that
+    checkpoint slot.  Records and energy maps against the oracle."""
+    monkeypatch.setenv("ADFWI_RESIDENT_IMAGES", "0")
+    g, m = elastic_random_case(30, 90, 6, 45, 3, True, seed=9)
+    ref = _run_oracle(g, m, seg=3)
+    res, _, _ = _dropin(g, m, seg=3, want_grad=True, ckpt_every=5)
+    for k in ELASTIC_NAMES:
+        assert rel_l2(res[k].detach().cpu().numpy(), ref[k]) <= 1e-5, k
+        assert rel_l2(res["forward_wavefield_" + k].cpu().numpy(), ref["fw_" + k]) <= 1e-5, ("energy", k)

@@ -93,15 +93,17 @@ def test_elastic_model_gradients_match_oracle(vti):
         assert rel_l2(got[n], ref) <= 1e-4, (n, rel_l2(got[n], ref))
 
 
-def test_elastic_finite_difference_along_random_direction():
+@pytest.mark.parametrize("fd_order", [4, 6])
+def test_elastic_finite_difference_along_random_direction(fd_order):
     """<grad, d> against the central difference of the loss along a seeded smooth direction d in
-    (vp, vs, rho), with the model classes' parameterisation recomputed for each perturbation."""
+    (vp, vs, rho), with the model classes' parameterisation recomputed for each perturbation, for
+    the 4th-order and the three-term (6th-order) stencil."""
     c = _elastic_case(False, seed=41)
     rng = np.random.default_rng(5)
     dirs = {n: _smooth(rng, c["nz"], c["nx"], -1.0, 1.0) * scale for n, scale in
             (("vp", 20.0), ("vs", 12.0), ("rho", 15.0))}
     prop, model = _elastic_propagator(c, False)
-    _elastic_loss(prop.forward(), c["wts"]).backward()
+    _elastic_loss(prop.forward(fd_order=fd_order), c["wts"]).backward()
     dd = sum(float((getattr(model, n).grad.double().cpu() * torch.tensor(d).double()).sum()) for n, d in dirs.items())
     vals = []
     for sgn in (1.0, -1.0):
@@ -110,7 +112,7 @@ def test_elastic_finite_difference_along_random_direction():
             cc[n] = (c[n] + sgn * d).astype(np.float32)
         p, _ = _elastic_propagator(cc, False, grad=False)
         with torch.no_grad():
-            vals.append(float(_elastic_loss(p.forward(), c["wts"])))
+            vals.append(float(_elastic_loss(p.forward(fd_order=fd_order), c["wts"])))
     fd = (vals[0] - vals[1]) / 2.0
     assert abs(fd - dd) <= 5e-3 * abs(dd), (fd, dd)
 
