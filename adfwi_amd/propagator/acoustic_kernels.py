@@ -210,25 +210,24 @@ class AcousticGeometry:
         return coef
 
 
-def resident_budget_bytes(device) -> int:
-    """Device memory that segments kept resident for the backward may take: three quarters of
-    the free memory less 4 GiB, or ADFWI_IMAGE_BUDGET_BYTES.  Free memory counts what torch's
-    caching allocator holds unused (the previous iteration's resident buffers, typically)."""
+def resident_budget_bytes(device, other_bytes: int) -> int:
+    """Device memory that segments kept resident for the backward may take: 80 % of the free
+    memory (what torch's caching allocator holds unused included: the previous iteration's
+    buffers, typically) less 8 GiB and less `other_bytes`, the checkpoints and the backward's own
+    buffers still to be allocated; ADFWI_IMAGE_BUDGET_BYTES overrides it."""
     budget = int(os.environ.get("ADFWI_IMAGE_BUDGET_BYTES", "0"))
     if not budget:
         free, _ = torch.cuda.mem_get_info(device)
         free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
-        budget = max(0, free * 3 // 4 - (4 << 30))
+        budget = max(0, free * 4 // 5 - (8 << 30) - int(other_bytes))
     return budget
 
 
 def resident_segments(geom: AcousticGeometry, k: int, need_a1: bool, need_a2: bool) -> int:
     """How many of the last checkpoint segments keep their stencil (and pressure) images from the
     forward pass, so that the backward skips their recomputation: at most geom.resident_images
-    (0 = none); with None, as many as fit in three quarters of the free device memory less 4 GiB
-    (ADFWI_IMAGE_BUDGET_BYTES overrides the budget).  What the backward allocates besides (two
-    adjoint states, one segment's recompute images, the gradient planes) stays well inside the
-    remaining quarter."""
+    (0 = none); with None, as many as resident_budget_bytes allows once the checkpoints, one
+    segment's recompute images and the four states of the backward are set aside."""
     planes = int(bool(need_a1)) + int(bool(need_a2))
     nseg = -(-geom.nt // k)
     cap = geom.resident_images
@@ -236,7 +235,8 @@ def resident_segments(geom: AcousticGeometry, k: int, need_a1: bool, need_a2: bo
         return 0
     seg_bytes = planes * k * geom.S * geom.Z * geom.pitch * 4
     if cap is None:
-        cap = resident_budget_bytes(geom.device) // seg_bytes
+        state_bytes = 3 * geom.S * geom.Z * geom.pitch * 4
+        cap = resident_budget_bytes(geom.device, (nseg + 4) * state_bytes + seg_bytes) // seg_bytes
     return int(max(0, min(cap, nseg)))
 
 

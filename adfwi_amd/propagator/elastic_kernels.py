@@ -179,13 +179,16 @@ def _energy(state: torch.Tensor, X: int, free_surface: bool, nn: int = 2):
 def resident_trace_segments(geom: ElasticGeometry, k: int) -> int:
     """How many of the last checkpoint segments keep their per-step states from the forward pass
     (the backward then skips their recomputation): at most geom.resident_traces (0 = none; None =
-    as many as resident_budget_bytes allows)."""
+    as many as resident_budget_bytes allows once the checkpoints, the backward's recompute trace
+    and its three states are set aside)."""
     nseg = -(-geom.nt // k)
     cap = geom.resident_traces
     if cap == 0:
         return 0
+    state_bytes = int(geom.layout.state_floats) * 4
+    seg_bytes = k * state_bytes
     if cap is None:
-        cap = resident_budget_bytes(geom.device) // (k * int(geom.layout.state_floats) * 4)
+        cap = resident_budget_bytes(geom.device, (nseg + 3) * state_bytes + seg_bytes) // seg_bytes
     return int(max(0, min(cap, nseg)))
 
 
