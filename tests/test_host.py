@@ -211,3 +211,31 @@ def test_ctypes_structures_match_the_c_header(tmp_path):
     subprocess.run([cc, "-std=c99", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
     got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     assert [g for g in got if g] == expect
+
+
+def test_resident_segment_counts(monkeypatch):
+    """How many checkpoint segments the forward keeps resident: capped by the geometry's setting,
+    by the number of segments and by the memory budget (here set through the environment), and
+    none when no image is needed."""
+    from types import SimpleNamespace
+    from adfwi_amd.propagator.acoustic_kernels import resident_segments
+    from adfwi_amd.propagator.elastic_kernels import resident_trace_segments
+    k, S, Z, pitch, nt = 8, 2, 10, 64, 61                    # 8 segments, the last 5 steps
+    seg = k * S * Z * pitch * 4
+    monkeypatch.setenv("ADFWI_IMAGE_BUDGET_BYTES", str(3 * seg + 5))
+    g = SimpleNamespace(nt=nt, S=S, Z=Z, pitch=pitch, resident_images=None, device="cpu")
+    assert resident_segments(g, k, True, False) == 3
+    assert resident_segments(g, k, True, True) == 1          # two planes per step
+    assert resident_segments(g, k, False, False) == 0
+    g.resident_images = 2
+    assert resident_segments(g, k, True, False) == 2
+    g.resident_images = 0
+    assert resident_segments(g, k, True, False) == 0
+    g.resident_images = None
+    monkeypatch.setenv("ADFWI_IMAGE_BUDGET_BYTES", str(100 * seg))
+    assert resident_segments(g, k, True, False) == 8         # every segment
+    e = SimpleNamespace(nt=nt, resident_traces=None, device="cpu", layout=SimpleNamespace(state_floats=1000))
+    monkeypatch.setenv("ADFWI_IMAGE_BUDGET_BYTES", str(2 * k * 1000 * 4))
+    assert resident_trace_segments(e, k) == 2
+    e.resident_traces = 1
+    assert resident_trace_segments(e, k) == 1
