@@ -135,12 +135,13 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
       vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
     }
-    // everything iteration k reads from memory, loaded one iteration ahead:
-    // velocity splits of row k+2, stress splits and moduli of row k, old velocity splits and
-    // buoyancies of row k-2
+    // everything iteration k reads from memory, loaded one iteration ahead: velocity splits of
+    // row k+2, stress splits and moduli of row k, buoyancies of row k-2.  The old velocity splits
+    // of row k-2 come from a register window of the splits loaded four iterations before.
     struct Row {
-      float v[4], s[6], cm[ANISO ? 6 : 4], u[4], b[2];
+      float v[4], s[6], cm[ANISO ? 6 : 4], b[2];
     };
+    float us[5][4] = {};                         // velocity splits of rows k-2 .. k+2
     auto load = [&](int k, Row &R) {
       const unsigned o2 = rb(k + 2), o0 = rb(k), om = rb(k - 2);
 #pragma unroll
@@ -153,8 +154,6 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         R.cm[4] = bld(rC, vc, C15 * planeb + o0); R.cm[5] = bld(rC, vc, C35 * planeb + o0);
       }
       if (k - 2 >= z0 && k - 2 < end_row) {   // only rows this strip updates
-#pragma unroll
-        for (int q = 0; q < 4; ++q) R.u[q] = bld(rIn, vc, F(VX_X + q) + om);
         R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
       }
     };
@@ -168,6 +167,13 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       // 1. velocity totals of row k+2 into the window (zero outside the grid)
       {
         const bool ok = k + 2 <= zmax;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) us[e][q] = us[e + 1][q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) us[4][q] = cur.v[q];
         const float a = cur.v[0] + cur.v[1], b = cur.v[2] + cur.v[3];
         vx0 = vx1; vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
         vz0 = vz1; vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
@@ -265,10 +271,10 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
         const float bxv = cur.b[0], bzv = cur.b[1];
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
-        const float nx_x = (pxn * cur.u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
-        const float nx_z = (pzn * cur.u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
-        float nz_x = (pxn * cur.u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
-        float nz_z = (pzn * cur.u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        const float nx_x = (pxn * us[0][0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * us[0][1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        float nz_x = (pxn * us[0][2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        float nz_z = (pzn * us[0][3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
         if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
           const float hf = 0.5f * (d.src_fz[s] * wsrc);
           nz_x = nz_x + hf;
