@@ -163,6 +163,7 @@ def test_dropin_forward_matches_reference_vectors(name, seg):
 
 
 @pytest.mark.parametrize("fs,aniso,spw,tz,abc", [(True, False, 1, 4, "PML"), (False, False, 2, 8, "PML"),
+                                                  (True, True, 1, 32, "PML"), (False, False, 2, 24, "PML"),
                                                   (True, True, 1, 16, "PML"), (False, True, 0, 0, "PML"),
                                                   (True, False, 1, 4, "gerjan"), (False, True, 2, 8, "sincos"),
                                                   (True, True, 0, 0, "gerjan"),
