@@ -15,7 +15,6 @@ from typing import Dict, Optional
 
 import numpy as np
 import torch
-import torch.nn.functional as F
 
 from .. import _native as nat
 
@@ -24,12 +23,23 @@ C2 = -1.0 / 24.0
 PITCH_ALIGN = 64  # floats: rows start on 256-byte boundaries
 
 
+def replicate_pad(a: torch.Tensor, left: int, right: int, top: int, bottom: int) -> torch.Tensor:
+    """Edge-replicating pad of a 2-D array (the values of F.pad(mode='replicate')) whose backward
+    sums each edge's pad cells with expand's reduction, which is deterministic, instead of the
+    atomic adds of the replicate-pad backward, whose order varies from run to run."""
+    nz, nx = a.shape
+    parts = ([a[:1].expand(top, nx)] if top else []) + [a] + ([a[-1:].expand(bottom, nx)] if bottom else [])
+    a = torch.cat(parts, 0) if len(parts) > 1 else a
+    Z = a.shape[0]
+    parts = ([a[:, :1].expand(Z, left)] if left else []) + [a] + ([a[:, -1:].expand(Z, right)] if right else [])
+    return torch.cat(parts, 1) if len(parts) > 1 else a
+
+
 def pad_torchSingle(v: torch.Tensor, pml: int, nz: int, nx: int, ns: int,
                     device: torch.device = torch.device("cpu")) -> torch.Tensor:
     """Edge-replicating pad (nz, nx) -> (nz+2*pml, nx+2*pml), float32 like the reference
     (acoustic_kernels.py:16-37 allocates the destination without a dtype)."""
-    out = F.pad(v.reshape(1, 1, nz, nx).to(torch.float32), (pml, pml, pml, pml), mode="replicate")
-    return out.reshape(nz + 2 * pml, nx + 2 * pml)
+    return replicate_pad(v.reshape(nz, nx).to(torch.float32), pml, pml, pml, pml)
 
 
 def checkpoint_interval(nt: int) -> int:

@@ -14,10 +14,9 @@ from typing import Dict, List, Optional
 
 import numpy as np
 import torch
-import torch.nn.functional as F
 
 from .. import _native as nat
-from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval, resident_budget_bytes
+from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval, replicate_pad, resident_budget_bytes
 
 NAMES = ("txx", "tzz", "txz", "vx", "vz")
 NFIELD = 10
@@ -42,10 +41,8 @@ def pad_torchSingle(data: torch.Tensor, pml: int, fs_offset: int, free_surface: 
     """Edge-replicating pad of an array by its own shape, float32 like the reference
     (elastic_kernels.py:175-216): fs_offset rows on top (plus pml without a free surface),
     pml rows below and pml columns on both sides."""
-    nz, nx = data.shape
     top = fs_offset if free_surface else fs_offset + pml
-    out = F.pad(data.to(torch.float32).reshape(1, 1, nz, nx), (pml, pml, top, pml), mode="replicate")
-    return out.reshape(nz + top + pml, nx + 2 * pml)
+    return replicate_pad(data.to(torch.float32), pml, pml, top, pml)
 
 
 def chunk_ends(nt: int, segments: int) -> List[int]:
