@@ -430,10 +430,13 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
     // forward velocity splits, the velocity adjoints and the buoyancies; row k of the stress
     // adjoints, the moduli and the forward stress splits; row k-2 of the velocity adjoints
     struct Row {
-      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4], fs[6], lr[4];
+      float fv[4], lv[4], b[2], ls[6], cm[ANISO ? 6 : 4], fs[6];
     };
+    // velocity adjoints of rows k-2 .. k+2 as loaded (the row k-2 ones are the old values the
+    // velocity stage needs, so they are not loaded a second time)
+    float lw[5][4] = {};
     auto load = [&](int k, Row &R) {
-      const unsigned o2 = rb(k + 2), o0 = rb(k), om = rb(k - 2);
+      const unsigned o2 = rb(k + 2), o0 = rb(k);
 #pragma unroll
       for (int q = 0; q < 4; ++q) R.fv[q] = bld(rFv, vc, F(VX_X + q) + o2);
 #pragma unroll
@@ -448,10 +451,6 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       }
 #pragma unroll
       for (int q = 0; q < 6; ++q) R.fs[q] = bld(rFs, vc, F(TXX_X + q) + o0);
-      if (k - 2 >= z0 && k - 2 < end_row) {   // only rows this strip updates
-#pragma unroll
-        for (int q = 0; q < 4; ++q) R.lr[q] = bld(rLi, vc, F(VX_X + q) + om);
-      }
     };
     Row cur;
     load(z0 - 2, cur);
@@ -463,6 +462,13 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       {
         const int rr = k + 2;
         const bool in_grid = rr <= zmax;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) lw[e][q] = lw[e + 1][q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lw[4][q] = cur.lv[q];
         const float fvx = in_grid ? cur.fv[0] + cur.fv[1] : 0.f;
         const float fvz = in_grid ? cur.fv[2] + cur.fv[3] : 0.f;
         float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
@@ -617,7 +623,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const unsigned o = rb(r);
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
-        const float l1 = cur.lr[0], l2 = cur.lr[1], l3 = cur.lr[2], l4 = cur.lr[3];
+        const float l1 = lw[0][0], l2 = lw[0][1], l3 = lw[0][2], l4 = lw[0][3];
         if (d.src_fz && gsrc && r == zs && c == xs)   // vertical force: f w / 2 entered both vz splits
           gsrc[(long long)s * d.nt + t] = gmt + 0.5f * (d.src_fz[s] * (l3 + l4));
         bst(vx_keep * l1 + lvx, rLo, vs, F(VX_X) + o); bst(vz_keep * l2 + lvx, rLo, vs, F(VX_Z) + o);
