@@ -239,3 +239,17 @@ def test_resident_segment_counts(monkeypatch):
     assert resident_trace_segments(e, k) == 2
     e.resident_traces = 1
     assert resident_trace_segments(e, k) == 1
+
+
+def test_forward_state_lookup_finds_work_checkpoint_and_trace_slots():
+    """The elastic forward driver maps the state pointer a call ends on back to its tensor: a
+    work buffer, a checkpoint slot or a trace slot (a call can end on a checkpoint step)."""
+    from adfwi_amd.propagator.elastic_kernels import _state_at
+    work = torch.zeros((2, 3, 4, 5))
+    ckpt = torch.zeros((4, 3, 4, 5))
+    trace = torch.zeros((7, 3, 4, 5))
+    assert _state_at(work[1].data_ptr(), work, [ckpt, trace]).data_ptr() == work[1].data_ptr()
+    assert _state_at(ckpt[2].data_ptr(), work, [ckpt, trace]).data_ptr() == ckpt[2].data_ptr()
+    assert _state_at(trace[6].data_ptr(), work, [ckpt, trace]).data_ptr() == trace[6].data_ptr()
+    with pytest.raises(RuntimeError):
+        _state_at(trace[6].data_ptr() + 4, work, [ckpt, trace])
