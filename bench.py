@@ -83,7 +83,7 @@ def build_problem(wl, dev):
     return prop, init, obs
 
 
-def build_elastic_problem(wl, dev, fd_order=4):
+def build_elastic_problem(wl, dev, fd_order=4, abc="PML"):
     """Elastic problem of an elastic workload: vertical-force Ricker shots; the true model gives
     the observed vx / vz, the initial model (each field smoothed, sigma 10 cells) carries the
     gradients; rho is a leaf."""
@@ -98,7 +98,7 @@ def build_elastic_problem(wl, dev, fd_order=4):
     survey = Survey(src, rcv)
     smooth = lambda a: gaussian_filter(a, 10, mode="reflect").astype(np.float32)
     m = wl.models
-    common = dict(free_surface=wl.free_surface, nabc=wl.nabc, auto_update_rho=False, device=dev)
+    common = dict(free_surface=wl.free_surface, nabc=wl.nabc, auto_update_rho=False, abc_type=abc, device=dev)
     if "eps" in m:
         zero = np.zeros_like(m["vp"])
         mk = lambda f, grad: AnisotropicElasticModel(
@@ -181,10 +181,10 @@ def cpu_baseline(seconds_budget=24.0):
             "cpu": _cpu_name()}
 
 
-def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4):
+def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4, abc="PML"):
     """The CPU oracle (eager torch fp32, the reference's op order, autograd) on a bounded sample of
     the elastic workload: all shots cut to `nb` shots and a few time steps, scaled per shot-step."""
-    from adfwi_amd.propagator.boundary_condition import bc_pml_xz
+    from adfwi_amd.propagator.boundary_condition import bc_gerjan, bc_pml_xz, bc_sincos
     from oracle import elastic as oe
     from oracle import model as om
     cores = min(16, os.cpu_count() or 1)
@@ -199,6 +199,11 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4):
     cc0, bx0, bz0, _, _ = om.elastic_cc(m["vp"], m["vs"], m["rho"], m.get("eps", zero), m.get("delta", zero),
                                         zero, nx, nz)
     bcx, bcz = bc_pml_xz(nx, nz, wl.dx, wl.dx, wl.nabc, float(m["vp"].max()), wl.free_surface)
+    damp = None
+    if abc == "gerjan":
+        damp = torch.tensor(bc_gerjan(nx, nz, wl.dx, wl.dx, wl.nabc, 0.0053, wl.free_surface), dtype=torch.float32)
+    elif abc == "sincos":
+        damp = torch.tensor(bc_sincos(nx, nz, wl.dx, wl.dx, wl.nabc, wl.free_surface), dtype=torch.float32)
     nb = 2
 
     def one(nt_run):
@@ -212,9 +217,9 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4):
         res = oe.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, wl.nabc, wl.free_surface,
                                 torch.tensor(wl.src_x[:nb]), torch.tensor(wl.src_z[:nb]), nb, wav,
                                 torch.tensor(np.tile(np.eye(3, dtype=np.float32), (nb, 1, 1))),
-                                torch.tensor(wl.rcv_x), torch.tensor(wl.rcv_z), len(wl.rcv_x), "PML",
+                                torch.tensor(wl.rcv_x), torch.tensor(wl.rcv_z), len(wl.rcv_x), abc,
                                 torch.tensor(bcx, dtype=torch.float32), torch.tensor(bcz, dtype=torch.float32),
-                                None, z, z, bx, bz, CC, fd_order=fd_order)
+                                damp, z, z, bx, bz, CC, fd_order=fd_order)
         (res["vx"].square().sum() + res["vz"].square().sum()).backward()
     t0 = time.perf_counter()
     one(10)
@@ -228,7 +233,7 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4):
     t = float(np.median(times)) * (wl.nt / nt_run) * (wl.nshots / nb)
     return {"value": wl.nshots / t, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
             "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward+backward, full "
-                       f"{nz}x{nx} grid, fd_order {fd_order}, {nb} of {wl.nshots} shots, {nt_run} of {wl.nt} steps, "
+                       f"{nz}x{nx} grid, fd_order {fd_order}, {abc}, {nb} of {wl.nshots} shots, {nt_run} of {wl.nt} steps, "
                        f"median of 3, "
                        f"scaled per shot and step; explosive moment-tensor shots (the reference has no "
                        f"vertical-force source; the stepping cost is the same)"),
@@ -285,7 +290,9 @@ def main():
     ap.add_argument("--workload", default="acoustic_large")
     ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs only)")
     ap.add_argument("--fd-order", type=int, default=4, choices=(4, 6, 8, 10),
-                    help="elastic workloads: FD order (6-10: the three-term PML scheme)")
+                    help="elastic workloads: FD order (6-10: the three-term schemes)")
+    ap.add_argument("--abc", default="PML", choices=("PML", "gerjan", "sincos"),
+                    help="elastic workloads: absorbing boundary (gerjan / sincos: the damping scheme)")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (PMC counter calibration)")
@@ -315,7 +322,9 @@ def main():
     if fd != 4 and not elastic:
         sys.exit("--fd-order applies to the elastic workloads")
     log(f"workload {wl.name}: grid {wl.nz}x{wl.nx}, nt {wl.nt}, {wl.nshots} shots, fd_order {fd}")
-    prop, model, obs = build_elastic_problem(wl, dev, fd) if elastic else build_problem(wl, dev)
+    if args.abc != "PML" and not elastic:
+        sys.exit("--abc applies to the elastic workloads")
+    prop, model, obs = build_elastic_problem(wl, dev, fd, args.abc) if elastic else build_problem(wl, dev)
     log("observed data modelled")
     for i in range(args.warmup):
         grad_step(prop, model, obs, wl.dt, fd)
@@ -347,6 +356,10 @@ def main():
             byts = d["steps"] * BYTES_ADJ * N * S
         elif name in ("el_fwd_step", "el3_fwd_step"):
             byts = d["steps"] * EL_BYTES_FWD * N * S
+        elif name in ("el_abl_fwd", "el3_abl_fwd"):
+            byts = d["steps"] * EL_BYTES_FWD / 2 * N * S
+        elif name in ("el_abl_adj", "el3_abl_adj"):
+            byts = d["steps"] * EL_BYTES_ADJ / 2 * N * S
         else:
             byts = d["steps"] * EL_BYTES_ADJ * N * S
         avg_ms = d["ms"] / max(d["launches"], 1)
@@ -373,7 +386,8 @@ def main():
                    "nt": wl.nt, "shots_per_launch": S, "receivers": len(wl.rcv_x), "dt": wl.dt, "dx": wl.dx,
                    "checkpoint_interval": ak.checkpoint_interval(wl.nt), "parallelism": "single GPU, shot batch"},
         "gpt_updates_per_s": 2 * S * N * wl.nt * args.steps / elapsed / 1e9,
-        "grad_bytes_roofline_frac": (EL_BYTES_GRAD if elastic else BYTES_GRAD) * S * N * wl.nt * args.steps
+        "grad_bytes_roofline_frac": ((EL_BYTES_GRAD / (2 if args.abc != "PML" else 1)) if elastic else BYTES_GRAD)
+                                    * S * N * wl.nt * args.steps
                                     / elapsed / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, wl.name),
@@ -385,6 +399,7 @@ def main():
     log(f"timed region {elapsed:.2f} s, {value:.2f} shot-gradients/s")
     if elastic:
         out["config"]["fd_order"] = fd
+        out["config"]["abc_type"] = args.abc
         out["config"]["free_surface"] = wl.free_surface
         out["config"]["components"] = ["vx", "vz"]
     if not args.no_baselines and not elastic:
@@ -394,7 +409,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         log("CPU baseline done")
     elif not args.no_baselines:
-        out["cpu_baseline"] = elastic_cpu_baseline(wl, fd_order=fd)
+        out["cpu_baseline"] = elastic_cpu_baseline(wl, fd_order=fd, abc=args.abc)
         log("CPU baseline done")
     else:
         out["cpu_baseline"] = None
