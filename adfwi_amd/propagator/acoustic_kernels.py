@@ -11,6 +11,8 @@ checkpoints and segment recomputation.
 import ctypes
 import math
 import os
+import threading
+import weakref
 from typing import Dict, Optional
 
 import numpy as np
@@ -220,16 +222,37 @@ class AcousticGeometry:
         return coef
 
 
+class ResidentSet(dict):
+    """{segment: buffers} kept resident by a forward pass for its backward.  While it lives, the
+    memory its backward will still allocate (recompute buffers, adjoint states) is set aside from
+    the budget of later forward passes, so that several pending backward passes (losses summed
+    over batches before one backward) cannot overcommit the device."""
+    _lock = threading.Lock()
+    pending_bytes = 0
+
+    def hold(self, nbytes: int) -> "ResidentSet":
+        nbytes = int(nbytes)
+        with ResidentSet._lock:
+            ResidentSet.pending_bytes += nbytes
+
+        def release():
+            with ResidentSet._lock:
+                ResidentSet.pending_bytes -= nbytes
+        weakref.finalize(self, release)
+        return self
+
+
 def resident_budget_bytes(device, other_bytes: int) -> int:
     """Device memory that segments kept resident for the backward may take: 80 % of the free
     memory (what torch's caching allocator holds unused included: the previous iteration's
-    buffers, typically) less 8 GiB and less `other_bytes`, the checkpoints and the backward's own
-    buffers still to be allocated; ADFWI_IMAGE_BUDGET_BYTES overrides it."""
+    buffers, typically) less 8 GiB, less `other_bytes` (the checkpoints and the backward's own
+    buffers still to be allocated) and less the reserves of pending backward passes
+    (ResidentSet); ADFWI_IMAGE_BUDGET_BYTES overrides it."""
     budget = int(os.environ.get("ADFWI_IMAGE_BUDGET_BYTES", "0"))
     if not budget:
         free, _ = torch.cuda.mem_get_info(device)
         free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
-        budget = max(0, free * 4 // 5 - (8 << 30) - int(other_bytes))
+        budget = max(0, free * 4 // 5 - (8 << 30) - int(other_bytes) - ResidentSet.pending_bytes)
     return budget
 
 
@@ -283,7 +306,11 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
         energy = torch.zeros((L.ngroups, 2, Z, geom.pitch), device=dev, dtype=torch.float32)
     d = geom.desc(coef, src)
     fused = bool(geom.fuse_steps) and (rec is None or geom.rcv_map is not None)
-    images = {}
+    images = ResidentSet()
+    if resident:
+        planes = int(bool(need_a1)) + int(bool(need_a2))
+        state_bytes = 3 * S * Z * geom.pitch * 4
+        images.hold(4 * state_bytes + (planes * ckpt_every * S * Z * geom.pitch * 4 if resident < nseg else 0))
 
     def call(state_in, t0, nsteps, ckpt_buf, img, img_p, kind):
         a = nat.AcousticFwdArgs()

@@ -16,7 +16,8 @@ import numpy as np
 import torch
 
 from .. import _native as nat
-from .acoustic_kernels import PITCH_ALIGN, TIMELINE, checkpoint_interval, replicate_pad, resident_budget_bytes
+from .acoustic_kernels import (PITCH_ALIGN, TIMELINE, ResidentSet, checkpoint_interval, replicate_pad,
+                               resident_budget_bytes)
 
 NAMES = ("txx", "tzz", "txz", "vx", "vz")
 NFIELD = 10
@@ -223,7 +224,10 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
     ckpt = None
     if k and head > 0:
         ckpt = geom.state(max(nseg - 1, 1) if not resident else max(nseg - resident, 1))
-    traces = {seg: geom.state(min(nt, (seg + 1) * k) - seg * k) for seg in range(nseg - resident, nseg)}
+    traces = ResidentSet((seg, geom.state(min(nt, (seg + 1) * k) - seg * k)) for seg in range(nseg - resident, nseg))
+    if resident:
+        state_bytes = int(geom.layout.state_floats) * 4
+        traces.hold(3 * state_bytes + (k * state_bytes if resident < nseg else 0))
     rec = [torch.zeros((S, nt, R), device=dev) for _ in range(5)] if record and R else None
     d = geom.desc(coef, pml_x, pml_z, src, scalars, damp)
     ends = sorted(set(energy_ends or [])) or [nt - 1]

@@ -253,3 +253,21 @@ def test_forward_state_lookup_finds_work_checkpoint_and_trace_slots():
     assert _state_at(trace[6].data_ptr(), work, [ckpt, trace]).data_ptr() == trace[6].data_ptr()
     with pytest.raises(RuntimeError):
         _state_at(trace[6].data_ptr() + 4, work, [ckpt, trace])
+
+
+def test_pending_backward_reserves_are_counted_until_released():
+    """A forward pass that keeps resident segments sets its backward's reserve aside for later
+    budgets until its resident set is released (after its backward, or when the graph is freed)."""
+    import gc
+    from adfwi_amd.propagator.acoustic_kernels import ResidentSet
+    base = ResidentSet.pending_bytes
+    a = ResidentSet({0: None}).hold(1000)
+    b = ResidentSet().hold(24)
+    assert ResidentSet.pending_bytes == base + 1024
+    del a
+    gc.collect()
+    assert ResidentSet.pending_bytes == base + 24
+    b.pop(0, None)
+    del b
+    gc.collect()
+    assert ResidentSet.pending_bytes == base
