@@ -85,12 +85,14 @@ def footprint_bytes(req: Request) -> int:
     b += (ncoef + 16) * plane                       # coefficient planes and model-side fields
     b += 3 * 4 * S * nt                             # wavelets and their gradient
     if grad:
-        b += max(nseg - 1, 1) * state               # checkpoints
-        b += 2 * state                              # adjoint ping-pong (acoustic: + recompute pair)
+        b += 2 * state                              # adjoint ping-pong
         if req.kind == "acoustic":
-            b += 2 * state + 2 * k * S * plane      # recompute pair, stencil and pressure images
+            # every segment's stencil images stay resident from the forward pass (no checkpoints,
+            # no recompute)
+            b += nt * S * plane
             b += 2 * S * plane                      # gradient accumulators (<= one plane per shot)
         else:
+            b += max(nseg - 1, 1) * state           # checkpoints
             b += k * state + state                  # segment trace, zero state
             b += 4 * S * ncoef * plane              # gradient accumulators (<= one set per shot)
         b += ncomp * S * nt * R * 4                 # record adjoints
@@ -187,7 +189,9 @@ def build_problem(req: Request, device, shots=None, graph_steps: int = 0):
                               nabc=req.nabc, free_surface=False, device=device)
         prop = AcousticPropagator(model, survey, device=device)
         prop.graph_steps = int(graph_steps)
-        prop.resident_images = 0     # the admission footprint counts checkpoints, not image buffers
+        # gradients keep every segment's stencil images (a small request's images fit; no
+        # recompute), a fixed amount that footprint_bytes counts
+        prop.resident_images = 1 << 30
     else:
         vp, vs = elastic_layers(rng, nz, nx, 6, 1800, 4500, 200.0)
         model = IsotropicElasticModel(0, 0, nx, nz, req.dx, req.dx, vp=vp, vs=vs, rho=gardner(vp),
@@ -262,10 +266,12 @@ def _packed_acoustic_gradient(reqs: List[Request], device):
     mask = torch.tensor(real, device=device, dtype=torch.float32).unsqueeze(1)
     src = (r0.dt * prop.wavelet.to(torch.float32)) * mask
     k = checkpoint_interval(r0.nt)
-    res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k)
+    nseg = -(-r0.nt // k)
+    res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k, resident=nseg,
+                      need_a1=True, need_a2=False)
     rec_p = res["rec"][0]
     planes, _, _ = run_backward(geom, coef, src.contiguous(), res["ckpt"], k, (rec_p, None, None), True, False,
-                                False, group_planes=True)
+                                False, group_planes=True, images=res["images"])
     g = geom.gradient_group
     results = []
     for r, o in zip(reqs, offsets):
