@@ -1282,6 +1282,7 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
     const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
     // velocity totals rows k-3..k+3; stress totals txz/tzz rows k-6..k, txx rows k-3..k
     float vx[7] = {}, vz[7] = {}, xz[7] = {}, zz[7] = {}, xx[4] = {};
+    float us[7][4] = {};                         // velocity splits of rows k-3 .. k+3 as loaded
     for (int rr = z0 - 6; rr <= z0 - 1; ++rr) {
       const unsigned o = rb(rr);
       const float a = bld(rIn, vc, F(VX_X) + o) + bld(rIn, vc, F(VX_Z) + o);
@@ -1311,18 +1312,24 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
 #pragma unroll 1
     for (int kk = 0; kk < nk; ++kk) {
       const int k = z0 - 3 + kk;
-      // old velocity splits and buoyancies of row k-3, consumed at the end of the iteration
-      float u[4] = {}, bb[2] = {};
+      // buoyancies of row k-3, consumed at the end of the iteration (its old velocity splits
+      // come from the register window us)
+      float bb[2] = {};
       if (k - 3 >= z0 && k - 3 < end_row) {
         const unsigned om = rb(k - 3);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) u[q] = bld(rIn, vc, F(VX_X + q) + om);
         bb[0] = bld(rC, vc, BX * planeb + om); bb[1] = bld(rC, vc, BZ * planeb + om);
       }
       Row nxt;
       load(k + 1, nxt);
       {
         const bool ok = k + 3 <= zmax;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) us[e][q] = us[e + 1][q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) us[6][q] = cur.v[q];
         push(vx, ok ? cur.v[0] + cur.v[1] : 0.f);
         push(vz, ok ? cur.v[2] + cur.v[3] : 0.f);
       }
@@ -1404,10 +1411,10 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
         const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
         const float bxv = bb[0], bzv = bb[1];
         const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
-        const float nx_x = (pxn * u[0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
-        const float nx_z = (pzn * u[1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
-        float nz_x = (pxn * u[2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
-        float nz_z = (pzn * u[3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
+        const float nx_x = (pxn * us[0][0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
+        const float nx_z = (pzn * us[0][1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
+        float nz_x = (pxn * us[0][2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
+        float nz_z = (pzn * us[0][3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
         if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
           const float hf = 0.5f * (d.src_fz[s] * wsrc);
           nz_x = nz_x + hf;
