@@ -336,11 +336,27 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
     state = work[0].data_ptr()          # zero initial state
     if head > 0:
         state = call(state, 0, head, ckpt, None, None, "forward")
+    # The forward writes the stencil image at every point of the strips, rows [zbase, Z) x
+    # columns [0, X), at every step; only the rows above the strips and the pitch padding need
+    # clearing.  ADFWI_POISON_RESIDENT=1 fills the rest with NaN first (tests: a point the
+    # adjoint read without the forward writing it would show in the gradient).
+    zbase = int(geom.layout.zbase)
+    poison = os.environ.get("ADFWI_POISON_RESIDENT") == "1"
+
+    def image_buffer(shape):
+        img = torch.empty(shape, device=dev, dtype=torch.float32)
+        if poison:
+            img.fill_(float("nan"))
+        if zbase:
+            img[:, :, :zbase].zero_()
+        img[..., X:].zero_()
+        return img
+
     for seg in range(nseg - resident, nseg):
         t_lo = seg * ckpt_every
         n = min(nt, t_lo + ckpt_every) - t_lo
         shape = (ckpt_every, S, Z, geom.pitch)
-        img = torch.zeros(shape, device=dev, dtype=torch.float32) if need_a1 else None
+        img = image_buffer(shape) if need_a1 else None
         img_p = torch.zeros(shape, device=dev, dtype=torch.float32) if need_a2 else None
         state = call(state, t_lo, n, None, img, img_p, "forward+images")
         images[seg] = (img, img_p)

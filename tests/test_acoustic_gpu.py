@@ -280,6 +280,7 @@ def test_resident_images_equal_recomputed_segments(monkeypatch, rho_grad):
     g, m = acoustic_inputs("acoustic_fs")
     nt = 61
     out = {}
+    monkeypatch.setenv("ADFWI_POISON_RESIDENT", "1")     # unwritten image points would be NaN
     for resident in ("0", "3", "100", None):
         if resident is None:
             monkeypatch.delenv("ADFWI_RESIDENT_IMAGES", raising=False)
