@@ -224,7 +224,24 @@ def run_forward(geom: ElasticGeometry, coef, pml_x, pml_z, src, scalars, *, reco
     ckpt = None
     if k and head > 0:
         ckpt = geom.state(max(nseg - 1, 1) if not resident else max(nseg - resident, 1))
-    traces = ResidentSet((seg, geom.state(min(nt, (seg + 1) * k) - seg * k)) for seg in range(nseg - resident, nseg))
+    # A trace slot receives every window point [nn, Z-nn-1] x [nn, X-nn-1] of every field at
+    # every step (and the free-surface rows the step writes); only the border outside the window
+    # and the pitch padding need clearing.  ADFWI_POISON_RESIDENT=1 fills the rest with NaN first
+    # (tests: a point the adjoint read without the forward writing it would show in the gradient).
+    poison = os.environ.get("ADFWI_POISON_RESIDENT") == "1"
+
+    def trace_buffer(n):
+        t = torch.empty((n, geom.nfield, S, geom.Z, geom.pitch), device=dev, dtype=torch.float32)
+        if poison:
+            t.fill_(float("nan"))
+        w = geom.nn
+        t[..., :w, :].zero_()
+        t[..., geom.Z - w:, :].zero_()
+        t[..., :w].zero_()
+        t[..., geom.X - w:].zero_()
+        return t
+
+    traces = ResidentSet((seg, trace_buffer(min(nt, (seg + 1) * k) - seg * k)) for seg in range(nseg - resident, nseg))
     if resident:
         state_bytes = int(geom.layout.state_floats) * 4
         traces.hold(3 * state_bytes + (k * state_bytes if resident < nseg else 0))

@@ -326,16 +326,18 @@ def test_elastic_propagator_damping_scheme_matches_reference_vectors(name):
         assert rel_l2(got, g[f"seg1_rec_{k}"]) <= 1e-5, (k, rel_l2(got, g[f"seg1_rec_{k}"]))
 
 
-@pytest.mark.parametrize("abc", ["PML", "gerjan6"])
-def test_resident_traces_equal_recomputed_segments(monkeypatch, abc):
+@pytest.mark.parametrize("abc,fs", [("PML", True), ("gerjan", True), ("PML6", False), ("gerjan6", True),
+                                    ("sincos", False)])
+def test_resident_traces_equal_recomputed_segments(monkeypatch, abc, fs):
     """Keeping the last segments' per-step states from the forward pass instead of recomputing
     them in the backward changes nothing (the same kernel writes them): records and energy maps
     are bit-identical with none, some and all segments resident, also when the energy-map chunks
     of checkpoint_segments=3 split a resident segment, and so are the gradients (the padding's
     backward is a deterministic sum)."""
     abc, fd = _split_abc(abc)
-    g, m = elastic_random_case(30, 90, 6, 45, 3, True, seed=8, aniso=True, abc=abc, fd_order=fd)
+    g, m = elastic_random_case(30, 90, 6, 45, 3, fs, seed=8, aniso=True, abc=abc, fd_order=fd)
     out = {}
+    monkeypatch.setenv("ADFWI_POISON_RESIDENT", "1")     # unwritten trace points would be NaN
     for resident in ("0", "2", "100", None):
         if resident is None:
             monkeypatch.delenv("ADFWI_RESIDENT_IMAGES", raising=False)
