@@ -355,7 +355,7 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
     for seg in range(nseg - resident, nseg):
         t_lo = seg * ckpt_every
         n = min(nt, t_lo + ckpt_every) - t_lo
-        shape = (ckpt_every, S, Z, geom.pitch)
+        shape = (n, S, Z, geom.pitch)              # the last segment may be shorter
         img = image_buffer(shape) if need_a1 else None
         img_p = torch.zeros(shape, device=dev, dtype=torch.float32) if need_a2 else None
         state = call(state, t_lo, n, None, img, img_p, "forward+images")
