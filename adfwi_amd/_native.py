@@ -29,7 +29,7 @@ class AcousticDesc(ctypes.Structure):
                 ("nx", _i32), ("pitch", _i32), ("fs", _i32), ("free_surface", _i32), ("nt", _i32),
                 ("nrec", _i32), ("shots_per_block", _i32), ("row_unroll", _i32), ("tile_z", _i32), ("adj_shots_per_block", _i32),
                 ("coef", _vp), ("src", _vp), ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp),
-                ("rcv_x", _vp)]
+                ("rcv_x", _vp), ("coef_set", _vp)]
 
 
 class Layout(ctypes.Structure):

@@ -51,7 +51,14 @@ struct AcDev {
   // captured launch sequence serves every segment of a run; first step of the u energy map
   const int *tbase;
   int eu_from;
+  // per-shot coefficient set (NULL: one set for all shots); all shots of a group share one
+  const int *coef_set;
 };
+
+// the five coefficient planes of the shot group that starts at shot s0
+__device__ __forceinline__ const float *coef_of(const AcDev &d, int s0) {
+  return d.coef_set ? d.coef + (long long)d.coef_set[s0] * 5 * d.plane : d.coef;
+}
 
 // the time base of a forward launch (one scalar load; 0 without a device time base)
 __device__ __forceinline__ int time_base(const AcDev &d) { return d.tbase ? *d.tbase : 0; }
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_step(
   auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
   const rsrc_t rP = rsrc(in, fieldb), rU = rsrc(in + d.field, fieldb), rW = rsrc(in + 2 * d.field, fieldb);
   const rsrc_t oP = rsrc(out, fieldb), oU = rsrc(out + d.field, fieldb), oW = rsrc(out + 2 * d.field, fieldb);
-  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rC = rsrc(coef_of(d, s0), 5u * planeb);
   const rsrc_t rI = rsrc(img_d, img_d ? fieldb : 0u), rIp = rsrc(img_p, img_p ? fieldb : 0u);
   const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const unsigned vc1 = vc + fieldb, vs1 = vs + fieldb;
   const rsrc_t rPU = rsrc(io.in, 2u * fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
   const rsrc_t oPU = rsrc(io.out, 2u * fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
-  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rC = rsrc(coef_of(d, s0), 5u * planeb);
   const rsrc_t rI = rsrc(img_d0, img_d0 ? 2u * fieldb : 0u), rJ = rsrc(img_p0, img_p0 ? 2u * fieldb : 0u);
   const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(NT) void ac_adj_step(
   auto rb = [&](int k) -> unsigned { return (unsigned)min(max(k, 0), zmax) * rowb; };
   const rsrc_t rP = rsrc(lin, fieldb), rU = rsrc(lin + d.field, fieldb), rW = rsrc(lin + 2 * d.field, fieldb);
   const rsrc_t oP = rsrc(lout, fieldb), oU = rsrc(lout + d.field, fieldb), oW = rsrc(lout + 2 * d.field, fieldb);
-  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rC = rsrc(coef_of(d, s0), 5u * planeb);
   const rsrc_t rI = rsrc(img_d, img_d ? fieldb : 0u), rIp = rsrc(img_p, img_p ? fieldb : 0u);
   const rsrc_t rG1 = rsrc(grad_a1, grad_a1 ? planeb * gridDim.y : 0u);
   const rsrc_t rG2 = rsrc(grad_a2, grad_a2 ? planeb * gridDim.y : 0u);
@@ -760,7 +767,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   const unsigned vc1 = vc + fieldb, vs1 = vs + fieldb;
   const rsrc_t rPU = rsrc(io.in, 2u * fieldb), rW = rsrc(io.in + 2 * d.field, fieldb);
   const rsrc_t oPU = rsrc(io.out, 2u * fieldb), oW = rsrc(io.out + 2 * d.field, fieldb);
-  const rsrc_t rC = rsrc(d.coef, 5u * planeb);
+  const rsrc_t rC = rsrc(coef_of(d, s0), 5u * planeb);
   const rsrc_t rI = rsrc(io.img_d0, want_a1 ? 2u * fieldb : 0u), rJ = rsrc(io.img_p0, A2 ? 2u * fieldb : 0u);
   const rsrc_t rG1 = rsrc(io.grad_a1, want_a1 ? planeb * gridDim.y : 0u);
   const rsrc_t rG2 = rsrc(io.grad_a2, A2 ? planeb * gridDim.y : 0u);
@@ -1108,6 +1115,7 @@ AcDev make_dev(const adfwi_acoustic_desc *d, const adfwi_layout &L) {
   v.plane = (long long)d->nz * d->pitch;
   v.field = (long long)d->nshots * v.plane;
   v.coef = d->coef; v.src = d->src; v.src_z = d->src_z; v.src_x = d->src_x;
+  v.coef_set = d->coef_set;
   v.rcv_z = d->rcv_z; v.rcv_x = d->rcv_x;
   v.tbase = nullptr;
   v.eu_from = d->nt;

@@ -160,6 +160,7 @@ class AcousticGeometry:
         # (resident_segments), 0 = none; ADFWI_RESIDENT_IMAGES sets it when the argument is None
         env = os.environ.get("ADFWI_RESIDENT_IMAGES")
         self.resident_images = (int(env) if env is not None and resident_images is None else resident_images)
+        self.coef_set = None      # per-shot coefficient set (set_coef_sets); None: one set
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
@@ -209,7 +210,30 @@ class AcousticGeometry:
         d.src = None if src is None else src.data_ptr()
         d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
         d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
+        d.coef_set = None if self.coef_set is None else self.coef_set.data_ptr()
         return d
+
+    def set_coef_sets(self, sets) -> None:
+        """One coefficient-set index per shot (None: one set for all shots): the coefficient
+        tensor then holds [nsets][5][Z][pitch].  Every aligned group of `coef_group` shots must
+        use one set."""
+        if sets is None:
+            self.coef_set = None
+            return
+        sets = np.asarray(sets, dtype=np.int64).reshape(-1)
+        if len(sets) != self.S:
+            raise ValueError(f"coefficient sets: one per shot ({self.S}), got {len(sets)}")
+        g = self.coef_group
+        for s0 in range(0, self.S, g):
+            if len(set(sets[s0:s0 + g].tolist())) != 1:
+                raise ValueError(f"shots {s0}..{min(s0 + g, self.S) - 1} of one group of {g} use different "
+                                 "coefficient sets; pad each member to a multiple of coef_group shots")
+        self.coef_set = torch.tensor(sets.astype(np.int32), device=self.device)
+
+    @property
+    def coef_group(self) -> int:
+        """Aligned shot groups within which the kernels load one coefficient set."""
+        return max(1, int(self.layout.shots_per_block), int(self.layout.reserved))
 
     # -- buffers ----------------------------------------------------------
     def state(self, n: int = 1) -> torch.Tensor:

@@ -2,7 +2,7 @@
 it, the plain path (one request at a time, first come first served); prints one JSON line.
 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
-                                [--model-pool 6] [--graph-steps 64] [--no-pack] [--burst]
+                                [--model-pool 6 | --shape-pool 4] [--graph-steps 64] [--no-pack] [--burst]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
@@ -10,7 +10,8 @@ that of a single FIFO server with those service times and the same arrivals.  Th
 options shrink nt, the shot count and the grid of every request so that a bounded run finishes
 in minutes; the JSON line states them.  --model-pool N draws every request against one of N
 shared models (the shot requests of an inversion), so that requests of one model and mode are
-packed into shared launches (--no-pack turns packing off); --graph-steps sets the HIP-graph
+packed into shared launches (--no-pack turns packing off); --shape-pool N draws every request
+with one of N shapes but its own model, so that acoustic modelling requests of one shape pack; --graph-steps sets the HIP-graph
 window of acoustic modelling (0: direct launches).  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).
 """
@@ -43,6 +44,8 @@ def main():
     ap.add_argument("--max-nz", type=int, default=256)
     ap.add_argument("--max-nx", type=int, default=768)
     ap.add_argument("--model-pool", type=int, default=0)
+    ap.add_argument("--shape-pool", type=int, default=0,
+                    help="requests take one of N (kind, grid, nt) shapes, each with its own model")
     ap.add_argument("--graph-steps", type=int, default=64)
     ap.add_argument("--no-pack", action="store_true")
     ap.add_argument("--burst", action="store_true")
@@ -53,7 +56,8 @@ def main():
 
     def stream(rate):
         return make_stream(args.requests, rate, args.seed, nz_range=(128, args.max_nz), nx_range=(256, args.max_nx),
-                           nt_range=nt_range, shots_range=(1, args.max_shots), model_pool=args.model_pool)
+                           nt_range=nt_range, shots_range=(1, args.max_shots), model_pool=args.model_pool,
+                           shape_pool=args.shape_pool)
     # warm-up outside the measurement: one small request of each kind and mode
     warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
                        shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
@@ -70,7 +74,8 @@ def main():
            "stream": {"requests": args.requests, "seed": args.seed,
                       "mix": "60% acoustic / 40% isotropic elastic, 50% forward / 50% gradient",
                       "nz": [128, args.max_nz], "nx": [256, args.max_nx], "nt": list(nt_range),
-                      "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool},
+                      "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool,
+                      "shape_pool": args.shape_pool},
            "slots": args.slots, "packing": not args.no_pack, "graph_steps": args.graph_steps,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),

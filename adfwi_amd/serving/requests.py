@@ -72,6 +72,8 @@ def footprint_bytes(req: Request) -> int:
     pitch = -(-X // PITCH_ALIGN) * PITCH_ALIGN
     plane = 4 * Z * pitch
     S, nt, R = req.nshots, req.nt, req.nx
+    if req.kind == "acoustic" and req.mode == "forward":
+        S = -(-S // 4) * 4        # packed with other models: padded to a coefficient group
     k = checkpoint_interval(nt)
     nseg = -(-nt // k)
     grad = req.mode == "gradient"
@@ -101,16 +103,19 @@ def footprint_bytes(req: Request) -> int:
 
 def make_stream(n: int = 2000, rate: float = 1.0, seed: int = 4, *, acoustic_frac: float = 0.6,
                 gradient_frac: float = 0.5, nz_range=(128, 512), nx_range=(256, 2048), nt_range=(1000, 6000),
-                shots_range=(1, 64), nabc: int = 20, model_pool: int = 0) -> List[Request]:
+                shots_range=(1, 64), nabc: int = 20, model_pool: int = 0, shape_pool: int = 0) -> List[Request]:
     """Seeded Poisson stream: kind 60/40 acoustic/elastic, mode 50/50 forward/gradient, nz and
     nx uniform over multiples of 32 in their ranges, nt and shot count uniform.
 
     model_pool > 0: every request is drawn against one of that many shared models (kind, grid
     and nt fixed per model; mode and shot count per request), as the shot requests of an
-    inversion are; requests of one model and mode can then share launches."""
+    inversion are; requests of one model and mode can then share launches.
+    shape_pool > 0: every request takes one of that many (kind, grid, nt) shapes but its own
+    model, as requests of many users on standard survey sizes do; acoustic modelling requests
+    of one shape can then share launches."""
     rng = np.random.default_rng(seed)
     pool = []
-    for m in range(model_pool):
+    for m in range(model_pool or shape_pool):
         kind = "acoustic" if rng.random() < acoustic_frac else "elastic"
         nz = int(rng.choice(np.arange(nz_range[0], nz_range[1] + 1, 32)))
         nx = int(rng.choice(np.arange(nx_range[0], nx_range[1] + 1, 32)))
@@ -122,6 +127,8 @@ def make_stream(n: int = 2000, rate: float = 1.0, seed: int = 4, *, acoustic_fra
         t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
         if pool:
             kind, nz, nx, nt, mseed = pool[int(rng.integers(len(pool)))]
+            if not model_pool:
+                mseed = None
         else:
             kind = "acoustic" if rng.random() < acoustic_frac else "elastic"
             nz = int(rng.choice(np.arange(nz_range[0], nz_range[1] + 1, 32)))
@@ -135,10 +142,14 @@ def make_stream(n: int = 2000, rate: float = 1.0, seed: int = 4, *, acoustic_fra
 
 
 def pack_key(req: Request):
-    """Requests with equal keys can share launches: one model, grid, nt and mode.  Modelling
-    requests of either kind pack (shots are independent); gradient requests pack for the
-    acoustic propagator, whose adjoint keeps one gradient plane per shot group so that every
-    member's gradient is summed from its own groups.  None: the request runs by itself."""
+    """Requests with equal keys can share launches.  Acoustic modelling requests need only one
+    grid and nt: each member's model becomes one coefficient set of the launch, indexed per shot
+    (`_packed_acoustic_forward`).  Otherwise one model, grid, nt and mode: elastic modelling
+    requests pack (shots are independent), and acoustic gradient requests, whose adjoint keeps
+    one gradient plane per shot group so that every member's gradient is summed from its own
+    groups.  None: the request runs by itself."""
+    if req.kind == "acoustic" and req.mode == "forward":
+        return (req.kind, req.mode, req.nz, req.nx, req.nt, req.nabc)
     if req.model is None or (req.mode == "gradient" and req.kind != "acoustic"):
         return None
     return (req.kind, req.mode, req.nz, req.nx, req.nt, req.nabc, int(req.model))
@@ -284,6 +295,55 @@ def _packed_acoustic_gradient(reqs: List[Request], device):
     return results
 
 
+def _packed_acoustic_forward(reqs: List[Request], device):
+    """Acoustic modelling requests of one grid and nt against different models in shared
+    launches: each member's coefficient planes form one coefficient set, its shots (padded with
+    silent shots to a multiple of the geometry's coefficient group, so that no kernel's shot
+    group mixes models) index that set, and its records are split back out.  Records are
+    bit-identical to the member's run alone (the same coefficient construction and kernels)."""
+    import torch
+    from ..propagator.acoustic_kernels import AcousticGeometry, coefficient_fields, run_forward
+    r0 = reqs[0]
+    nabc = r0.nabc
+    probs = [build_problem(r, device) for r in reqs]
+    prop0 = probs[0][0]
+    group = 1
+    for attempt in range(2):
+        sxs, szs, sets, offsets = [], [], [], []
+        for i, r in enumerate(reqs):
+            sx, sz = shot_positions(r)
+            pad = (-len(sx)) % group
+            offsets.append(len(sxs))
+            sxs += list(sx) + [int(sx[0])] * pad
+            szs += list(sz) + [int(sz[0])] * pad
+            sets += [i] * (len(sx) + pad)
+        geom = AcousticGeometry(prop0.nz + 2 * nabc, prop0.nx + 2 * nabc, r0.nt, False, nabc,
+                                np.array(szs) + nabc, np.array(sxs) + nabc, prop0.rcv_z.cpu().numpy() + nabc,
+                                prop0.rcv_x.cpu().numpy() + nabc, device)
+        if group % geom.coef_group == 0:
+            break
+        group = geom.coef_group
+    else:
+        return None
+    geom.set_coef_sets(sets)
+    coefs = []
+    src = torch.zeros((geom.S, r0.nt), device=device, dtype=torch.float32)
+    with torch.no_grad():
+        for (prop, model), r, o in zip(probs, reqs, offsets):
+            model.forward()
+            a1, k1, a2, k2, k3 = coefficient_fields(model.vp, model.rho, prop.damp, r.nz, r.nx, nabc, r.dt, r.dx,
+                                                    False, device)
+            coefs.append(geom.pack_coef(1.0 - k1, a1, 1.0 - k2, a2, 1.0 - k3))
+            src[o:o + r.nshots] = r.dt * prop.wavelet.to(torch.float32)
+        res = run_forward(geom, torch.stack(coefs).contiguous(), src, record=True, energy_u_from=None)
+    rec_p = res["rec"][0]
+    results = []
+    for r, o in zip(reqs, offsets):
+        out = {"p": rec_p[o:o + r.nshots]}
+        results.append((out, _digest(out)))
+    return results
+
+
 def execute_pack(reqs: List[Request], device, graph_steps: int = 0):
     """Runs requests with one `pack_key` in shared launches; returns [(result, digest)] in the
     members' order.  Each result is the member's own: its records (shots are independent), or
@@ -298,6 +358,9 @@ def execute_pack(reqs: List[Request], device, graph_steps: int = 0):
     if r0.mode == "gradient":
         res = _packed_acoustic_gradient(reqs, device)
         return res if res is not None else [execute(r, device) for r in reqs]
+    if r0.kind == "acoustic" and len({r.model_seed for r in reqs}) > 1:
+        res = _packed_acoustic_forward(reqs, device)
+        return res if res is not None else [execute(r, device, graph_steps) for r in reqs]
     pos = [shot_positions(r) for r in reqs]
     shots = (np.concatenate([p[0] for p in pos]), np.concatenate([p[1] for p in pos]))
     prop, _ = build_problem(r0, device, shots=shots, graph_steps=graph_steps)

@@ -72,6 +72,12 @@ typedef struct adfwi_acoustic_desc {
   const int32_t *src_x;    /* [nshots] padded source columns */
   const int32_t *rcv_z;    /* [nrec] padded receiver rows (< nz-1) */
   const int32_t *rcv_x;    /* [nrec] padded receiver columns (< nx-1) */
+  const int32_t *coef_set; /* [nshots] or NULL: index of each shot's coefficient set, the sets
+                              stored one after another in `coef` ([nsets][5][nz][pitch]), so
+                              that shots of requests against different models share launches.
+                              Every aligned group of max(layout.shots_per_block, layout.reserved)
+                              shots must use one set.  No counterpart in the reference (one model
+                              per propagator). */
 } adfwi_acoustic_desc;
 
 typedef struct adfwi_layout {

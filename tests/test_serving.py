@@ -110,9 +110,24 @@ def test_model_pool_stream_and_pack_keys():
         k = pack_key(r)
         if r.mode == "gradient" and r.kind == "elastic":
             assert k is None
+        elif r.kind == "acoustic" and r.mode == "forward":    # any model: one coefficient set each
+            assert k == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc)
         else:
             assert k == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc, r.model)
-    assert all(pack_key(r) is None for r in make_stream(50, 5.0, seed=9))   # own models never pack
+    # own models: only acoustic modelling requests pack (by shape)
+    for r in make_stream(50, 5.0, seed=9):
+        assert (pack_key(r) is None) == (r.kind != "acoustic" or r.mode != "forward")
+
+
+def test_shape_pool_stream():
+    reqs = make_stream(300, 5.0, seed=11, shape_pool=4)
+    assert all(r.model is None for r in reqs)
+    shapes = {(r.kind, r.nz, r.nx, r.nt) for r in reqs}
+    assert len(shapes) <= 4
+    seeds = {r.model_seed for r in reqs}
+    assert len(seeds) == len(reqs)                           # every request its own model
+    fwd = [r for r in reqs if r.kind == "acoustic" and r.mode == "forward"]
+    assert len({pack_key(r) for r in fwd}) <= 4
 
 
 def test_packs_share_a_slot_and_respect_shots_and_budget():

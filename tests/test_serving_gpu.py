@@ -77,3 +77,24 @@ def test_packed_requests_equal_their_results_alone():
                 assert np.array_equal(x, y), (a.rid, k, rel_l2(x, y))
             else:
                 assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
+
+
+def test_modelling_requests_of_different_models_share_launches():
+    """Acoustic modelling requests of one grid and nt but each with its own model are packed
+    into shared launches: every member's model is one coefficient set, indexed per shot (members
+    padded to the coefficient group with silent shots).  Every member's records are
+    bit-identical to its run alone; elastic modelling requests with their own models stay alone."""
+    from adfwi_amd.serving import Request, ServingLoop, run_alone
+    spec = [("acoustic", 1), ("acoustic", 3), ("acoustic", 2), ("elastic", 2), ("acoustic", 5), ("elastic", 1)]
+    reqs = [Request(i, kind, "forward", 128, 288, 90, ns, 2000 + i, 0.0, 20, keep=True)
+            for i, (kind, ns) in enumerate(spec)]
+    rep = ServingLoop(DEV, max_slots=2, graph_steps=0).run(copy.deepcopy(reqs), realtime=False)
+    assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+    sizes = {tuple(p) for p in rep.packs}
+    assert (0, 1, 2, 4) in sizes and (3,) in sizes and (5,) in sizes, sizes
+    alone = run_alone(reqs, DEV)
+    for a, b in zip(rep.outcomes, alone):
+        assert a.rid == b.rid and a.result.keys() == b.result.keys()
+        for k in b.result:
+            x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+            assert np.array_equal(x, y), (a.rid, k, rel_l2(x, y))
