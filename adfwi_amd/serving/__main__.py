@@ -2,7 +2,8 @@
 it, the plain path (one request at a time, first come first served); prints one JSON line.
 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
-                                [--model-pool 6 | --shape-pool 4] [--graph-steps 64] [--no-pack] [--burst]
+                                [--model-pool 6 | --shape-pool 4] [--graph-steps 64] [--no-pack]
+                                [--pack-across-models] [--burst]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
@@ -11,7 +12,8 @@ options shrink nt, the shot count and the grid of every request so that a bounde
 in minutes; the JSON line states them.  --model-pool N draws every request against one of N
 shared models (the shot requests of an inversion), so that requests of one model and mode are
 packed into shared launches (--no-pack turns packing off); --shape-pool N draws every request
-with one of N shapes but its own model, so that acoustic modelling requests of one shape pack; --graph-steps sets the HIP-graph
+with one of N shapes but its own model, so that acoustic modelling requests of one shape pack
+(and, with --pack-across-models, acoustic gradient requests of one shape); --graph-steps sets the HIP-graph
 window of acoustic modelling (0: direct launches).  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).
 """
@@ -48,6 +50,8 @@ def main():
                     help="requests take one of N (kind, grid, nt) shapes, each with its own model")
     ap.add_argument("--graph-steps", type=int, default=64)
     ap.add_argument("--no-pack", action="store_true")
+    ap.add_argument("--pack-across-models", action="store_true",
+                    help="acoustic gradient requests of one grid and nt pack whatever their models")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     from .loop import ServingLoop, run_alone
@@ -76,7 +80,8 @@ def main():
                       "nz": [128, args.max_nz], "nx": [256, args.max_nx], "nt": list(nt_range),
                       "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool,
                       "shape_pool": args.shape_pool},
-           "slots": args.slots, "packing": not args.no_pack, "graph_steps": args.graph_steps,
+           "slots": args.slots, "packing": not args.no_pack, "pack_across_models": args.pack_across_models,
+           "graph_steps": args.graph_steps,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),
            "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
@@ -89,7 +94,7 @@ def main():
             for r in reqs:
                 r.arrival = 0.0
         rep = ServingLoop("cuda:0", max_slots=args.slots, pack=not args.no_pack,
-                          graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
+                          pack_across_models=args.pack_across_models, graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
         s = rep.summary()
         fifo = fifo_latencies([r.arrival for r in reqs], service)
         same = all(o.digest.keys() == digests[o.rid].keys() and all(

@@ -67,11 +67,15 @@ class StreamReport:
 
 class ServingLoop:
     """Serves a list of requests on `device` with at most `max_slots` in flight and at most
-    `budget_bytes` of admitted footprint (default: 85 % of the free device memory)."""
+    `budget_bytes` of admitted footprint (default: 85 % of the free device memory).
+    `pack_across_models`: acoustic gradient requests of one grid and nt pack whatever their
+    models (off by default: on bursts of small requests the fewer, longer packs balance worse
+    over the slots than the requests alone)."""
 
     def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
                  footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4,
-                 pack: bool = True, max_pack_shots: int = 64, graph_steps: int = 64):
+                 pack: bool = True, max_pack_shots: int = 64, graph_steps: int = 64,
+                 pack_across_models: bool = False):
         import torch
         from .. import _native as nat
         nat.require_gpu(device)
@@ -83,7 +87,8 @@ class ServingLoop:
             if graph_steps:       # the graph contexts each worker thread keeps
                 from ..propagator import graphs
                 budget_bytes -= max_slots * graphs.MAX_CONTEXTS * graphs.MAX_CONTEXT_BYTES
-        self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, pack_key if pack else None, max_pack_shots)
+        key = (lambda r: pack_key(r, True)) if pack_across_models else pack_key
+        self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, key if pack else None, max_pack_shots)
         self.footprint = footprint
         self.max_slots = max_slots
         self.graph_steps = int(graph_steps)

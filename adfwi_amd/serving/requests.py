@@ -72,7 +72,7 @@ def footprint_bytes(req: Request) -> int:
     pitch = -(-X // PITCH_ALIGN) * PITCH_ALIGN
     plane = 4 * Z * pitch
     S, nt, R = req.nshots, req.nt, req.nx
-    if req.kind == "acoustic" and req.mode == "forward":
+    if req.kind == "acoustic":
         S = -(-S // 4) * 4        # packed with other models: padded to a coefficient group
     k = checkpoint_interval(nt)
     nseg = -(-nt // k)
@@ -141,14 +141,15 @@ def make_stream(n: int = 2000, rate: float = 1.0, seed: int = 4, *, acoustic_fra
     return out
 
 
-def pack_key(req: Request):
+def pack_key(req: Request, gradients_across_models: bool = False):
     """Requests with equal keys can share launches.  Acoustic modelling requests need only one
     grid and nt: each member's model becomes one coefficient set of the launch, indexed per shot
-    (`_packed_acoustic_forward`).  Otherwise one model, grid, nt and mode: elastic modelling
-    requests pack (shots are independent), and acoustic gradient requests, whose adjoint keeps
-    one gradient plane per shot group so that every member's gradient is summed from its own
-    groups.  None: the request runs by itself."""
-    if req.kind == "acoustic" and req.mode == "forward":
+    (`_packed_acoustic_forward`).  Acoustic gradient requests pack against one model, or with
+    `gradients_across_models` against any models of one grid and nt: the adjoint keeps one
+    gradient plane per shot group, so that every member's gradient is summed from its own groups
+    (`_packed_acoustic_gradient`).  Elastic modelling requests pack against one model (shots are
+    independent); elastic gradients run alone.  None: the request runs by itself."""
+    if req.kind == "acoustic" and (req.mode == "forward" or gradients_across_models):
         return (req.kind, req.mode, req.nz, req.nx, req.nt, req.nabc)
     if req.model is None or (req.mode == "gradient" and req.kind != "acoustic"):
         return None
@@ -237,45 +238,64 @@ def _digest(out):
 
 
 def _packed_acoustic_gradient(reqs: List[Request], device):
-    """Gradients of acoustic requests against one model in shared launches.
+    """Gradients of acoustic requests of one grid and nt in shared launches.
 
     Every member's shots are padded with silent shots (zero wavelet: their wavefield, records,
-    adjoint and gradient terms are exactly zero) to a multiple of the adjoint's gradient group,
-    so that each accumulator plane holds one member's shots only; a member's coefficient
-    gradient is the sum of its own planes, carried to vp through the coefficient construction.
-    Returns None when the layout does not give aligned groups (the members then run alone)."""
+    adjoint and gradient terms are exactly zero) to a multiple of the adjoint's gradient group
+    (of the coefficient group when the members have different models, each model then being one
+    coefficient set indexed per shot), so that each accumulator plane holds one member's shots
+    only; a member's coefficient gradient is the sum of its own planes, carried to vp through its
+    model's coefficient construction.  Returns None when the layout does not give aligned groups
+    (the members then run alone)."""
     import torch
     from ..propagator.acoustic_kernels import (AcousticGeometry, checkpoint_interval, coefficient_fields,
                                                run_backward, run_forward)
     r0 = reqs[0]
+    shared = len({r.model_seed for r in reqs}) == 1
     sxs, szs, real = [], [], []
     group = 2
     for attempt in range(2):
-        sxs, szs, real, offsets = [], [], [], []
-        for r in reqs:
+        sxs, szs, real, offsets, sets = [], [], [], [], []
+        for i, r in enumerate(reqs):
             sx, sz = shot_positions(r)
             pad = (-len(sx)) % group
             offsets.append(len(real))
             sxs += list(sx) + [int(sx[0])] * pad
             szs += list(sz) + [int(sz[0])] * pad
             real += [True] * len(sx) + [False] * pad
+            sets += [i] * (len(sx) + pad)
         prop, model = build_problem(r0, device, shots=(np.array(sxs), np.array(szs)))
         nabc = r0.nabc
         geom = AcousticGeometry(prop.nz + 2 * nabc, prop.nx + 2 * nabc, r0.nt, False, nabc,
                                 np.array(szs) + nabc, np.array(sxs) + nabc,
                                 prop.rcv_z.cpu().numpy() + nabc, prop.rcv_x.cpu().numpy() + nabc, device)
-        g = geom.gradient_group
-        if group % g == 0:
+        g = geom.gradient_group if shared else geom.coef_group
+        if group % g == 0 and group % geom.gradient_group == 0:
             break
         group = g
     else:
         return None
-    model.forward()
-    a1, k1, a2, k2, k3 = coefficient_fields(model.vp, model.rho, prop.damp, r0.nz, r0.nx, nabc, r0.dt, r0.dx,
-                                            False, device)
-    coef = geom.pack_coef(1.0 - k1, a1.detach(), 1.0 - k2, a2.detach(), 1.0 - k3)
     mask = torch.tensor(real, device=device, dtype=torch.float32).unsqueeze(1)
-    src = (r0.dt * prop.wavelet.to(torch.float32)) * mask
+    if shared:
+        model.forward()
+        a1, k1, a2, k2, k3 = coefficient_fields(model.vp, model.rho, prop.damp, r0.nz, r0.nx, nabc, r0.dt, r0.dx,
+                                                False, device)
+        coef = geom.pack_coef(1.0 - k1, a1.detach(), 1.0 - k2, a2.detach(), 1.0 - k3)
+        src = (r0.dt * prop.wavelet.to(torch.float32)) * mask
+        members = [(a1, model)] * len(reqs)
+    else:
+        geom.set_coef_sets(sets)
+        coefs, members = [], []
+        src = torch.zeros((geom.S, r0.nt), device=device, dtype=torch.float32)
+        for r, o in zip(reqs, offsets):
+            p_r, m_r = build_problem(r, device)
+            m_r.forward()
+            a1, k1, a2, k2, k3 = coefficient_fields(m_r.vp, m_r.rho, p_r.damp, r.nz, r.nx, nabc, r.dt, r.dx,
+                                                    False, device)
+            coefs.append(geom.pack_coef(1.0 - k1, a1.detach(), 1.0 - k2, a2.detach(), 1.0 - k3))
+            src[o:o + r.nshots] = r.dt * p_r.wavelet.to(torch.float32)
+            members.append((a1, m_r))
+        coef = torch.stack(coefs).contiguous()
     k = checkpoint_interval(r0.nt)
     nseg = -(-r0.nt // k)
     res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k, resident=nseg,
@@ -285,10 +305,10 @@ def _packed_acoustic_gradient(reqs: List[Request], device):
                                 False, group_planes=True, images=res["images"])
     g = geom.gradient_group
     results = []
-    for r, o in zip(reqs, offsets):
+    for r, o, (a1, m_r) in zip(reqs, offsets, members):
         n = r.nshots + (-r.nshots) % group
         ga1 = planes[o // g:(o + n) // g].sum(0)
-        grad, = torch.autograd.grad(a1, model.vp, ga1, retain_graph=True)
+        grad, = torch.autograd.grad(a1, m_r.vp, ga1, retain_graph=True)
         rp = rec_p[o:o + r.nshots]
         out = {"vp": grad, "loss": (0.5 * rp.square().sum()).detach().reshape(1)}
         results.append((out, _digest(out)))
@@ -352,8 +372,8 @@ def execute_pack(reqs: List[Request], device, graph_steps: int = 0):
     if len(reqs) == 1:
         return [execute(reqs[0], device, graph_steps)]
     r0 = reqs[0]
-    key = pack_key(r0)
-    if key is None or any(pack_key(r) != key for r in reqs):
+    key = pack_key(r0, True)      # the widest packing the launches support
+    if key is None or any(pack_key(r, True) != key for r in reqs):
         raise ValueError("requests of a pack must share one pack key")
     if r0.mode == "gradient":
         res = _packed_acoustic_gradient(reqs, device)

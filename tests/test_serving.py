@@ -114,9 +114,12 @@ def test_model_pool_stream_and_pack_keys():
             assert k == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc)
         else:
             assert k == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc, r.model)
-    # own models: only acoustic modelling requests pack (by shape)
+        if r.kind == "acoustic":  # gradients too when packed across models
+            assert pack_key(r, True) == (r.kind, r.mode, r.nz, r.nx, r.nt, r.nabc)
+    # own models: acoustic modelling requests pack by shape, gradients only across models
     for r in make_stream(50, 5.0, seed=9):
         assert (pack_key(r) is None) == (r.kind != "acoustic" or r.mode != "forward")
+        assert (pack_key(r, True) is None) == (r.kind != "acoustic")
 
 
 def test_shape_pool_stream():
@@ -126,8 +129,9 @@ def test_shape_pool_stream():
     assert len(shapes) <= 4
     seeds = {r.model_seed for r in reqs}
     assert len(seeds) == len(reqs)                           # every request its own model
-    fwd = [r for r in reqs if r.kind == "acoustic" and r.mode == "forward"]
-    assert len({pack_key(r) for r in fwd}) <= 4
+    for mode in ("forward", "gradient"):
+        ac = [r for r in reqs if r.kind == "acoustic" and r.mode == mode]
+        assert len({pack_key(r, True) for r in ac}) <= 4
 
 
 def test_packs_share_a_slot_and_respect_shots_and_budget():

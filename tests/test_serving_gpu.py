@@ -79,11 +79,14 @@ def test_packed_requests_equal_their_results_alone():
                 assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
 
 
-def test_modelling_requests_of_different_models_share_launches():
+def test_requests_of_different_models_share_launches():
     """Acoustic modelling requests of one grid and nt but each with its own model are packed
     into shared launches: every member's model is one coefficient set, indexed per shot (members
     padded to the coefficient group with silent shots).  Every member's records are
-    bit-identical to its run alone; elastic modelling requests with their own models stay alone."""
+    bit-identical to its run alone; elastic modelling requests with their own models stay alone.
+    Acoustic gradient requests of different models pack the same way when the loop packs them
+    across models: every member's gradient and loss equal its run alone within 1e-6 (the pack's
+    gradient planes group shots differently)."""
     from adfwi_amd.serving import Request, ServingLoop, run_alone
     spec = [("acoustic", 1), ("acoustic", 3), ("acoustic", 2), ("elastic", 2), ("acoustic", 5), ("elastic", 1)]
     reqs = [Request(i, kind, "forward", 128, 288, 90, ns, 2000 + i, 0.0, 20, keep=True)
@@ -98,3 +101,18 @@ def test_modelling_requests_of_different_models_share_launches():
         for k in b.result:
             x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
             assert np.array_equal(x, y), (a.rid, k, rel_l2(x, y))
+
+    spec = [("acoustic", 3), ("acoustic", 1), ("elastic", 1), ("acoustic", 2)]
+    reqs = [Request(i, kind, "gradient", 96, 224, 80, ns, 3000 + i, 0.0, 20, keep=True)
+            for i, (kind, ns) in enumerate(spec)]
+    rep = ServingLoop(DEV, max_slots=2, graph_steps=0, pack_across_models=True).run(copy.deepcopy(reqs),
+                                                                                  realtime=False)
+    assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+    sizes = {tuple(p) for p in rep.packs}
+    assert (0, 1, 3) in sizes and (2,) in sizes, sizes
+    alone = run_alone(reqs, DEV)
+    for a, b in zip(rep.outcomes, alone):
+        assert a.rid == b.rid and a.result.keys() == b.result.keys()
+        for k in b.result:
+            x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+            assert np.abs(y).max() > 0 and rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
