@@ -19,6 +19,39 @@ from ..propagator.acoustic_kernels import PITCH_ALIGN, checkpoint_interval
 
 KINDS = ("acoustic", "elastic")
 MODES = ("forward", "gradient")
+# an acoustic gradient request keeps every step's stencil images from its forward pass (no
+# checkpoints, no recompute) when they take at most this many bytes; larger requests keep
+# sqrt(nt) checkpoints and recompute each segment's images in the backward
+RESIDENT_IMAGE_CAP_BYTES = 32 << 30
+# a gradient request whose footprint would exceed this runs its shots in equal chunks, one
+# forward and backward per chunk, so that two of the largest requests fit one GPU together
+REQUEST_CAP_BYTES = 128 << 30
+
+
+def _padded_shots(req: "Request", shots: Optional[int] = None) -> int:
+    s = req.nshots if shots is None else int(shots)
+    return -(-s // 4) * 4 if req.kind == "acoustic" else s
+
+
+def images_resident(req: "Request", shots: Optional[int] = None) -> bool:
+    """Whether an acoustic gradient launch of `shots` shots (default: the request's, in its shot
+    chunks) keeps all its stencil images resident."""
+    Z, X = req.padded
+    pitch = -(-X // PITCH_ALIGN) * PITCH_ALIGN
+    if shots is None:
+        shots = shot_chunk(req)
+    return req.nt * _padded_shots(req, shots) * 4 * Z * pitch <= RESIDENT_IMAGE_CAP_BYTES
+
+
+def shot_chunk(req: "Request") -> int:
+    """Shots per launch: all of the request's, except for a gradient request whose footprint
+    would exceed REQUEST_CAP_BYTES; that one runs in chunks of at most this many shots (halved
+    until one chunk fits), its gradients summed over the chunks."""
+    n = req.nshots
+    if req.mode == "gradient":
+        while n > 1 and _footprint(req, n) > REQUEST_CAP_BYTES:
+            n = -(-n // 2)
+    return n
 
 
 @dataclass
@@ -67,13 +100,17 @@ class Request:
 def footprint_bytes(req: Request) -> int:
     """Upper bound of the device memory one request holds while it runs: state and work sets,
     sqrt(nt) checkpoints, the segment images (acoustic) or trace (elastic), records, their
-    adjoints, coefficient planes, gradient accumulators and model-side tensors."""
+    adjoints, coefficient planes, gradient accumulators and model-side tensors.  A gradient
+    request too large for one launch counts one of its shot chunks (`shot_chunk`)."""
+    return _footprint(req, shot_chunk(req))
+
+
+def _footprint(req: Request, shots: int) -> int:
+    """footprint_bytes of one launch of `shots` of the request's shots."""
     Z, X = req.padded
     pitch = -(-X // PITCH_ALIGN) * PITCH_ALIGN
     plane = 4 * Z * pitch
-    S, nt, R = req.nshots, req.nt, req.nx
-    if req.kind == "acoustic":
-        S = -(-S // 4) * 4        # packed with other models: padded to a coefficient group
+    S, nt, R = _padded_shots(req, shots), req.nt, req.nx   # acoustic: padded to a coefficient group
     k = checkpoint_interval(nt)
     nseg = -(-nt // k)
     grad = req.mode == "gradient"
@@ -89,9 +126,13 @@ def footprint_bytes(req: Request) -> int:
     if grad:
         b += 2 * state                              # adjoint ping-pong
         if req.kind == "acoustic":
-            # every segment's stencil images stay resident from the forward pass (no checkpoints,
-            # no recompute)
-            b += nt * S * plane
+            if images_resident(req, shots):
+                # every segment's stencil images stay resident from the forward pass (no
+                # checkpoints, no recompute)
+                b += nt * S * plane
+            else:
+                b += max(nseg - 1, 1) * state       # checkpoints
+                b += 2 * state + 2 * k * S * plane  # recompute pair, stencil and pressure images
             b += 2 * S * plane                      # gradient accumulators (<= one plane per shot)
         else:
             b += max(nseg - 1, 1) * state           # checkpoints
@@ -148,7 +189,10 @@ def pack_key(req: Request, gradients_across_models: bool = False):
     `gradients_across_models` against any models of one grid and nt: the adjoint keeps one
     gradient plane per shot group, so that every member's gradient is summed from its own groups
     (`_packed_acoustic_gradient`).  Elastic modelling requests pack against one model (shots are
-    independent); elastic gradients run alone.  None: the request runs by itself."""
+    independent); elastic gradients, and gradient requests that run in shot chunks, run alone.
+    None: the request runs by itself."""
+    if req.mode == "gradient" and shot_chunk(req) < req.nshots:
+        return None
     if req.kind == "acoustic" and (req.mode == "forward" or gradients_across_models):
         return (req.kind, req.mode, req.nz, req.nx, req.nt, req.nabc)
     if req.model is None or (req.mode == "gradient" and req.kind != "acoustic"):
@@ -201,9 +245,10 @@ def build_problem(req: Request, device, shots=None, graph_steps: int = 0):
                               nabc=req.nabc, free_surface=False, device=device)
         prop = AcousticPropagator(model, survey, device=device)
         prop.graph_steps = int(graph_steps)
-        # gradients keep every segment's stencil images (a small request's images fit; no
-        # recompute), a fixed amount that footprint_bytes counts
-        prop.resident_images = 1 << 30
+        # gradients keep every segment's stencil images when they fit under the cap (no
+        # recompute), otherwise none (checkpoints and recompute): fixed amounts that
+        # footprint_bytes counts
+        prop.resident_images = (1 << 30) if images_resident(req) else 0
     else:
         vp, vs = elastic_layers(rng, nz, nx, 6, 1800, 4500, 200.0)
         model = IsotropicElasticModel(0, 0, nx, nz, req.dx, req.dx, vp=vp, vs=vs, rho=gardner(vp),
@@ -225,11 +270,17 @@ def execute(req: Request, device, graph_steps: int = 0):
             rec = prop.forward()
         out = {k: rec[k] for k in comps}
     else:
-        rec = prop.forward()
-        loss = 0.5 * sum(rec[k].square().sum() for k in comps)
-        loss.backward()
+        chunk = shot_chunk(req)
+        loss = 0.0
+        for s0 in range(0, req.nshots, chunk):     # one launch, or equal shot chunks
+            shots = None if chunk == req.nshots else np.arange(s0, min(s0 + chunk, req.nshots))
+            rec = prop.forward(shot_index=shots)
+            part = 0.5 * sum(rec[k].square().sum() for k in comps)
+            part.backward()                        # the parameters' .grad sum over the chunks
+            loss = loss + part.detach()
+            del rec, part
         out = {n: p.grad for n, p in model.named_parameters() if p.grad is not None}
-        out["loss"] = loss.detach().reshape(1)
+        out["loss"] = loss.reshape(1)
     return out, _digest(out)
 
 
@@ -297,9 +348,10 @@ def _packed_acoustic_gradient(reqs: List[Request], device):
             members.append((a1, m_r))
         coef = torch.stack(coefs).contiguous()
     k = checkpoint_interval(r0.nt)
-    nseg = -(-r0.nt // k)
-    res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k, resident=nseg,
-                      need_a1=True, need_a2=False)
+    # the members' footprints count their images when every member keeps them, else checkpoints
+    resident = -(-r0.nt // k) if all(images_resident(r) for r in reqs) else 0
+    res = run_forward(geom, coef, src.contiguous(), record=True, energy_u_from=None, ckpt_every=k,
+                      resident=resident, need_a1=True, need_a2=False)
     rec_p = res["rec"][0]
     planes, _, _ = run_backward(geom, coef, src.contiguous(), res["ckpt"], k, (rec_p, None, None), True, False,
                                 False, group_planes=True, images=res["images"])

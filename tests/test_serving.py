@@ -122,6 +122,31 @@ def test_model_pool_stream_and_pack_keys():
         assert (pack_key(r, True) is None) == (r.kind != "acoustic")
 
 
+def test_full_size_gradient_requests_fit_one_gpu():
+    """Acoustic gradient requests keep all their stencil images only under the cap, and gradient
+    requests whose footprint exceeds the request cap run in shot chunks: the largest requests of
+    the full-size mix (nz 512, nx 2048, nt 6000, 64 shots) fall back to checkpoints and
+    recompute, elastic gradients of that size run in chunks of 16 shots (one launch of all 64
+    would need 489 GiB), and every request's footprint fits the 288 GB of one MI355X."""
+    from adfwi_amd.serving.requests import (REQUEST_CAP_BYTES, RESIDENT_IMAGE_CAP_BYTES, _footprint,
+                                            images_resident, shot_chunk)
+    small = Request(0, "acoustic", "gradient", 128, 256, 200, 3, 1)
+    big = Request(1, "acoustic", "gradient", 512, 2048, 6000, 64, 2)
+    assert images_resident(small) and not images_resident(big)
+    Z, X = big.padded
+    assert big.nt * 64 * Z * X * 4 > RESIDENT_IMAGE_CAP_BYTES
+    assert shot_chunk(small) == 3 and pack_key(small, True) is not None
+    el = Request(2, "elastic", "gradient", 512, 2048, 6000, 64, 3)
+    assert _footprint(el, 64) > 400 * 2**30 and shot_chunk(el) == 16 and pack_key(el, True) is None
+    for kind in ("acoustic", "elastic"):
+        for mode in ("forward", "gradient"):
+            r = Request(2, kind, mode, 512, 2048, 6000, 64, 3)
+            assert footprint_bytes(r) <= max(REQUEST_CAP_BYTES, _footprint(r, 1)), (kind, mode)
+            assert (shot_chunk(r) < 64) == (mode == "gradient"), (kind, mode)
+    for r in make_stream(300, 1.0, seed=4):
+        assert footprint_bytes(r) <= REQUEST_CAP_BYTES, r
+
+
 def test_shape_pool_stream():
     reqs = make_stream(300, 5.0, seed=11, shape_pool=4)
     assert all(r.model is None for r in reqs)

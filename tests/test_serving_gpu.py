@@ -116,3 +116,32 @@ def test_requests_of_different_models_share_launches():
         for k in b.result:
             x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
             assert np.abs(y).max() > 0 and rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
+
+
+def test_large_requests_run_in_shot_chunks_within_their_footprint(monkeypatch):
+    """With the caps lowered, gradient requests run in shot chunks (their parameters' gradients
+    summed over the chunks) and acoustic ones recompute their segments instead of keeping every
+    stencil image: gradients and losses equal the single-launch resident runs within 1e-6, and
+    the peak allocation stays within the (chunk's) footprint."""
+    from adfwi_amd.serving import Request, execute, footprint_bytes
+    from adfwi_amd.serving import requests as rq
+    reqs = [Request(0, "acoustic", "gradient", 128, 288, 120, 7, 41, keep=True),
+            Request(1, "elastic", "gradient", 128, 256, 100, 5, 42, keep=True)]
+    whole = [execute(r, DEV)[0] for r in reqs]
+    fp_whole = [footprint_bytes(r) for r in reqs]
+    monkeypatch.setattr(rq, "RESIDENT_IMAGE_CAP_BYTES", 0)
+    monkeypatch.setattr(rq, "REQUEST_CAP_BYTES", min(fp_whole) // 2)
+    for r, w in zip(reqs, whole):
+        n = rq.shot_chunk(r)
+        assert 1 <= n < r.nshots and not rq.images_resident(r), (r.kind, n)
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        out, _ = execute(r, DEV)
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - base
+        assert peak <= footprint_bytes(r), (r.kind, peak, footprint_bytes(r))
+        assert out.keys() == w.keys()
+        for k in w:
+            x, y = out[k].detach().cpu().numpy(), w[k].detach().cpu().numpy()
+            assert np.abs(y).max() > 0 and rel_l2(x, y) <= 1e-6, (r.kind, k, rel_l2(x, y))
