@@ -82,7 +82,10 @@ class ServingLoop:
         nat.load()
         self.device = torch.device(device)
         if budget_bytes is None:
+            # free device memory, including what torch's allocator holds cached but unused (an
+            # earlier loop's buffers): requests allocate through that cache
             free, _ = torch.cuda.mem_get_info(self.device)
+            free += torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
             budget_bytes = int(0.85 * free)
             if graph_steps:       # the graph contexts each worker thread keeps
                 from ..propagator import graphs
