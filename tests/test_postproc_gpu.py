@@ -21,11 +21,15 @@ def _records(seed, S=3, nt=240, R=37):
     return rec.astype(np.float32), obs.astype(np.float32)
 
 
+@pytest.mark.parametrize("shape", [(3, 240, 37), (2, 5, 130), (1, 1001, 64)])
 @pytest.mark.parametrize("normalize", [False, True])
-def test_trace_l2_kernel_matches_oracle(normalize):
+def test_trace_l2_kernel_matches_oracle(normalize, shape):
+    """Shapes: receivers fewer than, more than and exactly one 64-lane group; nt shorter than
+    the number of time chunks and not a multiple of it."""
     from adfwi_amd.fwi import Misfit_waveform_L2, NormalizedWaveformL2
     from oracle import misfit as om
-    rec, obs = _records(5)
+    S, nt, R = shape
+    rec, obs = _records(5, S, nt, R)
     dt = 0.002
     obs_n = obs / np.abs(obs).max(axis=1, keepdims=True)
     a = torch.tensor(rec, device=DEV, requires_grad=True)
