@@ -3,7 +3,7 @@ it, the plain path (one request at a time, first come first served); prints one 
 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
                                 [--model-pool 6 | --shape-pool 4] [--graph-steps 64] [--no-pack]
-                                [--pack-across-models] [--burst]
+                                [--pack-across-models] [--exclusive-s 1.0] [--burst]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
@@ -14,7 +14,8 @@ shared models (the shot requests of an inversion), so that requests of one model
 packed into shared launches (--no-pack turns packing off); --shape-pool N draws every request
 with one of N shapes but its own model, so that acoustic modelling requests of one shape pack
 (and, with --pack-across-models, acoustic gradient requests of one shape); --graph-steps sets the HIP-graph
-window of acoustic modelling (0: direct launches).  --burst submits the whole stream at once
+window of acoustic modelling (0: direct launches).  --exclusive-s S gives every request
+estimated to take longer than S seconds alone the device to itself.  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).
 """
 import argparse
@@ -52,6 +53,8 @@ def main():
     ap.add_argument("--no-pack", action="store_true")
     ap.add_argument("--pack-across-models", action="store_true",
                     help="acoustic gradient requests of one grid and nt pack whatever their models")
+    ap.add_argument("--exclusive-s", type=float, default=None,
+                    help="requests estimated to take longer than this alone run with the device to themselves")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     from .loop import ServingLoop, run_alone
@@ -81,6 +84,7 @@ def main():
                       "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool,
                       "shape_pool": args.shape_pool},
            "slots": args.slots, "packing": not args.no_pack, "pack_across_models": args.pack_across_models,
+           "exclusive_s": args.exclusive_s,
            "graph_steps": args.graph_steps,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),
@@ -94,7 +98,8 @@ def main():
             for r in reqs:
                 r.arrival = 0.0
         rep = ServingLoop("cuda:0", max_slots=args.slots, pack=not args.no_pack,
-                          pack_across_models=args.pack_across_models, graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
+                          pack_across_models=args.pack_across_models, exclusive_s=args.exclusive_s,
+                          graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
         s = rep.summary()
         fifo = fifo_latencies([r.arrival for r in reqs], service)
         same = all(o.digest.keys() == digests[o.rid].keys() and all(

@@ -25,7 +25,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from .requests import Request, execute, execute_pack, footprint_bytes, pack_key
+from .requests import Request, execute, execute_pack, footprint_bytes, pack_key, service_estimate_s
 from .scheduler import Scheduler
 
 
@@ -70,12 +70,14 @@ class ServingLoop:
     `budget_bytes` of admitted footprint (default: 85 % of the free device memory).
     `pack_across_models`: acoustic gradient requests of one grid and nt pack whatever their
     models (off by default: on bursts of small requests the fewer, longer packs balance worse
-    over the slots than the requests alone)."""
+    over the slots than the requests alone).  `exclusive_s`: a request estimated to take longer
+    than this alone (`service_estimate_s`) is charged the whole budget, so it starts when the
+    device is idle and runs alone (None: every request shares the device)."""
 
     def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
                  footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4,
                  pack: bool = True, max_pack_shots: int = 64, graph_steps: int = 64,
-                 pack_across_models: bool = False):
+                 pack_across_models: bool = False, exclusive_s: Optional[float] = None):
         import torch
         from .. import _native as nat
         nat.require_gpu(device)
@@ -93,6 +95,7 @@ class ServingLoop:
         key = (lambda r: pack_key(r, True)) if pack_across_models else pack_key
         self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, key if pack else None, max_pack_shots)
         self.footprint = footprint
+        self.exclusive_s = exclusive_s
         self.max_slots = max_slots
         self.graph_steps = int(graph_steps)
         self._local = threading.local()
@@ -139,7 +142,10 @@ class ServingLoop:
                 now = time.perf_counter() - t0
                 while i < len(pending) and pending[i].arrival <= now:
                     r = pending[i]
-                    if not self.scheduler.submit(r, self.footprint(r)):
+                    nbytes = self.footprint(r)
+                    if self.exclusive_s is not None and service_estimate_s(r) > self.exclusive_s:
+                        nbytes = max(nbytes, self.scheduler.budget)   # runs alone
+                    if not self.scheduler.submit(r, nbytes):
                         outcomes.append(Outcome(r.rid, r.arrival, now, now, {}, None,
                                                 self.scheduler.rejected[-1][1]))
                     i += 1

@@ -97,6 +97,18 @@ class Request:
         return self.nz + self.nabc + 2, self.nx + 2 * self.nabc
 
 
+# service-time model of one request on one MI355X: padded points x nt x shots at the measured
+# rate of each kind and mode (the 4th-order kernels of the benchmark workloads, rounded)
+_PT_STEPS_PER_S = {("acoustic", "forward"): 230e9, ("acoustic", "gradient"): 66e9,
+                   ("elastic", "forward"): 30e9, ("elastic", "gradient"): 10e9}
+
+
+def service_estimate_s(req: Request) -> float:
+    """Seconds one request takes alone on the device (a rough model, for scheduling)."""
+    Z, X = req.padded
+    return Z * X * req.nt * req.nshots / _PT_STEPS_PER_S[(req.kind, req.mode)]
+
+
 def footprint_bytes(req: Request) -> int:
     """Upper bound of the device memory one request holds while it runs: state and work sets,
     sqrt(nt) checkpoints, the segment images (acoustic) or trace (elastic), records, their

@@ -180,3 +180,28 @@ def test_packs_share_a_slot_and_respect_shots_and_budget():
     packs = s.admit_packs()   # 3 leads a new pack with 5 (2 + 2 shots, 85 B fits in 90 B); no slot for 4
     assert [[r.rid for r in p] for p in packs] == [[3, 5]]
     assert s.used == 95 and [w.req.rid for w in s.waiting] == [4]
+
+
+def test_exclusive_requests_run_alone():
+    """A request charged the whole budget (the loop's exclusive policy for long requests) starts
+    only when nothing runs and blocks admission while it runs; smaller ones pass it at most
+    max_bypass times."""
+    from adfwi_amd.serving.requests import service_estimate_s
+    s = Scheduler(budget_bytes=100, max_slots=4, max_bypass=2)
+    s.submit(_req(0), 10)
+    assert [r.rid for r in s.admit()] == [0]
+    s.submit(_req(1), 100)                   # exclusive
+    for rid in (2, 3, 4):
+        s.submit(_req(rid), 10)
+    assert [r.rid for r in s.admit()] == [2, 3]   # 1 passed twice, then it blocks the queue
+    s.finish(0)
+    s.finish(2)
+    assert s.admit() == []
+    s.finish(3)
+    assert [r.rid for r in s.admit()] == [1]
+    assert s.admit() == [] and s.used == 100
+    s.finish(1)
+    assert [r.rid for r in s.admit()] == [4]
+    small = Request(0, "acoustic", "forward", 128, 256, 600, 4, 0)
+    big = Request(1, "elastic", "gradient", 512, 2048, 6000, 64, 0)
+    assert service_estimate_s(small) < 0.01 < 10 < service_estimate_s(big)
