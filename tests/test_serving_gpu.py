@@ -145,3 +145,19 @@ def test_large_requests_run_in_shot_chunks_within_their_footprint(monkeypatch):
         for k in w:
             x, y = out[k].detach().cpu().numpy(), w[k].detach().cpu().numpy()
             assert np.abs(y).max() > 0 and rel_l2(x, y) <= 1e-6, (r.kind, k, rel_l2(x, y))
+
+
+def test_exclusive_requests_run_alone_on_the_device():
+    """With exclusive_s = 0 every request is charged the whole budget: the loop runs them one
+    at a time, and every result equals its run alone."""
+    from adfwi_amd.serving import ServingLoop, run_alone
+    reqs = _small_stream(5, seed=13)
+    rep = ServingLoop(DEV, max_slots=3, exclusive_s=0.0).run(copy.deepcopy(reqs), realtime=False)
+    assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+    assert rep.max_concurrent == 1 and not rep.rejected
+    alone = run_alone(reqs, DEV)
+    for a, b in zip(rep.outcomes, alone):
+        assert a.rid == b.rid and a.result.keys() == b.result.keys()
+        for k in b.result:
+            x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+            assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
