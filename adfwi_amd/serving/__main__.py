@@ -55,6 +55,8 @@ def main():
                     help="acoustic gradient requests of one grid and nt pack whatever their models")
     ap.add_argument("--exclusive-s", type=float, default=None,
                     help="requests estimated to take longer than this alone run with the device to themselves")
+    ap.add_argument("--long-charge", type=float, default=1.0,
+                    help="fraction of the budget charged to those requests (0.51: one at a time)")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     from .loop import ServingLoop, run_alone
@@ -84,7 +86,7 @@ def main():
                       "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool,
                       "shape_pool": args.shape_pool},
            "slots": args.slots, "packing": not args.no_pack, "pack_across_models": args.pack_across_models,
-           "exclusive_s": args.exclusive_s,
+           "exclusive_s": args.exclusive_s, "long_charge": args.long_charge,
            "graph_steps": args.graph_steps,
            "plain_capacity_req_per_s": capacity,
            "plain_service_p50_s": float(np.percentile(service, 50)),
@@ -99,6 +101,7 @@ def main():
                 r.arrival = 0.0
         rep = ServingLoop("cuda:0", max_slots=args.slots, pack=not args.no_pack,
                           pack_across_models=args.pack_across_models, exclusive_s=args.exclusive_s,
+                          long_charge=args.long_charge,
                           graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
         s = rep.summary()
         fifo = fifo_latencies([r.arrival for r in reqs], service)

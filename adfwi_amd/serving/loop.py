@@ -71,13 +71,16 @@ class ServingLoop:
     `pack_across_models`: acoustic gradient requests of one grid and nt pack whatever their
     models (off by default: on bursts of small requests the fewer, longer packs balance worse
     over the slots than the requests alone).  `exclusive_s`: a request estimated to take longer
-    than this alone (`service_estimate_s`) is charged the whole budget, so it starts when the
-    device is idle and runs alone (None: every request shares the device)."""
+    than this alone (`service_estimate_s`) is charged `long_charge` of the whole budget (at least
+    its footprint): 1.0 starts it on an idle device and runs it alone; just over 0.5 keeps two
+    long requests from sharing the device while short ones still run beside one (None: every
+    request is charged its footprint)."""
 
     def __init__(self, device="cuda:0", max_slots: int = 4, budget_bytes: Optional[int] = None,
                  footprint: Callable[[Request], int] = footprint_bytes, max_bypass: int = 4,
                  pack: bool = True, max_pack_shots: int = 64, graph_steps: int = 64,
-                 pack_across_models: bool = False, exclusive_s: Optional[float] = None):
+                 pack_across_models: bool = False, exclusive_s: Optional[float] = None,
+                 long_charge: float = 1.0):
         import torch
         from .. import _native as nat
         nat.require_gpu(device)
@@ -95,7 +98,10 @@ class ServingLoop:
         key = (lambda r: pack_key(r, True)) if pack_across_models else pack_key
         self.scheduler = Scheduler(budget_bytes, max_slots, max_bypass, key if pack else None, max_pack_shots)
         self.footprint = footprint
+        if not 0.0 < long_charge <= 1.0:
+            raise ValueError(f"long_charge must be in (0, 1] (got {long_charge})")
         self.exclusive_s = exclusive_s
+        self.long_charge = float(long_charge)
         self.max_slots = max_slots
         self.graph_steps = int(graph_steps)
         self._local = threading.local()
@@ -144,7 +150,8 @@ class ServingLoop:
                     r = pending[i]
                     nbytes = self.footprint(r)
                     if self.exclusive_s is not None and service_estimate_s(r) > self.exclusive_s:
-                        nbytes = max(nbytes, self.scheduler.budget)   # runs alone
+                        # an oversized footprint stays as it is, so that it is rejected
+                        nbytes = max(nbytes, int(self.long_charge * self.scheduler.budget))
                     if not self.scheduler.submit(r, nbytes):
                         outcomes.append(Outcome(r.rid, r.arrival, now, now, {}, None,
                                                 self.scheduler.rejected[-1][1]))
