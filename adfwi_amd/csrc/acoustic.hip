@@ -1317,6 +1317,12 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   AcDev w = pair_dev(v);
   w.spb = L.reserved;
   const dim3 pgrid((w.nstrips + WPB - 1) / WPB, (d->nshots + w.spb - 1) / w.spb);
+  // the one-step kernel runs at the finer of the two shot groupings, so that both adjoint
+  // kernels add a shot's image terms into the same accumulator plane (plane = shot / finer):
+  // the planes of a group then hold that group's shots only, whichever kernel reversed a step
+  AcDev v1 = v;
+  v1.spb = L.shots_per_block < L.reserved ? L.shots_per_block : L.reserved;
+  const dim3 grid1(grid.x, (d->nshots + v1.spb - 1) / v1.spb);
   auto slot = [&](int t) -> long long { return (long long)(t - a->image_t0) * L.field_floats; };
   float *cur = const_cast<float *>(a->lam_in);
   for (int k = 0; k < a->nsteps;) {
@@ -1342,7 +1348,7 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
     } else {
       const float *imd = a->image ? a->image + slot(t) : nullptr;
       const float *imp = a->image_p ? a->image_p + slot(t) : nullptr;
-      launch_adj(v, grid, block, st, cur, nxt, t, imd, imp, a->grad_a1, a->grad_a2, a->grad_src);
+      launch_adj(v1, grid1, block, st, cur, nxt, t, imd, imp, a->grad_a1, a->grad_a2, a->grad_src);
       if ((rc = adfwi::check_launch("ac_adj_step"))) return rc;
     }
     cur = nxt;

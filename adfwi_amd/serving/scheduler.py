@@ -75,13 +75,24 @@ class Scheduler:
                 key = self.pack_key(w.req) if self.pack_key is not None else None
                 if key is not None:
                     shots = w.req.nshots
+                    between: List[_Waiting] = []   # waiting requests after the lead, not taken
+                    passed: List[_Waiting] = []    # those a member has been taken past
                     for e in queue[i + 1:]:
-                        if (id(e) not in taken and e.nbytes <= free and shots + e.req.nshots <= self.max_pack_shots
+                        if id(e) in taken:
+                            continue
+                        if (e.nbytes <= free and shots + e.req.nshots <= self.max_pack_shots
                                 and self.pack_key(e.req) == key):
+                            if any(p.bypassed >= self.max_bypass for p in between):
+                                break           # a request that may not be passed again is ahead
                             pack.append(e)
                             taken.add(id(e))
                             free -= e.nbytes
                             shots += e.req.nshots
+                            passed = list(between)
+                        else:
+                            between.append(e)
+                    for p in passed:            # taking a member past a request passes it once
+                        p.bypassed += 1
                 slots -= 1
                 for m in pack:
                     self.running[m.req.rid] = m.nbytes

@@ -5,17 +5,163 @@ src_type, rcv_type, nt, dt, t, data), `record_data`, `save`, `load` and `parse_*
 file is an .npz whose entries are all plain arrays: each record component `k` is stored as
 `data_k`, the component names in `data_keys`, the type lists as string arrays and the scalars
 as 0-d arrays, so `load` reads it with `allow_pickle=False`.  The reference stores its record
-dictionary as one pickled object (`np.savez(..., data=dict)`, data.py:63-75); such a file is
-refused with an explanation rather than unpickled.  `to_device` puts the records on the GPU,
+dictionary as one pickled object (`np.savez(..., data=dict)`, data.py:63-75); `load` reads such
+files too, through `read_reference_npz`, whose decoder admits only containers, strings, numbers
+and numpy arrays, dtypes and scalars rebuilt by this module's own code: no class or function
+named in the file is imported or called.  `to_device` puts the records on the GPU,
 optionally normalised per trace, as the FWI drivers use the observed data
 (fwi/acoustic_fwi.py:66-69).
 """
+import io
+import pickle
+import zipfile
 from typing import Dict, Optional
 
 import numpy as np
 import torch
 
 FORMAT = "adfwi_amd.seismic_data.v1"
+
+# -- the reference's pickled layout, decoded without running anything from the file -------------
+# numpy writes an object array (the record dictionary) as a pickle of the array; its arrays,
+# dtypes and scalars are rebuilt through numpy.core.multiarray._reconstruct (numpy._core in
+# numpy 2), numpy.dtype and numpy.core.multiarray.scalar.  The decoder maps exactly those names
+# to the builders below and refuses every other global, so the pickle can only produce
+# containers, strings, numbers and plain numeric / string arrays.
+_MULTIARRAY = ("numpy.core.multiarray", "numpy._core.multiarray")
+_MAX_ENTRY_BYTES = 1 << 36
+
+
+class _ArrayTag:
+    """Stands for numpy.ndarray in the stream (only used as _reconstruct's first argument)."""
+
+
+class _DtypeState:
+    def __init__(self, name, align=False, copy=True):
+        if not isinstance(name, str):
+            raise pickle.UnpicklingError("dtype name must be a string")
+        self.dtype = np.dtype(name)
+        if self.dtype.fields is not None or self.dtype.subdtype is not None:
+            raise pickle.UnpicklingError(f"structured dtype {name!r} is not admitted")
+
+    def __setstate__(self, state):
+        if not isinstance(state, tuple) or len(state) < 2 or state[1] not in ("<", ">", "|", "="):
+            raise pickle.UnpicklingError("malformed dtype state")
+        if len(state) >= 5 and (state[3] is not None or state[4] is not None):
+            raise pickle.UnpicklingError("structured dtypes are not admitted")
+        kind = self.dtype.kind
+        if kind not in "biufcUSO":
+            raise pickle.UnpicklingError(f"dtype kind {kind!r} is not admitted")
+        if kind in "US" and len(state) >= 6 and isinstance(state[5], int) and state[5] > 0:
+            # flexible types carry their item size in bytes here (4 bytes per unicode character)
+            self.dtype = np.dtype(f"{kind}{state[5] // 4 if kind == 'U' else state[5]}")
+        if state[1] in "<>" and self.dtype.byteorder != "|":
+            self.dtype = self.dtype.newbyteorder(state[1])
+
+
+class _ArrayState:
+    def __init__(self):
+        self.value = None
+
+    def __setstate__(self, state):
+        if not isinstance(state, tuple) or len(state) != 5:
+            raise pickle.UnpicklingError("malformed array state")
+        _, shape, dt, fortran, raw = state
+        if not isinstance(dt, _DtypeState) or not isinstance(shape, tuple) or \
+                not all(isinstance(n, int) and n >= 0 for n in shape):
+            raise pickle.UnpicklingError("malformed array state")
+        count = int(np.prod(shape, dtype=np.int64))
+        if dt.dtype.kind == "O":
+            if not isinstance(raw, list) or len(raw) != count:
+                raise pickle.UnpicklingError("malformed object array")
+            out = np.empty(count, dtype=object)
+            for i, v in enumerate(raw):
+                out[i] = v
+            self.value = out.reshape(shape, order="F" if fortran else "C")
+            return
+        if not isinstance(raw, (bytes, bytearray)) or len(raw) != count * dt.dtype.itemsize:
+            raise pickle.UnpicklingError("array data does not match its shape and dtype")
+        self.value = np.frombuffer(bytes(raw), dtype=dt.dtype).reshape(shape, order="F" if fortran else "C").copy()
+
+
+def _reconstruct(tag, shape, typecode):
+    if tag is not _ArrayTag:
+        raise pickle.UnpicklingError("only numpy arrays may be reconstructed")
+    return _ArrayState()
+
+
+def _scalar(dt, raw=None):
+    if not isinstance(dt, _DtypeState) or dt.dtype.kind == "O" or not isinstance(raw, (bytes, bytearray)) \
+            or len(raw) != dt.dtype.itemsize:
+        raise pickle.UnpicklingError("malformed numpy scalar")
+    return np.frombuffer(bytes(raw), dtype=dt.dtype)[0]
+
+
+class _RecordUnpickler(pickle.Unpickler):
+    def find_class(self, module, name):
+        if module in _MULTIARRAY and name == "_reconstruct":
+            return _reconstruct
+        if module in _MULTIARRAY and name == "scalar":
+            return _scalar
+        if module == "numpy" and name == "ndarray":
+            return _ArrayTag
+        if module == "numpy" and name == "dtype":
+            return _DtypeState
+        raise pickle.UnpicklingError(f"{module}.{name} is not admitted in a seismic-data file")
+
+
+def _finish(v):
+    """Replaces the decoder's array stand-ins by the arrays they built, recursively."""
+    if isinstance(v, _ArrayState):
+        if v.value is None:
+            raise pickle.UnpicklingError("array without state")
+        a = v.value
+        if a.dtype == object:
+            flat = [_finish(x) for x in a.reshape(-1)]
+            if a.shape == ():
+                return flat[0]
+            out = np.empty(len(flat), dtype=object)
+            out[:] = flat
+            return out.reshape(a.shape)
+        return a
+    if isinstance(v, (_DtypeState, _ArrayTag)) or v is _ArrayTag:
+        raise pickle.UnpicklingError("dtype or array type outside an array")
+    if isinstance(v, dict):
+        return {_finish(k): _finish(x) for k, x in v.items()}
+    if isinstance(v, list):
+        return [_finish(x) for x in v]
+    if isinstance(v, tuple):
+        return tuple(_finish(x) for x in v)
+    if v is None or isinstance(v, (str, bytes, bool, int, float, complex, np.generic)):
+        return v
+    raise pickle.UnpicklingError(f"{type(v).__name__} is not admitted in a seismic-data file")
+
+
+def read_reference_npz(path: str) -> Dict[str, object]:
+    """Every entry of an .npz written by np.savez, as the reference's SeismicData.save writes it
+    (data.py:63-75): plain arrays are read with allow_pickle=False; an object array (the record
+    dictionary) is decoded by the restricted decoder above.  Raises pickle.UnpicklingError for
+    anything but containers, strings, numbers and numeric / string arrays."""
+    out = {}
+    with zipfile.ZipFile(path) as z:
+        for info in z.infolist():
+            if not info.filename.endswith(".npy"):
+                continue
+            if info.file_size > _MAX_ENTRY_BYTES:
+                raise ValueError(f"{info.filename}: entry of {info.file_size} bytes is too large")
+            f = io.BytesIO(z.read(info))
+            version = np.lib.format.read_magic(f)
+            if version == (1, 0):
+                shape, fortran, dtype = np.lib.format.read_array_header_1_0(f)
+            else:
+                shape, fortran, dtype = np.lib.format.read_array_header_2_0(f)
+            key = info.filename[:-4]
+            if dtype.hasobject:
+                out[key] = _finish(_RecordUnpickler(f).load())
+            else:
+                f.seek(0)
+                out[key] = np.lib.format.read_array(f, allow_pickle=False)
+    return out
 
 
 def _to_numpy(v):
@@ -65,15 +211,28 @@ class SeismicData:
             names = set(f.files)
             if "data_keys" not in names:
                 if "data" in names:
-                    raise ValueError(f"{path} stores its records as one pickled object (the reference's layout); "
-                                     "this loader never unpickles files: write the records with "
-                                     "SeismicData.save, which keeps every component as a plain array")
+                    self._load_reference_layout(path)
+                    return
                 raise ValueError(f"{path} is not a seismic-data file (no data_keys entry)")
             self.data = {str(k): f[f"data_{k}"] for k in f["data_keys"]}
             self.src_loc, self.rcv_loc = f["src_loc"], f["rcv_loc"]
             self.src_num, self.rcv_num = int(f["src_num"]), int(f["rcv_num"])
             self.rcv_type, self.src_type = f["rcv_type"], f["src_type"]
             self.t, self.nt, self.dt = f["t"], int(f["nt"]), float(f["dt"])
+
+    def _load_reference_layout(self, path: str) -> None:
+        """The reference's file (data.py:63-99): the record dictionary pickled as `data`, the
+        other fields as arrays."""
+        f = read_reference_npz(path)
+        data = f["data"]
+        if not isinstance(data, dict) or not all(isinstance(k, str) and isinstance(v, np.ndarray)
+                                                 for k, v in data.items()):
+            raise ValueError(f"{path}: `data` is not a dictionary of arrays")
+        self.data = data
+        self.src_loc, self.rcv_loc = f["src_loc"], f["rcv_loc"]
+        self.src_num, self.rcv_num = int(f["src_num"]), int(f["rcv_num"])
+        self.rcv_type, self.src_type = f["rcv_type"], f["src_type"]
+        self.t, self.nt, self.dt = f["t"], int(f["nt"]), float(f["dt"])
 
     @staticmethod
     def _norm(a):

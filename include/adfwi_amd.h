@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define ADFWI_ABI_VERSION 1
+#define ADFWI_ABI_VERSION 2
 
 #define ADFWI_OK 0
 #define ADFWI_E_VERSION -1   /* abi_version / struct_size mismatch */

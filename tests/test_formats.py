@@ -1,5 +1,6 @@
 """On-disk formats (no GPU): SeismicData records and model files round-trip with plain arrays
-only, and a file in the reference's pickled layout is refused, never unpickled."""
+only; a file in the reference's pickled layout is read by a decoder that admits nothing but
+containers, strings, numbers and plain arrays."""
 import numpy as np
 import pytest
 import torch
@@ -45,10 +46,75 @@ def test_elastic_parse_gives_negative_stress_sum_as_pressure():
     assert np.array_equal(pressure, -(d.data["txx"] + d.data["tzz"])) and vz is d.data["vz"]
 
 
-def test_reference_layout_is_refused_not_unpickled(tmp_path):
+def _reference_layout_file(path, recs, numpy1_names=False):
+    """An .npz in the layout the reference's SeismicData.save writes (survey/data.py:63-75): the
+    record dictionary as one object entry, the other fields as np.savez stores them.  With
+    numpy1_names, the pickled entry names numpy.core.multiarray as numpy 1.x (the reference's
+    pinned 1.24.4) writes it."""
+    np.savez(path, data=dict(recs), src_loc=np.array([[3, 2], [9, 2]]), rcv_loc=np.array([[0, 1], [1, 1]]),
+             src_num=2, rcv_num=2, rcv_type=["pr", "pr"], src_type=["mt", "mt"], t=np.arange(4) * 1e-3,
+             nt=4, dt=1e-3)
+    if numpy1_names:
+        import zipfile
+        with zipfile.ZipFile(path) as z:
+            entries = {n: z.read(n) for n in z.namelist()}
+        raw = entries["data.npy"]
+        n = raw.count(b"\x8c\x16numpy._core.multiarray")
+        assert n >= 1
+        raw = raw.replace(b"\x8c\x16numpy._core.multiarray", b"\x8c\x15numpy.core.multiarray")
+        f = raw.index(b"\x80\x04\x95") + 3       # protocol 4: one frame, its length follows
+        size = int.from_bytes(raw[f:f + 8], "little") - n
+        entries["data.npy"] = raw[:f] + size.to_bytes(8, "little") + raw[f + 8:]
+        with zipfile.ZipFile(path, "w") as z:
+            for n, b in entries.items():
+                z.writestr(n, b)
+
+
+@pytest.mark.parametrize("numpy1_names", [False, True])
+def test_reference_layout_is_read_without_unpickling_code(tmp_path, numpy1_names):
+    rng = np.random.default_rng(5)
+    recs = {"p": rng.standard_normal((2, 4, 2)).astype(np.float32), "u": np.zeros((2, 4, 2), np.float32),
+            "forward_wavefield_p": rng.random((3, 5)), "n": np.arange(3, dtype=">i8")}
     path = tmp_path / "ref_layout.npz"
-    np.savez(path, data={"p": np.zeros((1, 2, 3))}, src_loc=np.zeros((1, 2)), nt=2, dt=0.1)
-    with pytest.raises(ValueError, match="pickled"):
+    _reference_layout_file(path, recs, numpy1_names)
+    d = SeismicData(_survey())
+    d.load(str(path))
+    assert set(d.data) == set(recs)
+    for k, v in recs.items():
+        assert d.data[k].dtype == v.dtype and np.array_equal(d.data[k], v), k
+    assert (d.nt, d.dt, d.src_num, d.rcv_num) == (4, 1e-3, 2, 2)
+    assert list(d.rcv_type) == ["pr", "pr"] and list(d.src_type) == ["mt", "mt"]
+    assert np.array_equal(d.src_loc, [[3, 2], [9, 2]]) and np.array_equal(d.t, np.arange(4) * 1e-3)
+    p, u, w = d.parse_acoustic_data() if "w" in d.data else (d.data["p"], d.data["u"], None)
+    assert np.array_equal(p, recs["p"])
+
+
+class _Trap:
+    """Pickles as a call of a function the decoder must refuse (nothing is ever called here:
+    the test asserts the refusal happens before any global is resolved to a callable)."""
+
+    def __init__(self, target):
+        self.target = target
+
+    def __reduce__(self):
+        return (self.target, ("unexpected",))
+
+
+@pytest.mark.parametrize("payload", ["builtin", "numpy_other", "structured"])
+def test_reference_layout_decoder_refuses_other_globals(tmp_path, payload):
+    import pickle
+    from adfwi_amd.survey.data import read_reference_npz
+    if payload == "builtin":
+        bad = {"p": _Trap(print)}
+    elif payload == "numpy_other":
+        bad = {"p": _Trap(np.load)}
+    else:
+        bad = {"p": np.zeros(2, dtype=[("a", "f4"), ("b", "i4")])}
+    path = tmp_path / "bad.npz"
+    np.savez(path, data=np.array(bad, dtype=object), nt=1)
+    with pytest.raises(pickle.UnpicklingError):
+        read_reference_npz(str(path))
+    with pytest.raises(pickle.UnpicklingError):
         SeismicData(_survey()).load(str(path))
 
 

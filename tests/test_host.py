@@ -271,3 +271,20 @@ def test_pending_backward_reserves_are_counted_until_released():
     del b
     gc.collect()
     assert ResidentSet.pending_bytes == base
+
+
+def test_integration_stub_matches_the_binding():
+    """The ctypes stub a maintainer copies from INTEGRATION.md declares the same descriptor
+    fields and ABI version as the binding the package uses (and so as include/adfwi_amd.h)."""
+    import ctypes as ct
+    from adfwi_amd import _native as nat
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"class AcousticDesc\(ctypes\.Structure\):\s*_fields_ = (\[.*?\])\n", text, re.S)
+    assert m, "stub not found"
+    fields = eval(m.group(1), {"_i32": ct.c_int32, "_vp": ct.c_void_p})
+    assert [f[0] for f in fields] == [f[0] for f in nat.AcousticDesc._fields_]
+    assert [f[1] for f in fields] == [f[1] for f in nat.AcousticDesc._fields_]
+    v = re.search(r"AcousticDesc\(abi_version=(\d+)", text)
+    assert v and int(v.group(1)) == nat.ABI_VERSION
+    hdr = open(os.path.join(ROOT, "include", "adfwi_amd.h")).read()
+    assert int(re.search(r"#define ADFWI_ABI_VERSION (\d+)", hdr).group(1)) == nat.ABI_VERSION

@@ -205,3 +205,43 @@ def test_exclusive_requests_run_alone():
     small = Request(0, "acoustic", "forward", 128, 256, 600, 4, 0)
     big = Request(1, "elastic", "gradient", 512, 2048, 6000, 64, 0)
     assert service_estimate_s(small) < 0.01 < 10 < service_estimate_s(big)
+
+
+def test_pack_members_taken_past_a_waiting_request_count_as_bypasses():
+    """A pack lead takes later members of its key past an earlier waiting request that does not
+    fit: each such pack passes that request once, and once it has been passed max_bypass times
+    no member is taken past it any more, so packing cannot starve it."""
+    def req(rid, key, nbytes):
+        r = Request(rid, "acoustic", "forward", 128, 256, 100, 1, rid, 0.0)
+        r.extra.update(key=key, nbytes=nbytes)
+        return r
+    s = Scheduler(budget_bytes=100, max_slots=8, max_bypass=2, pack_key=lambda r: r.extra["key"])
+    s.submit(req(0, None, 40), 40)
+    assert [r.rid for r in s.admit()] == [0]
+    # 1 leads a pack; 2 (70 B) does not fit beside 0 and 1; 3 and 4 share 1's key
+    for r in (req(1, "a", 10), req(2, None, 70), req(3, "a", 10)):
+        s.submit(r, r.extra["nbytes"])
+    assert [[r.rid for r in p] for p in s.admit_packs()] == [[1, 3]]
+    assert [(w.req.rid, w.bypassed) for w in s.waiting] == [(2, 1)]
+    for r in (req(4, "a", 5), req(5, "a", 5)):
+        s.submit(r, r.extra["nbytes"])
+    # 4 leads from behind 2 and takes 5 along: the pack passes 2 once more (its second time)
+    assert [[r.rid for r in p] for p in s.admit_packs()] == [[4, 5]]
+    assert [(w.req.rid, w.bypassed) for w in s.waiting] == [(2, 2)]
+    s.submit(req(6, "a", 5), 5)
+    assert s.admit_packs() == []          # 2 blocks the queue until it fits
+    for rid in (0, 1, 3, 4, 5):
+        s.finish(rid)
+    assert [[r.rid for r in p] for p in s.admit_packs()] == [[2], [6]]
+
+    # a lead waiting ahead of a request that has reached max_bypass takes no member past it
+    s = Scheduler(budget_bytes=100, max_slots=4, max_bypass=1, pack_key=lambda r: r.extra["key"])
+    s.submit(req(0, None, 40), 40)
+    assert [r.rid for r in s.admit()] == [0]
+    for r in (req(1, "a", 65), req(2, None, 70), req(3, None, 10)):
+        s.submit(r, r.extra["nbytes"])
+    assert [r.rid for r in s.admit()] == [3]            # passes 1 and 2 once each
+    s.submit(req(4, "a", 10), 10)
+    s.finish(0)
+    assert [[r.rid for r in p] for p in s.admit_packs()] == [[1]]   # 4 stays behind 2
+    assert [(w.req.rid, w.bypassed) for w in s.waiting] == [(2, 1), (4, 0)]

@@ -161,3 +161,28 @@ def test_exclusive_requests_run_alone_on_the_device():
         for k in b.result:
             x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
             assert rel_l2(x, y) <= 1e-6, (a.rid, k, rel_l2(x, y))
+
+
+def test_packed_gradients_with_four_shots_per_wavefront_and_odd_segments(monkeypatch):
+    """Four shots per forward wavefront and two per two-step adjoint wavefront, with nt 80 so
+    that every checkpoint segment (k = 9) has an odd length and its first reversed step runs
+    the one-step adjoint kernel: both adjoint kernels must add a shot's image terms into the
+    plane of its gradient group, so every pack member's gradient equals its run alone (members
+    of 1-3 shots, padded to the group, against one shared model and against models of their
+    own)."""
+    from adfwi_amd.serving import Request, ServingLoop, run_alone
+    monkeypatch.setenv("ADFWI_SHOTS_PER_BLOCK", "4")
+    for across, base in ((False, 4000), (True, 4100)):
+        spec = [3, 1, 2, 3]
+        reqs = [Request(i, "acoustic", "gradient", 96, 224, 80, ns, base + i, 0.0, 20, keep=True,
+                        model=None if across else 303) for i, ns in enumerate(spec)]
+        rep = ServingLoop(DEV, max_slots=1, graph_steps=0, pack_across_models=across).run(copy.deepcopy(reqs),
+                                                                                        realtime=False)
+        assert all(o.error is None for o in rep.outcomes), [o.error for o in rep.outcomes]
+        assert any(len(p) >= 3 for p in rep.packs), rep.packs
+        alone = run_alone(reqs, DEV)
+        for a, b in zip(rep.outcomes, alone):
+            assert a.rid == b.rid and a.result.keys() == b.result.keys()
+            for k in b.result:
+                x, y = a.result[k].detach().cpu().numpy(), b.result[k].detach().cpu().numpy()
+                assert np.abs(y).max() > 0 and rel_l2(x, y) <= 1e-6, (across, a.rid, k, rel_l2(x, y))
