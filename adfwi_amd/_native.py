@@ -13,6 +13,8 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadfwi_amd.so")
 ABI_VERSION = 2
+# floats of the library's per-GPU scratch array (ADFWI_NOALLOC_SCRATCH_FLOATS, 128 MiB)
+NOALLOC_SCRATCH_FLOATS = 32 << 20
 
 
 class NativeLibraryError(RuntimeError):
@@ -88,6 +90,8 @@ _SIGNATURES = {
                                               ctypes.POINTER(AcousticFwdArgs), _vp]),
     "adfwi_acoustic_adjoint": (ctypes.c_int, [ctypes.POINTER(AcousticDesc),
                                               ctypes.POINTER(AcousticAdjArgs), _vp]),
+    "adfwi_acoustic_adjoint_noalloc": (ctypes.c_int, [ctypes.POINTER(AcousticDesc),
+                                                      ctypes.POINTER(AcousticAdjArgs), _vp, _vp, _vp]),
     "adfwi_elastic_get_layout": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(Layout)]),
     "adfwi_elastic_forward": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticFwdArgs), _vp]),
     "adfwi_elastic_adjoint": (ctypes.c_int, [ctypes.POINTER(ElasticDesc), ctypes.POINTER(ElasticAdjArgs), _vp]),

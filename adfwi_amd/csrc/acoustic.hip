@@ -1359,3 +1359,111 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free adjoint (adfwi_acoustic_adjoint_noalloc)
+// ---------------------------------------------------------------------------
+// The adjoint kernels add each shot group's zero-lag image terms into a plane of its own
+// (deterministic, no atomics); the caller of adfwi_acoustic_adjoint supplies those planes.
+// Callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc) use the variant below instead: the planes live in this
+// file-scope array, are cleared by a fill kernel, and are summed in a fixed order into the
+// caller's one-plane outputs after the time steps.  Each GPU holds its own copy of the array
+// until the process ends; every call on a GPU shares it, so calls that overlap on one GPU
+// (two streams, two threads) race on it.
+#include <mutex>
+
+namespace {
+
+constexpr long long NOALLOC_SCRATCH_FLOATS = ADFWI_NOALLOC_SCRATCH_FLOATS;   // 128 MiB per GPU
+constexpr int MAX_DEVICES = 64;
+
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.  The host
+// symbol `scratch` is only a stand-in and must never be passed to a kernel.
+int scratch_of_current_device(float **out) {
+  int dev = -1;
+  ADFWI_CHECK_HIP(hipGetDevice(&dev));
+  ADFWI_CHECK_ARG(dev >= 0 && dev < MAX_DEVICES, "device %d outside [0, %d)", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    ADFWI_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ADFWI_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = static_cast<float *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return ADFWI_OK;
+}
+
+__global__ void fill_zero(float *__restrict__ p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+// out[i] += planes[0][i] + planes[1][i] + ... in plane order (deterministic)
+__global__ void add_plane_sum(const float *__restrict__ planes, int nplanes, long long plane, float *__restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < plane; i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int g = 0; g < nplanes; ++g) s += planes[g * plane + i];
+    out[i] += s;
+  }
+}
+
+dim3 flat_grid(long long n) {
+  long long b = (n + 255) / 256;
+  return dim3((unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096));
+}
+
+}  // namespace
+
+extern "C" {
+
+int adfwi_acoustic_adjoint_noalloc(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a, float *grad_a1_sum,
+                                   float *grad_a2_sum, adfwi_stream_t stream) {
+  if (int rc = adfwi::fault_injected("adfwi_acoustic_adjoint_noalloc")) return rc;
+  adfwi_layout L;
+  int rc = layout_of(d, &L);
+  if (rc) return rc;
+  ADFWI_CHECK_ARG(a != nullptr, "null adjoint args");
+  ADFWI_CHECK_ARG(!a->grad_a1 && !a->grad_a2,
+                  "grad_a1/grad_a2 must be NULL: the per-group planes live in the library's scratch; "
+                  "pass grad_a1_sum/grad_a2_sum");
+  ADFWI_CHECK_ARG(!grad_a1_sum || a->image, "grad_a1_sum needs the stencil image");
+  ADFWI_CHECK_ARG(!grad_a2_sum || a->image_p, "grad_a2_sum needs the pressure image");
+  const long long plane = (long long)d->nz * d->pitch;
+  const long long nplanes = L.accum_floats / plane;
+  const long long need = nplanes * plane * ((grad_a1_sum ? 1 : 0) + (grad_a2_sum ? 1 : 0));
+  ADFWI_CHECK_ARG(need <= NOALLOC_SCRATCH_FLOATS,
+                  "gradient planes need %lld floats of scratch (%lld planes of %d x %d), the maximum is %lld "
+                  "(128 MiB); use adfwi_acoustic_adjoint with caller-owned planes",
+                  need, nplanes * ((grad_a1_sum ? 1 : 0) + (grad_a2_sum ? 1 : 0)), d->nz, d->pitch,
+                  NOALLOC_SCRATCH_FLOATS);
+  float *base = nullptr;
+  if (need > 0 && (rc = scratch_of_current_device(&base))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  adfwi_acoustic_adj_args b = *a;
+  b.grad_a1 = grad_a1_sum ? base : nullptr;
+  b.grad_a2 = grad_a2_sum ? base + (grad_a1_sum ? nplanes * plane : 0) : nullptr;
+  if (need > 0) {
+    hipLaunchKernelGGL(fill_zero, flat_grid(need), dim3(256), 0, st, base, need);
+    if ((rc = adfwi::check_launch("fill_zero"))) return rc;
+  }
+  if ((rc = adfwi_acoustic_adjoint(d, &b, stream))) return rc;
+  a->lam_out = b.lam_out;
+  if (grad_a1_sum) {
+    hipLaunchKernelGGL(add_plane_sum, flat_grid(plane), dim3(256), 0, st, b.grad_a1, (int)nplanes, plane, grad_a1_sum);
+    if ((rc = adfwi::check_launch("add_plane_sum"))) return rc;
+  }
+  if (grad_a2_sum) {
+    hipLaunchKernelGGL(add_plane_sum, flat_grid(plane), dim3(256), 0, st, b.grad_a2, (int)nplanes, plane, grad_a2_sum);
+    if ((rc = adfwi::check_launch("add_plane_sum"))) return rc;
+  }
+  return ADFWI_OK;
+}
+
+}  // extern "C"

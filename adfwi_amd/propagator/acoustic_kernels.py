@@ -397,7 +397,7 @@ def run_forward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, *
 
 def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, ckpt: Optional[torch.Tensor],
                  ckpt_every: int, g_rec, need_a1: bool, need_a2: bool, need_src: bool,
-                 group_planes: bool = False, images: Optional[dict] = None):
+                 group_planes: bool = False, images: Optional[dict] = None, noalloc: bool = False):
     """Hand-written adjoint over all steps with segment recomputation from the checkpoints.
 
     g_rec: tuple of three (S, nt, R) adjoint sources (or None each).
@@ -406,7 +406,11 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
     Returns (grad_a1 (Z, X) or None, grad_a2 or None, grad_src (S, nt) or None).
     group_planes: grad_a1 / grad_a2 are returned unsummed, (G, Z, X) with one plane per group of
     `geom.gradient_group` consecutive shots (their sum is the gradient of all shots).
+    noalloc: the adjoint runs through adfwi_acoustic_adjoint_noalloc, whose per-group planes live
+    in the library's scratch array and are summed into one plane per gradient by each call.
     """
+    if noalloc and group_planes:
+        raise ValueError("noalloc sums the group planes inside each call; group_planes needs them apart")
     lib = nat.load()
     dev = geom.device
     S, Z, X, nt = geom.S, geom.Z, geom.X, geom.nt
@@ -419,7 +423,7 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
     recompute_any = len(images) < nseg
     image = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 and recompute_any else None
     image_p = torch.zeros((k, S, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 and recompute_any else None
-    nacc = int(L.accum_floats) // (Z * geom.pitch)   # accumulator planes (finest shot grouping)
+    nacc = 1 if noalloc else int(L.accum_floats) // (Z * geom.pitch)   # accumulator planes (finest shot grouping)
     ga1 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a1 else None
     ga2 = torch.zeros((nacc, Z, geom.pitch), device=dev, dtype=torch.float32) if need_a2 else None
     gsrc = torch.zeros((S, nt), device=dev, dtype=torch.float32) if need_src else None
@@ -464,15 +468,24 @@ def run_backward(geom: AcousticGeometry, coef: torch.Tensor, src: torch.Tensor, 
         aa.image = None if seg_image is None or not need_a1 else seg_image.data_ptr()
         aa.image_p = None if seg_image_p is None or not need_a2 else seg_image_p.data_ptr()
         aa.grad_rec_p, aa.grad_rec_u, aa.grad_rec_w = gp
-        aa.grad_a1 = None if ga1 is None else ga1.data_ptr()
-        aa.grad_a2 = None if ga2 is None else ga2.data_ptr()
+        if noalloc:
+            aa.grad_a1 = aa.grad_a2 = None
+        else:
+            aa.grad_a1 = None if ga1 is None else ga1.data_ptr()
+            aa.grad_a2 = None if ga2 is None else ga2.data_ptr()
         aa.grad_src = None if gsrc is None else gsrc.data_ptr()
         aa.rcv_map = None if geom.rcv_map is None else geom.rcv_map.data_ptr()
         aa.fuse_steps = geom.fuse_steps
         fused = bool(geom.fuse_steps) and (all(x is None for x in g) or geom.rcv_map is not None)
+        if noalloc:
+            call = lambda: nat.check(lib.adfwi_acoustic_adjoint_noalloc(
+                ctypes.byref(d), ctypes.byref(aa), nat.ptr(ga1), nat.ptr(ga2), stream),
+                "adfwi_acoustic_adjoint_noalloc")
+        else:
+            call = lambda: nat.check(lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream),
+                                     "adfwi_acoustic_adjoint")
         TIMELINE.call("adjoint", "ac_adj_pair" if fused else "ac_adj_step", (n + 1) // 2 if fused else n, dev,
-                      lambda: nat.check(lib.adfwi_acoustic_adjoint(ctypes.byref(d), ctypes.byref(aa), stream),
-                                        "adfwi_acoustic_adjoint"), steps=n)
+                      call, steps=n)
         lam = lw[0] if aa.lam_out == lw[0].data_ptr() else lw[1]
         kept = seg_image = seg_image_p = None     # a kept segment's images are free for reuse now
     if group_planes:

@@ -5,7 +5,9 @@
  * Conventions
  *   - Every buffer is caller-owned device memory; the library never allocates
  *     or frees device memory and keeps no mutable global state, so calls on
- *     distinct streams may run concurrently.
+ *     distinct streams may run concurrently.  The one exception is
+ *     adfwi_acoustic_adjoint_noalloc, which uses a per-GPU scratch array of the
+ *     library (see its declaration).
  *   - Functions return 0 on success or a negative ADFWI_E* code;
  *     adfwi_last_error() returns a thread-local message for the last failure
  *     on the calling thread.  Arguments are checked before any launch.
@@ -150,6 +152,25 @@ typedef struct adfwi_acoustic_adj_args {
  * acoustic_kernels.py:115-169), with zero-lag imaging of dL/dA1 and dL/dA2. */
 int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a,
                            adfwi_stream_t stream);
+
+/* Size of the library's per-GPU scratch array used by adfwi_acoustic_adjoint_noalloc, in
+ * floats (128 MiB). */
+#define ADFWI_NOALLOC_SCRATCH_FLOATS (32ll << 20)
+
+/* Allocation-free variant of adfwi_acoustic_adjoint.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels run; the per-shot-group gradient
+ * planes live in a file-scope device array of the library (ADFWI_NOALLOC_SCRATCH_FLOATS floats
+ * per GPU, held until the process ends), are cleared by a fill kernel at the start of every
+ * call and summed in plane order into the caller's one-plane outputs at its end:
+ *   grad_a1_sum, grad_a2_sum: [nz][pitch] (+=) or NULL (then the gradient is not formed);
+ *   a->grad_a1 and a->grad_a2 must be NULL; every other field of `a` as for
+ *   adfwi_acoustic_adjoint (the caller still passes inputs, outputs and work state sets).
+ * Fails with ADFWI_E_ARG when the planes (layout.accum_floats per requested gradient) exceed
+ * the scratch array.  Calls no hipMalloc/hipFree.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int adfwi_acoustic_adjoint_noalloc(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args *a,
+                                   float *grad_a1_sum, float *grad_a2_sum, adfwi_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Elastic velocity-stress propagator, 4th order, split-field PML             */

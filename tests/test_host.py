@@ -166,6 +166,33 @@ def test_abi_rejects_bad_arguments_with_messages():
         nat.check(-2, "test")
 
 
+def test_noalloc_adjoint_rejects_before_any_device_access():
+    """adfwi_acoustic_adjoint_noalloc checks its arguments and the size of the gradient planes
+    against its scratch array before it touches a device (these calls run without a GPU)."""
+    from adfwi_amd import _native as nat
+    lib = nat.load()
+    fake = ctypes.c_void_p(0x1000)
+    # planes of one gradient: (shot groups) x Z x pitch floats; 64 single-shot groups of
+    # 2000 x 2048 exceed the scratch array, one group of 60 x 192 fits
+    d = _desc(S=64, Z=2000, X=2000)
+    d.shots_per_block, d.adj_shots_per_block = 1, 1
+    a = nat.AcousticAdjArgs()
+    a.image = fake.value
+    assert lib.adfwi_acoustic_adjoint_noalloc(ctypes.byref(d), ctypes.byref(a), fake, None, None) == -2
+    assert b"scratch" in lib.adfwi_last_error() and b"maximum" in lib.adfwi_last_error()
+    L = nat.Layout()
+    assert lib.adfwi_acoustic_get_layout(ctypes.byref(d), ctypes.byref(L)) == 0
+    assert L.accum_floats > nat.NOALLOC_SCRATCH_FLOATS
+    a.grad_a1 = fake.value          # the planes are the library's: caller planes are refused
+    assert lib.adfwi_acoustic_adjoint_noalloc(ctypes.byref(_desc()), ctypes.byref(a), None, None, None) == -2
+    assert b"must be NULL" in lib.adfwi_last_error()
+    a.grad_a1, a.image = None, None  # a gradient sum without the stencil image is refused
+    assert lib.adfwi_acoustic_adjoint_noalloc(ctypes.byref(_desc()), ctypes.byref(a), fake, None, None) == -2
+    assert b"stencil image" in lib.adfwi_last_error()
+    assert "#define ADFWI_NOALLOC_SCRATCH_FLOATS (32ll << 20)" in open(HEADER).read()
+    assert nat.NOALLOC_SCRATCH_FLOATS == 32 << 20
+
+
 def test_fault_injection_surfaces_as_error():
     code = ("import ctypes; from adfwi_amd import _native as nat; lib = nat.load(); "
             "print(lib.adfwi_acoustic_forward(None, None, None))")
