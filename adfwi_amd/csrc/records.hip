@@ -9,6 +9,9 @@
 // with m = max_t |a| reached first at t* (the max's gradient goes to that sample, as torch's
 // max over a dimension routes it); d loss / d b = -w rsd.  A trace with zero residual has
 // e = 0 and, as in the reference, a NaN gradient.
+// Least-squares mode (ADFWI_L2_SQUARED): trace loss = e / 2 and w = dt, the objective
+// 1/2 sum (n - b)^2 dt, whose gradient is defined on every trace (no counterpart in the
+// reference's misfit set).
 //
 // One workgroup per (shot, 64 receivers): each lane takes one receiver, so every time sample
 // is a coalesced 256-byte row of the records, and the TW wavefronts of the workgroup take
@@ -27,7 +30,8 @@ namespace {
 constexpr int TW = 8;   // wavefronts per workgroup: chunks of the time axis
 
 __global__ __launch_bounds__(TW * 64) void trace_l2(const float *__restrict__ a, const float *__restrict__ b,
-                                                    int nt, int R, float dt, int normalize, float *__restrict__ loss,
+                                                    int nt, int R, float dt, int normalize, int squared,
+                                                    float *__restrict__ loss,
                                                     float *__restrict__ ga, float *__restrict__ gb) {
   __shared__ float part_m[TW][64], part_e[TW][64], part_q[TW][64];
   __shared__ int part_t[TW][64];
@@ -82,9 +86,9 @@ __global__ __launch_bounds__(TW * 64) void trace_l2(const float *__restrict__ a,
   }
   const float root = sqrtf(e);
   if (!act) return;
-  if (wv == 0) loss[(long long)s * R + r] = root;
+  if (wv == 0) loss[(long long)s * R + r] = squared ? 0.5f * e : root;
   if (!ga && !gb) return;
-  const float w = dt / root;
+  const float w = squared ? dt : dt / root;
 #pragma unroll 4
   for (int t = t0; t < t1; ++t) {
     const long long i = base + (long long)t * R;
@@ -106,16 +110,18 @@ __global__ __launch_bounds__(TW * 64) void trace_l2(const float *__restrict__ a,
 }  // namespace
 
 extern "C" int adfwi_trace_l2(const float *a, const float *b, int32_t nshots, int32_t nt, int32_t nrec, float dt,
-                              int32_t normalize_a, float *trace_loss, float *grad_a, float *grad_b,
+                              int32_t flags, float *trace_loss, float *grad_a, float *grad_b,
                               adfwi_stream_t stream) {
   if (int rc = adfwi::fault_injected("adfwi_trace_l2")) return rc;
   ADFWI_CHECK_ARG(a && b && trace_loss, "null record or trace-loss pointer");
   ADFWI_CHECK_ARG(nshots >= 1 && nt >= 1 && nrec >= 1, "record shape %d x %d x %d invalid", nshots, nt, nrec);
   ADFWI_CHECK_ARG(dt > 0.0f, "dt must be positive (got %g)", (double)dt);
+  ADFWI_CHECK_ARG((flags & ~(ADFWI_L2_NORMALIZE | ADFWI_L2_SQUARED)) == 0, "unknown flags 0x%x", flags);
   const long long nblocks = (long long)nshots * ((nrec + 63) / 64);
   ADFWI_CHECK_ARG(nblocks < (1LL << 31), "record shape %d x %d x %d too large for one launch", nshots, nt, nrec);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(trace_l2, dim3((unsigned)nblocks), dim3(TW * 64), 0, st, a, b, nt, nrec, dt,
-                     normalize_a ? 1 : 0, trace_loss, grad_a, grad_b);
+                     (flags & ADFWI_L2_NORMALIZE) ? 1 : 0, (flags & ADFWI_L2_SQUARED) ? 1 : 0, trace_loss,
+                     grad_a, grad_b);
   return adfwi::check_launch("trace_l2");
 }

@@ -552,7 +552,9 @@ def coefficient_fields(v, rho, damp, nz, nx, nabc, dt, dz, free_surface, device)
     den = pad_torchSingle(rho, nabc, nz, nx, 0, device)
     Z, X = nz + 2 * nabc, nx + 2 * nabc
     fs = nabc if free_surface else 1
-    alpha1 = den * c * c * dt / dz
+    # true division, as the reference's CPU arithmetic does: torch on a GPU turns division by a
+    # Python number into multiplication by its reciprocal, which differs in the last bit
+    alpha1 = den * c * c * dt / torch.full([], dz, dtype=den.dtype, device=den.device)
     kappa1 = damp * dt
     alpha2 = dt / (den * dz)
     kappa2 = torch.zeros_like(damp)

@@ -260,14 +260,19 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
 /* Record post-processing                                                     */
 /* ------------------------------------------------------------------------ */
 
+#define ADFWI_L2_NORMALIZE 1  /* n = a / max_t |a| per trace (fwi/acoustic_fwi.py:149-150) */
+#define ADFWI_L2_SQUARED 2    /* least-squares trace loss (no counterpart in the reference) */
+
 /* Waveform L2 misfit per trace of records a against b ([nshots][nt][nrec], device):
- * trace_loss[s][r] = sqrt(sum_t (n - b)^2 dt) with n = a / max_t |a| per trace when
- * normalize_a (the FWI drivers' trace normalisation, fwi/acoustic_fwi.py:149-150) or n = a
- * (Misfit_waveform_L2, fwi/misfit/L2.py:26-29); the misfit is the sum of trace_loss.
- * grad_a / grad_b (or NULL): d misfit / d a and d misfit / d b, through the normalisation
- * (whose max routes its gradient to the first sample reaching it). */
+ * trace_loss[s][r] = sqrt(sum_t (n - b)^2 dt) (Misfit_waveform_L2, fwi/misfit/L2.py:26-29), or
+ * 1/2 sum_t (n - b)^2 dt with ADFWI_L2_SQUARED (the least-squares objective, whose gradient is
+ * defined on traces with zero residual, where the square root's is NaN as in the reference);
+ * n = a / max_t |a| per trace with ADFWI_L2_NORMALIZE (the FWI drivers' trace normalisation),
+ * else n = a.  The misfit is the sum of trace_loss.  grad_a / grad_b (or NULL): d misfit / d a
+ * and d misfit / d b, through the normalisation (whose max routes its gradient to the first
+ * sample reaching it).  flags = 0 or 1 keeps the meaning of the earlier normalize_a argument. */
 int adfwi_trace_l2(const float *a, const float *b, int32_t nshots, int32_t nt, int32_t nrec, float dt,
-                   int32_t normalize_a, float *trace_loss, float *grad_a, float *grad_b, adfwi_stream_t stream);
+                   int32_t flags, float *trace_loss, float *grad_a, float *grad_b, adfwi_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -108,7 +108,9 @@ def coefficients(vp: torch.Tensor, rho: torch.Tensor, damp: torch.Tensor, nz: in
     den = pad_edge(rho, nabc, nz, nx)
     Z, X = nz + 2 * nabc, nx + 2 * nabc
     fs = nabc if free_surface else 1
-    a1 = den * c * c * dt / dz
+    # dz as a 0-d tensor on the data's device: on a GPU torch turns division by a Python number
+    # into multiplication by its reciprocal; the reference divides (it runs on the CPU)
+    a1 = den * c * c * dt / torch.full([], dz, dtype=den.dtype, device=den.device)
     k1 = damp * dt
     a2 = dt / (den * dz)
     k2 = torch.zeros_like(damp)

@@ -100,6 +100,10 @@ def _steps(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int
     pzn = 1 - 0.5 * dt * bzw
     dt_dx = dt / dx
     dt_dz = dt / dz
+    # divisors as 0-d tensors on the data's device: on a GPU torch turns division by a Python
+    # number into multiplication by its reciprocal; the reference divides (it runs on the CPU)
+    dxs = torch.full([], dx, dtype=vx.dtype, device=vx.device)
+    dzs = torch.full([], dz, dtype=vx.dtype, device=vx.device)
     pxi = 1.0 / pxd
     pzi = 1.0 / pzd
     shots = torch.arange(S, device=vx.device)
@@ -148,10 +152,10 @@ def _steps(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: int
         dzb_txz = _dzb(txz, fdc, Z, X)
         dxb_txz = _dxb(txz, fdc, Z, X)
         dzf_tzz = _dzf(tzz, fdc, Z, X)
-        vx_x[:, W0:WZ, W0:WX] = (pxn * vx_x[:, W0:WZ, W0:WX] + dt * BX * dxf_txx / dx) / pxd
-        vx_z[:, W0:WZ, W0:WX] = (pzn * vx_z[:, W0:WZ, W0:WX] + dt * BX * dzb_txz / dz) / pzd
-        vz_x[:, W0:WZ, W0:WX] = (pxn * vz_x[:, W0:WZ, W0:WX] + dt * BZ * dxb_txz / dx) / pxd
-        vz_z[:, W0:WZ, W0:WX] = (pzn * vz_z[:, W0:WZ, W0:WX] + dt * BZ * dzf_tzz / dz) / pzd
+        vx_x[:, W0:WZ, W0:WX] = (pxn * vx_x[:, W0:WZ, W0:WX] + dt * BX * dxf_txx / dxs) / pxd
+        vx_z[:, W0:WZ, W0:WX] = (pzn * vx_z[:, W0:WZ, W0:WX] + dt * BX * dzb_txz / dzs) / pzd
+        vz_x[:, W0:WZ, W0:WX] = (pxn * vz_x[:, W0:WZ, W0:WX] + dt * BZ * dxb_txz / dxs) / pxd
+        vz_z[:, W0:WZ, W0:WX] = (pzn * vz_z[:, W0:WZ, W0:WX] + dt * BZ * dzf_tzz / dzs) / pzd
         vx[:] = vx_x + vx_z
         vz[:] = vz_x + vz_z
         if free_surface:
@@ -195,6 +199,10 @@ def _steps_abl(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc:
     W0, WZ, WX = 2, Z - 2, X - 2
     shots = torch.arange(S, device=vx.device)
     scaling = -1.0 / 3
+    # divisors as 0-d tensors on the data's device: on a GPU torch turns division by a Python
+    # number into multiplication by its reciprocal; the reference divides (it runs on the CPU)
+    dxs = torch.full([], dx, dtype=vx.dtype, device=vx.device)
+    dzs = torch.full([], dz, dtype=vx.dtype, device=vx.device)
     row0 = 2 if free_surface else 2 + nabc
     C11 = c11[W0:WZ, W0:WX]
     C13 = c13[W0:WZ, W0:WX]
@@ -213,12 +221,12 @@ def _steps_abl(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc:
         dzf_vx = _dzf(vx, fdc, Z, X)
         dxf_vz = _dxf(vz, fdc, Z, X)
         dzf_vz = _dzf(vz, fdc, Z, X)
-        txx[:, W0:WZ, W0:WX] = txx[:, W0:WZ, W0:WX] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dx
-                                                            + (C15 * dzb_vx + C13 * dzb_vz) / dz)
-        tzz[:, W0:WZ, W0:WX] = tzz[:, W0:WZ, W0:WX] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dx
-                                                            + (C35 * dzb_vx + C33 * dzb_vz) / dz)
-        txz[:, W0:WZ, W0:WX] = txz[:, W0:WZ, W0:WX] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dx
-                                                            + (C55 * dzf_vx + C35 * dzf_vz) / dz)
+        txx[:, W0:WZ, W0:WX] = txx[:, W0:WZ, W0:WX] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dxs
+                                                            + (C15 * dzb_vx + C13 * dzb_vz) / dzs)
+        tzz[:, W0:WZ, W0:WX] = tzz[:, W0:WZ, W0:WX] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dxs
+                                                            + (C35 * dzb_vx + C33 * dzb_vz) / dzs)
+        txz[:, W0:WZ, W0:WX] = txz[:, W0:WZ, W0:WX] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dxs
+                                                            + (C55 * dzf_vx + C35 * dzf_vz) / dzs)
         sv = src_v[shots, t]
         txx[shots, src_z, src_x] += scaling * mt[:, 0, 0] * sv
         tzz[shots, src_z, src_x] += scaling * mt[:, 2, 2] * sv
@@ -232,8 +240,8 @@ def _steps_abl(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc:
         dzb_txz = _dzb(txz, fdc, Z, X)
         dxb_txz = _dxb(txz, fdc, Z, X)
         dzf_tzz = _dzf(tzz, fdc, Z, X)
-        vx[:, W0:WZ, W0:WX] += dt * BX * (dxf_txx / dx + dzb_txz / dz)
-        vz[:, W0:WZ, W0:WX] += dt * BZ * (dxb_txz / dx + dzf_tzz / dz)
+        vx[:, W0:WZ, W0:WX] += dt * BX * (dxf_txx / dxs + dzb_txz / dzs)
+        vz[:, W0:WZ, W0:WX] += dt * BZ * (dxb_txz / dxs + dzf_tzz / dzs)
         if free_surface:
             vz[:, 1, W0:WX] = vz[:, 2, W0:WX]
             vx[:, 1, W0:WX] = vz[:, 1, W0 + 1:WX + 1] - vz[:, 1, W0:WX] + vz[:, 2, W0 + 1:WX + 1] \
@@ -298,6 +306,10 @@ def _steps6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: in
     pxd, pxn = 1 + 0.5 * dt * bxw, 1 - 0.5 * dt * bxw
     pzd, pzn = 1 + 0.5 * dt * bzw, 1 - 0.5 * dt * bzw
     dt_dx, dt_dz = dt / dx, dt / dz
+    # divisors as 0-d tensors on the data's device: on a GPU torch turns division by a Python
+    # number into multiplication by its reciprocal; the reference divides (it runs on the CPU)
+    dxs = torch.full([], dx, dtype=vx.dtype, device=vx.device)
+    dzs = torch.full([], dz, dtype=vx.dtype, device=vx.device)
     pxi, pzi = 1.0 / pxd, 1.0 / pzd
     shots = torch.arange(S, device=vx.device)
     half = mt / 2
@@ -330,10 +342,10 @@ def _steps6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc: in
             txz[:, h - 2, :] = -txz[:, h - 1, :]
             txz[:, h - 3, :] = -txz[:, h, :]
         dxf_txx, dzb_txz, dxb_txz, dzf_tzz = dxf(txx), dzb(txz), dxb(txz), dzf(tzz)
-        vx_x[:, I, J] = (pxn * vx_x[:, I, J] + dt * BX * dxf_txx / dx) / pxd
-        vx_z[:, I, J] = (pzn * vx_z[:, I, J] + dt * BX * dzb_txz / dz) / pzd
-        vz_x[:, I, J] = (pxn * vz_x[:, I, J] + dt * BZ * dxb_txz / dx) / pxd
-        vz_z[:, I, J] = (pzn * vz_z[:, I, J] + dt * BZ * dzf_tzz / dz) / pzd
+        vx_x[:, I, J] = (pxn * vx_x[:, I, J] + dt * BX * dxf_txx / dxs) / pxd
+        vx_z[:, I, J] = (pzn * vx_z[:, I, J] + dt * BX * dzb_txz / dzs) / pzd
+        vz_x[:, I, J] = (pxn * vz_x[:, I, J] + dt * BZ * dxb_txz / dxs) / pxd
+        vz_z[:, I, J] = (pzn * vz_z[:, I, J] + dt * BZ * dzf_tzz / dzs) / pzd
         vx[:] = vx_x + vx_z
         vz[:] = vz_x + vz_z
         if free_surface:
@@ -379,6 +391,10 @@ def _steps_abl6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc
         return c0 * (sh(a, 0, 0) - sh(a, -1, 0)) + c1 * (sh(a, 1, 0) - sh(a, -2, 0)) + c2 * (sh(a, 2, 0) - sh(a, -3, 0))
     shots = torch.arange(S, device=vx.device)
     scaling = -1.0 / 3
+    # divisors as 0-d tensors on the data's device: on a GPU torch turns division by a Python
+    # number into multiplication by its reciprocal; the reference divides (it runs on the CPU)
+    dxs = torch.full([], dx, dtype=vx.dtype, device=vx.device)
+    dzs = torch.full([], dz, dtype=vx.dtype, device=vx.device)
     h = nn + 1
     row0 = nn if free_surface else nn + nabc
     C11, C13, C15, C33, C35, C55 = (a[I, J] for a in (c11, c13, c15, c33, c35, c55))
@@ -386,9 +402,9 @@ def _steps_abl6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc
     for t in range(nt):
         dxb_vx, dxb_vz, dzb_vx, dzb_vz = dxb(vx), dxb(vz), dzb(vx), dzb(vz)
         dxf_vx, dzf_vx, dxf_vz, dzf_vz = dxf(vx), dzf(vx), dxf(vz), dzf(vz)
-        txx[:, I, J] = txx[:, I, J] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dx + (C15 * dzb_vx + C13 * dzb_vz) / dz)
-        tzz[:, I, J] = tzz[:, I, J] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dx + (C35 * dzb_vx + C33 * dzb_vz) / dz)
-        txz[:, I, J] = txz[:, I, J] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dx + (C55 * dzf_vx + C35 * dzf_vz) / dz)
+        txx[:, I, J] = txx[:, I, J] + dt * ((C11 * dxb_vx + C15 * dxb_vz) / dxs + (C15 * dzb_vx + C13 * dzb_vz) / dzs)
+        tzz[:, I, J] = tzz[:, I, J] + dt * ((C13 * dxb_vx + C35 * dxb_vz) / dxs + (C35 * dzb_vx + C33 * dzb_vz) / dzs)
+        txz[:, I, J] = txz[:, I, J] + dt * ((C15 * dxf_vx + C55 * dxf_vz) / dxs + (C55 * dzf_vx + C35 * dzf_vz) / dzs)
         sv = src_v[shots, t]
         txx[shots, src_z, src_x] += scaling * mt[:, 0, 0] * sv
         tzz[shots, src_z, src_x] += scaling * mt[:, 2, 2] * sv
@@ -399,8 +415,8 @@ def _steps_abl6(free_surface: bool, nx: int, nz: int, dx: float, dz: float, nabc
             txz[:, h - 2, :] = -txz[:, h - 1, :]
             txz[:, h - 3, :] = -txz[:, h, :]
         dxf_txx, dzb_txz, dxb_txz, dzf_tzz = dxf(txx), dzb(txz), dxb(txz), dzf(tzz)
-        vx[:, I, J] += dt * BX * (dxf_txx / dx + dzb_txz / dz)
-        vz[:, I, J] += dt * BZ * (dxb_txz / dx + dzf_tzz / dz)
+        vx[:, I, J] += dt * BX * (dxf_txx / dxs + dzb_txz / dzs)
+        vz[:, I, J] += dt * BZ * (dxb_txz / dxs + dzf_tzz / dzs)
         if free_surface:
             J1 = slice(nn + 1, X - nn + 1)
             vz[:, h - 2, J] = vz[:, h - 1, J]

@@ -23,6 +23,14 @@ def waveform_l2(obs: torch.Tensor, syn: torch.Tensor, dt: float) -> torch.Tensor
     return torch.sum(torch.sqrt(per_trace))
 
 
+def waveform_l2_squared(obs: torch.Tensor, syn: torch.Tensor, dt: float) -> torch.Tensor:
+    """The least-squares objective 1/2 sum (obs - syn)^2 dt (no counterpart in the reference; the
+    product's Misfit_waveform_L2_squared).  Pinned by its closed form: on traces where
+    waveform_l2's per-trace root r is positive, this equals 1/2 sum r^2."""
+    rsd = obs - syn
+    return 0.5 * torch.sum(rsd * rsd * dt)
+
+
 def driver_misfit(rec: torch.Tensor, obs: torch.Tensor, dt: float, normalize: bool = True) -> torch.Tensor:
     """The misfit of one batch as AcousticFWI forms it (acoustic_fwi.py:148-153): the synthetic
     records normalised per trace when waveform_normalize, then L2 against the (already

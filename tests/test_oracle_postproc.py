@@ -32,6 +32,26 @@ def test_misfit_equals_loops(normalize):
     assert abs(got - _loops_misfit(rec, obs, 0.004, normalize)) <= 1e-12 * abs(got)
 
 
+def test_least_squares_misfit_equals_loops_and_half_the_squared_roots():
+    """waveform_l2_squared: brute-force loops; and on every trace, half the square of
+    waveform_l2's per-trace root (so a dropped 1/2 or dt, or a sum over the wrong axis, fails)."""
+    rng = np.random.default_rng(3)
+    rec = rng.standard_normal((2, 7, 4))
+    obs = rng.standard_normal((2, 7, 4)) * 0.5
+    dt = 0.003
+    got = float(om.waveform_l2_squared(torch.tensor(obs), torch.tensor(rec), dt))
+    loops = sum(0.5 * sum((obs[s, t, r] - rec[s, t, r]) ** 2 * dt for t in range(7))
+                for s in range(2) for r in range(4))
+    assert abs(got - loops) <= 1e-12 * abs(loops)
+    roots = [float(om.waveform_l2(torch.tensor(obs[s:s + 1, :, r:r + 1]), torch.tensor(rec[s:s + 1, :, r:r + 1]), dt))
+             for s in range(2) for r in range(4)]
+    assert abs(got - 0.5 * sum(x * x for x in roots)) <= 1e-12 * abs(got)
+    # gradient: d/d syn = -(obs - syn) dt (known answer)
+    syn = torch.tensor(rec, requires_grad=True)
+    om.waveform_l2_squared(torch.tensor(obs), syn, dt).backward()
+    assert np.allclose(syn.grad.numpy(), -(obs - rec) * dt, rtol=1e-12, atol=0)
+
+
 def test_misfit_known_answer():
     # one trace [1, -2, 0.5] normalised by 2 against zeros: sum of squares 0.25 + 1 + 0.0625
     rec = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64).view(1, 3, 1)

@@ -46,6 +46,52 @@ def test_trace_l2_kernel_matches_oracle(normalize, shape):
         assert rel_l2(b.grad.cpu().numpy(), rb.grad.numpy()) <= 1e-5
 
 
+@pytest.mark.parametrize("shape", [(3, 240, 37), (1, 1001, 64)])
+@pytest.mark.parametrize("normalize", [False, True])
+def test_least_squares_mode_matches_oracle(normalize, shape):
+    """ADFWI_L2_SQUARED: 1/2 sum (n - b)^2 dt and its adjoint sources against float64 autograd of
+    the oracle expression, with and without the trace normalisation."""
+    from adfwi_amd.fwi.misfit import NORMALIZE, SQUARED, _TraceL2
+    from oracle import misfit as om
+    S, nt, R = shape
+    rec, obs = _records(6, S, nt, R)
+    dt = 0.002
+    a = torch.tensor(rec, device=DEV, requires_grad=True)
+    b = torch.tensor(obs, device=DEV, requires_grad=True)
+    loss = _TraceL2.apply(a, b, dt, SQUARED | (NORMALIZE if normalize else 0))
+    loss.backward()
+    ra = torch.tensor(rec, dtype=torch.float64, requires_grad=True)
+    rb = torch.tensor(obs, dtype=torch.float64, requires_grad=True)
+    ref = om.waveform_l2_squared(om.normalize_traces(ra) if normalize else ra, rb, dt)
+    ref.backward()
+    assert abs(float(loss) - float(ref)) <= 1e-6 * abs(float(ref)), (float(loss), float(ref))
+    assert rel_l2(a.grad.cpu().numpy(), ra.grad.numpy()) <= 1e-5
+    assert rel_l2(b.grad.cpu().numpy(), rb.grad.numpy()) <= 1e-5
+
+
+def test_zero_residual_trace_nan_for_root_zero_for_least_squares():
+    """A trace whose synthetic and observed samples are all equal (a far-offset trace that no
+    arrival reaches) has a NaN adjoint source under the reference's per-trace square root, as
+    torch autograd gives there, and a zero one under the least-squares objective; the other
+    traces are unaffected."""
+    from adfwi_amd.fwi import Misfit_waveform_L2, Misfit_waveform_L2_squared
+    from oracle import misfit as om
+    rec, obs = _records(7, 2, 120, 5)
+    rec[1, :, 3] = 0.0
+    obs[1, :, 3] = 0.0
+    ref = torch.tensor(rec, dtype=torch.float64, requires_grad=True)
+    om.waveform_l2(torch.tensor(obs, dtype=torch.float64), ref, 0.01).backward()
+    assert torch.isnan(ref.grad[1, :, 3]).all()
+    for cls, nan in ((Misfit_waveform_L2, True), (Misfit_waveform_L2_squared, False)):
+        a = torch.tensor(rec, device=DEV, requires_grad=True)
+        cls(0.01).forward(torch.tensor(obs, device=DEV), a).backward()
+        g = a.grad.cpu()
+        assert bool(torch.isnan(g[1, :, 3]).all()) == nan, cls.__name__
+        assert torch.isfinite(g[0]).all() and torch.isfinite(g[1, :, :3]).all()
+        if not nan:
+            assert (g[1, :, 3] == 0).all()
+
+
 def test_normalised_misfit_drives_the_propagator_gradient():
     """vp gradient through the drop-in propagator with the fused misfit equals the gradient
     with the reference's torch expressions of the same misfit."""
