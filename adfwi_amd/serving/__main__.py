@@ -36,6 +36,69 @@ def fifo_latencies(arrivals, service):
     return np.array(out)
 
 
+def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_range=(256, 2048),
+                   nt_range=(1000, 6000), shots_range=(1, 64), model_pool=0, shape_pool=0, graph_steps=64,
+                   pack=True, pack_across_models=False, exclusive_s=None, long_charge=1.0, burst=False,
+                   device="cuda:0", log=None):
+    """Runs the seeded stream through the plain path (every request alone, one after the other:
+    the service times and the capacity) and then through the serving loop at each load (arrival
+    rate = load x plain capacity); returns the report dictionary."""
+    from .loop import ServingLoop, run_alone
+    from .requests import make_stream
+    log = log or (lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
+
+    def stream(rate):
+        return make_stream(requests, rate, seed, nz_range=nz_range, nx_range=nx_range, nt_range=nt_range,
+                           shots_range=shots_range, model_pool=model_pool, shape_pool=shape_pool)
+    # warm-up outside the measurement: one small request of each kind and mode
+    warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
+                       shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
+    ServingLoop(device, max_slots=2, graph_steps=graph_steps).run(warm, realtime=False)
+    base = stream(1.0)
+    t0 = time.perf_counter()
+    alone = run_alone(copy.deepcopy(base), device)
+    plain_wall = time.perf_counter() - t0
+    service = np.array([o.end - o.start for o in alone])
+    capacity = len(base) / service.sum()
+    log(f"plain path: {len(base)} requests in {plain_wall:.2f} s, capacity {capacity:.2f}/s")
+    out = {"metric": "serving latency (p50/p99, s) and throughput (requests/s)",
+           "stream": {"requests": requests, "seed": seed,
+                      "mix": "60% acoustic / 40% isotropic elastic, 50% forward / 50% gradient",
+                      "nz": list(nz_range), "nx": list(nx_range), "nt": list(nt_range),
+                      "shots": list(shots_range), "arrivals": "Poisson", "model_pool": model_pool,
+                      "shape_pool": shape_pool},
+           "slots": slots, "packing": pack, "pack_across_models": pack_across_models,
+           "exclusive_s": exclusive_s, "long_charge": long_charge,
+           "graph_steps": graph_steps,
+           "plain_capacity_req_per_s": capacity,
+           "plain_service_p50_s": float(np.percentile(service, 50)),
+           "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
+    digests = {o.rid: o.digest for o in alone}
+    for load in ([0.0] if burst else list(loads)):
+        rate = load * capacity
+        reqs = stream(rate if not burst else 1.0)
+        if burst:
+            for r in reqs:
+                r.arrival = 0.0
+        rep = ServingLoop(device, max_slots=slots, pack=pack, pack_across_models=pack_across_models,
+                          exclusive_s=exclusive_s, long_charge=long_charge,
+                          graph_steps=graph_steps).run(reqs, realtime=not burst)
+        s = rep.summary()
+        fifo = fifo_latencies([r.arrival for r in reqs], service)
+        same = all(o.digest.keys() == digests[o.rid].keys() and all(
+            abs(o.digest[k][1] - digests[o.rid][k][1]) <= 1e-6 * max(abs(digests[o.rid][k][1]), 1e-30)
+            for k in o.digest) for o in rep.outcomes if o.error is None)
+        run = {"load": "burst" if burst else load, "rate_req_per_s": None if burst else rate, "serving": s,
+               "plain_fifo": {"latency_p50_s": float(np.percentile(fifo, 50)),
+                              "latency_p99_s": float(np.percentile(fifo, 99)),
+                              "throughput_req_per_s": len(reqs) / max(fifo[-1] + reqs[-1].arrival, 1e-9)},
+               "results_match_plain": bool(same)}
+        out["runs"].append(run)
+        log(f"load {load}: p50 {s['latency_p50_s']:.3f} s, p99 {s['latency_p99_s']:.3f} s "
+            f"(plain FIFO p50 {run['plain_fifo']['latency_p50_s']:.3f}, p99 {run['plain_fifo']['latency_p99_s']:.3f})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=60)
@@ -59,64 +122,13 @@ def main():
                     help="fraction of the budget charged to those requests (0.51: one at a time)")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
-    from .loop import ServingLoop, run_alone
-    from .requests import make_stream
     nt_range = (max(50, int(1000 * args.scale_nt)), max(60, int(6000 * args.scale_nt)))
-
-    def stream(rate):
-        return make_stream(args.requests, rate, args.seed, nz_range=(128, args.max_nz), nx_range=(256, args.max_nx),
-                           nt_range=nt_range, shots_range=(1, args.max_shots), model_pool=args.model_pool,
-                           shape_pool=args.shape_pool)
-    # warm-up outside the measurement: one small request of each kind and mode
-    warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
-                       shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
-    ServingLoop("cuda:0", max_slots=2, graph_steps=args.graph_steps).run(warm, realtime=False)
-    base = stream(1.0)
-    t0 = time.perf_counter()
-    alone = run_alone(copy.deepcopy(base))
-    plain_wall = time.perf_counter() - t0
-    service = np.array([o.end - o.start for o in alone])
-    capacity = len(base) / service.sum()
-    print(f"[serving] plain path: {len(base)} requests in {plain_wall:.2f} s, capacity {capacity:.2f}/s",
-          file=sys.stderr, flush=True)
-    out = {"metric": "serving latency (p50/p99, s) and throughput (requests/s)",
-           "stream": {"requests": args.requests, "seed": args.seed,
-                      "mix": "60% acoustic / 40% isotropic elastic, 50% forward / 50% gradient",
-                      "nz": [128, args.max_nz], "nx": [256, args.max_nx], "nt": list(nt_range),
-                      "shots": [1, args.max_shots], "arrivals": "Poisson", "model_pool": args.model_pool,
-                      "shape_pool": args.shape_pool},
-           "slots": args.slots, "packing": not args.no_pack, "pack_across_models": args.pack_across_models,
-           "exclusive_s": args.exclusive_s, "long_charge": args.long_charge,
-           "graph_steps": args.graph_steps,
-           "plain_capacity_req_per_s": capacity,
-           "plain_service_p50_s": float(np.percentile(service, 50)),
-           "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
-    digests = {o.rid: o.digest for o in alone}
-    loads = [0.0] if args.burst else [float(x) for x in args.load.split(",")]
-    for load in loads:
-        rate = load * capacity
-        reqs = stream(rate if not args.burst else 1.0)
-        if args.burst:
-            for r in reqs:
-                r.arrival = 0.0
-        rep = ServingLoop("cuda:0", max_slots=args.slots, pack=not args.no_pack,
-                          pack_across_models=args.pack_across_models, exclusive_s=args.exclusive_s,
-                          long_charge=args.long_charge,
-                          graph_steps=args.graph_steps).run(reqs, realtime=not args.burst)
-        s = rep.summary()
-        fifo = fifo_latencies([r.arrival for r in reqs], service)
-        same = all(o.digest.keys() == digests[o.rid].keys() and all(
-            abs(o.digest[k][1] - digests[o.rid][k][1]) <= 1e-6 * max(abs(digests[o.rid][k][1]), 1e-30)
-            for k in o.digest) for o in rep.outcomes if o.error is None)
-        run = {"load": "burst" if args.burst else load, "rate_req_per_s": None if args.burst else rate, "serving": s,
-               "plain_fifo": {"latency_p50_s": float(np.percentile(fifo, 50)),
-                              "latency_p99_s": float(np.percentile(fifo, 99)),
-                              "throughput_req_per_s": len(reqs) / max(fifo[-1] + reqs[-1].arrival, 1e-9)},
-               "results_match_plain": bool(same)}
-        out["runs"].append(run)
-        print(f"[serving] load {load}: p50 {s['latency_p50_s']:.3f} s, p99 {s['latency_p99_s']:.3f} s "
-              f"(plain FIFO p50 {run['plain_fifo']['latency_p50_s']:.3f}, p99 {run['plain_fifo']['latency_p99_s']:.3f})",
-              file=sys.stderr, flush=True)
+    out = measure_stream(args.requests, [float(x) for x in args.load.split(",")], slots=args.slots, seed=args.seed,
+                         nz_range=(128, args.max_nz), nx_range=(256, args.max_nx), nt_range=nt_range,
+                         shots_range=(1, args.max_shots), model_pool=args.model_pool, shape_pool=args.shape_pool,
+                         graph_steps=args.graph_steps, pack=not args.no_pack,
+                         pack_across_models=args.pack_across_models, exclusive_s=args.exclusive_s,
+                         long_charge=args.long_charge, burst=args.burst)
     print(json.dumps(out))
 
 

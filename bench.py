@@ -15,10 +15,13 @@ checkpoint interval ceil(sqrt(nt)) = 64.  Synthetic seeded models; the
 observed data are modelled from the true model before timing.
 
 Prints one JSON line: metric, value (shot-gradients/s), roofline of the
-dominant kernel (HIP-event timed inside the timed region), cpu_baseline
-(the CPU oracle on the 100 x 200 / 10-shot / nt 1000 workload, bounded sample)
-and the plain-autograd path on the same GPU (bounded sample, extrapolated
-per time step).
+dominant kernel (HIP-event timed inside the timed region; algorithmic and
+counted-traffic fractions), cpu_baseline (the CPU oracle on the 100 x 200 /
+10-shot / nt 1000 workload, all steps, median of 3), the plain-autograd path on
+the same GPU (the whole workload, ceil(sqrt(nt)) checkpoint segments) and, after
+all timing, a serving leg: p50/p99 latency and throughput of a bounded stream
+drawn from the 2000-request serving workload's ranges at a Poisson rate of 1x
+the plain path's measured capacity, beside that plain path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload acoustic_large]
 """
@@ -137,50 +140,76 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_budget=24.0):
-    """The CPU oracle (eager torch restatement of the reference path, fp32) on the
-    100 x 200 / 10-shot / nt 1000 workload: forward + L2 + autograd backward to vp.grad."""
+def cpu_cores():
+    """(threads, host physical cores): the threads the CPU baseline uses are the cores this process
+    may run on (its affinity), at most OMP_NUM_THREADS when that is set (a GPU's share of the host
+    on a shared box); the host's physical core count is reported beside it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    phys = set()
+    try:
+        pid = cid = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":")[1].strip()
+                    phys.add((pid, cid))
+    except OSError:
+        pass
+    return n, (len(phys) or None)
+
+
+def cpu_baseline(repeats=3):
+    """The CPU oracle (eager torch restatement of the reference path, fp32, the reference's op
+    order, autograd) on the 100 x 200 / 10-shot / nt 1000 workload: forward of all shots, the
+    least-squares misfit against data the oracle models from the true model, and the backward to
+    vp.grad; one short warm-up, then the median of `repeats` full runs."""
     from adfwi_amd.propagator.boundary_condition import bc_pml
     from adfwi_amd.workloads import acoustic_small
     from oracle import acoustic as oa
-    cores = min(16, os.cpu_count() or 1)
-    try:
-        cores = min(cores, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        pass
+    from oracle import misfit as om
+    cores, phys = cpu_cores()
     torch.set_num_threads(cores)
     wl = acoustic_small()
     nz, nx, nabc = wl.nz, wl.nx, wl.nabc
     damp = torch.tensor(bc_pml(nx, nz, wl.dx, wl.dx, nabc, float(wl.models["vp_init"].max()), False),
                         dtype=torch.float32)
     wav = torch.tensor(wl.wavelets())
-    args = (torch.tensor(wl.src_x), torch.tensor(wl.src_z), wl.nshots, wav, torch.tensor(wl.rcv_x),
-            torch.tensor(wl.rcv_z), len(wl.rcv_x), damp)
+    args = (torch.tensor(wl.src_x), torch.tensor(wl.src_z), wl.nshots)
+    rcv = (torch.tensor(wl.rcv_x), torch.tensor(wl.rcv_z), len(wl.rcv_x), damp)
+
+    def forward(vp, rho, nt_run):
+        return oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, nabc, False, *args, wav[:, :nt_run], *rcv,
+                                 vp, rho, checkpoint_segments=1)["p"]
+    with torch.no_grad():
+        obs = forward(torch.tensor(wl.models["vp_true"]), torch.tensor(wl.models["rho_true"]), wl.nt)
     rho = torch.tensor(wl.models["rho_init"])
 
-    def one(nt_run, obs=None):
+    def one(nt_run):
         v = torch.tensor(wl.models["vp_init"], requires_grad=True)
-        res = oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, nabc, False, *args[:3],
-                                args[3][:, :nt_run], *args[4:], v, rho, checkpoint_segments=1)
-        target = torch.zeros_like(res["p"]) if obs is None else obs
-        oa.l2_misfit(target, res["p"], wl.dt).backward()
+        om.waveform_l2_squared(obs[:, :nt_run], forward(v, rho, nt_run), wl.dt).backward()
         return v.grad
-    # bounded sample: full shots and grid, time steps cut so the sample fits the budget
-    t0 = time.perf_counter()
-    one(50)
-    per_step = (time.perf_counter() - t0) / 50
-    nt_run = int(max(50, min(wl.nt, seconds_budget / 3 / max(per_step, 1e-9))))
+    one(100)
     times = []
-    for _ in range(3):
+    for _ in range(repeats):
         t0 = time.perf_counter()
-        one(nt_run)
+        one(wl.nt)
         times.append(time.perf_counter() - t0)
-    t = float(np.median(times)) * wl.nt / nt_run
+    t = float(np.median(times))
     N = (nz + 2 * nabc) * (nx + 2 * nabc)
     return {"value": wl.nshots / t, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
-            "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward+L2+backward, "
-                       f"100x200 grid, 20-cell PML, 10 shots, {nt_run} of 1000 steps, median of 3, "
-                       f"scaled to nt=1000"),
+            "host_physical_cores": phys,
+            "sample": (f"oracle (eager torch fp32, reference op order, autograd) forward + least-squares misfit + "
+                       f"backward, 100x200 grid, 20-cell PML, 10 shots, all 1000 steps, median of {repeats} "
+                       f"({', '.join(f'{x:.2f}' for x in times)} s) after a warm-up, {cores} threads "
+                       f"(this process's cores, capped at the GPU's host share)"),
             "gpt_updates_per_s": 2 * wl.nshots * wl.nt * N / t / 1e9,
             "cpu": _cpu_name()}
 
@@ -191,11 +220,7 @@ def elastic_cpu_baseline(wl, seconds_budget=24.0, fd_order=4, abc="PML"):
     from adfwi_amd.propagator.boundary_condition import bc_gerjan, bc_pml_xz, bc_sincos
     from oracle import elastic as oe
     from oracle import model as om
-    cores = min(16, os.cpu_count() or 1)
-    try:
-        cores = min(cores, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        pass
+    cores, _ = cpu_cores()
     torch.set_num_threads(cores)
     m = {k: torch.tensor(v) for k, v in wl.models.items()}
     nz, nx = wl.nz, wl.nx
@@ -255,35 +280,54 @@ def _cpu_name():
     return "unknown"
 
 
-def plain_gpu_baseline(wl, dev, nt_sample=200):
-    """The same oracle (plain autograd path, reentrant checkpointing with ceil(sqrt(nt)) segments)
-    on the GPU for the benchmark workload, nt_sample steps, extrapolated per time step."""
+def plain_gpu_baseline(wl, dev, obs):
+    """The oracle (the reference's plain autograd path: eager torch, TorchScript steps, reentrant
+    checkpointing) on the GPU for the whole benchmark workload: all shots and steps, ceil(sqrt(nt))
+    checkpoint segments, the same least-squares misfit against the same observed data, backward to
+    vp.grad; timed once after a short warm-up."""
     from adfwi_amd.propagator.boundary_condition import bc_pml
     from oracle import acoustic as oa
+    from oracle import misfit as om
     nz, nx, nabc = wl.nz, wl.nx, wl.nabc
     damp = torch.tensor(bc_pml(nx, nz, wl.dx, wl.dx, nabc, float(wl.models["vp_init"].max()), False),
                         dtype=torch.float32, device=dev)
-    wav = torch.tensor(wl.wavelets()[:, :nt_sample], device=dev)
+    wav = torch.tensor(wl.wavelets(), device=dev)
     rho = torch.tensor(wl.models["rho_init"], device=dev)
-    segs = int(math.ceil(math.sqrt(nt_sample)))
+    segs = int(math.ceil(math.sqrt(wl.nt)))
 
-    def one():
+    def one(nt_run, nseg):
         v = torch.tensor(wl.models["vp_init"], device=dev, requires_grad=True)
-        res = oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_sample, wl.dt, nabc, False,
+        res = oa.forward_kernel(nx, nz, wl.dx, wl.dx, nt_run, wl.dt, nabc, False,
                                 torch.tensor(wl.src_x, device=dev), torch.tensor(wl.src_z, device=dev),
-                                wl.nshots, wav, torch.tensor(wl.rcv_x, device=dev),
+                                wl.nshots, wav[:, :nt_run], torch.tensor(wl.rcv_x, device=dev),
                                 torch.tensor(wl.rcv_z, device=dev), len(wl.rcv_x), damp, v, rho,
-                                checkpoint_segments=segs, device=dev)
-        oa.l2_misfit(torch.zeros_like(res["p"]), res["p"], wl.dt).backward()
-    one()
+                                checkpoint_segments=nseg, device=dev)
+        om.waveform_l2_squared(obs[:, :nt_run], res["p"], wl.dt).backward()
+    one(2 * (wl.nt // segs), 2)     # warm-up: TorchScript profiles the step function
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    one()
+    one(wl.nt, segs)
     torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) * wl.nt / nt_sample
-    return {"value": wl.nshots / t, "unit": "shot-gradients/s",
-            "sample": f"oracle on GPU, {nt_sample} of {wl.nt} steps, {segs} checkpoint segments, "
-                      f"extrapolated per step"}
+    t = time.perf_counter() - t0
+    return {"value": wl.nshots / t, "unit": "shot-gradients/s", "seconds": t,
+            "sample": f"oracle on the GPU, the whole workload: {wl.nshots} shots, all {wl.nt} steps, "
+                      f"{segs} checkpoint segments, least-squares misfit, timed once after a warm-up"}
+
+
+def serving_leg(requests=32, load=1.0):
+    """The serving loop on a bounded stream drawn from the 2000-request serving workload's own
+    ranges (seed 4), at a Poisson arrival rate of `load` x the plain path's measured capacity,
+    beside the plain path (every request alone, first come first served)."""
+    from adfwi_amd.serving.__main__ import measure_stream
+    rep = measure_stream(requests, [load], log=lambda m: log("serving: " + m))
+    run = rep["runs"][0]
+    s = run["serving"]
+    return {"stream": rep["stream"], "slots": rep["slots"], "rate_req_per_s": run["rate_req_per_s"],
+            "load_x_plain_capacity": load, "latency_p50_s": s["latency_p50_s"], "latency_p99_s": s["latency_p99_s"],
+            "throughput_req_per_s": s["throughput_req_per_s"], "completed": s["requests"] - s["failed"],
+            "max_concurrent": s.get("max_concurrent"),
+            "plain_fifo": run["plain_fifo"], "plain_capacity_req_per_s": rep["plain_capacity_req_per_s"],
+            "results_match_plain": run["results_match_plain"]}
 
 
 def main():
@@ -298,6 +342,10 @@ def main():
     ap.add_argument("--abc", default="PML", choices=("PML", "gerjan", "sincos"),
                     help="elastic workloads: absorbing boundary (gerjan / sincos: the damping scheme)")
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=32,
+                    help="requests of the serving leg (0: none): a bounded stream from the serving workload's ranges")
+    ap.add_argument("--serving-load", type=float, default=1.0,
+                    help="serving leg arrival rate as a multiple of the plain path's measured capacity")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -370,9 +418,14 @@ def main():
         kern[name] = {"total_ms": d["ms"], "launches": d["launches"], "steps": d["steps"],
                       "steps_by_call": d["steps_by_call"], "avg_launch_ms": avg_ms,
                       "achieved_gbs": byts / (d["ms"] / 1e3) / 1e9 if d["ms"] else 0.0,
-                      "bytes_per_launch": byts / max(d["launches"], 1)}
+                      "bytes_per_launch": byts / max(d["launches"], 1), "bytes_total": byts}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     ach = kern[dom]["achieved_gbs"]
+    traffic = profiled_traffic(dom, wl.name)
+    # bytes of the steps actually executed (resident segments skip part of the recompute) against
+    # the specified algorithm's per-pt-step figure
+    executed = sum(k["bytes_total"] for k in kern.values())
+    spec_bytes = ((EL_BYTES_GRAD / (2 if args.abc != "PML" else 1)) if elastic else BYTES_GRAD) * S * N * wl.nt
     out = {
         "metric": "shot-gradients/s",
         "value": value,
@@ -390,11 +443,17 @@ def main():
                    "nt": wl.nt, "shots_per_launch": S, "receivers": len(wl.rcv_x), "dt": wl.dt, "dx": wl.dx,
                    "checkpoint_interval": ak.checkpoint_interval(wl.nt), "parallelism": "single GPU, shot batch"},
         "gpt_updates_per_s": 2 * S * N * wl.nt * args.steps / elapsed / 1e9,
-        "grad_bytes_roofline_frac": ((EL_BYTES_GRAD / (2 if args.abc != "PML" else 1)) if elastic else BYTES_GRAD)
-                                    * S * N * wl.nt * args.steps
-                                    / elapsed / 1e9 / HBM_PEAK_GBS,
+        # the executed steps' algorithmic bytes over the whole step time, against 8 TB/s
+        "grad_bytes_roofline_frac": executed / elapsed / 1e9 / HBM_PEAK_GBS,
+        "executed_bytes_per_shot_gradient": executed / (S * args.steps),
+        # the specified algorithm's bytes (every segment recomputed) over the same time
+        "spec_grad_bytes_roofline_frac": spec_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, wl.name),
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     # counted HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE) over the launch time
+                     "traffic_gbs": traffic / (kern[dom]["avg_launch_ms"] / 1e3) / 1e9 if traffic else None,
+                     "traffic_frac": (traffic / (kern[dom]["avg_launch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
+                                      if traffic else None),
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "avg_launch_ms": kern[dom]["avg_launch_ms"]},
         "kernels": kern,
@@ -406,10 +465,15 @@ def main():
         out["config"]["abc_type"] = args.abc
         out["config"]["free_surface"] = wl.free_surface
         out["config"]["components"] = ["vx", "vz"]
+    for p in model.parameters():
+        p.grad = None
+    del prop, model
+    torch.cuda.empty_cache()
     if not args.no_baselines and not elastic:
-        out["plain_autograd_gpu"] = plain_gpu_baseline(wl, dev)
-        log("plain autograd GPU baseline done")
+        out["plain_autograd_gpu"] = plain_gpu_baseline(wl, dev, obs)
+        log(f"plain autograd GPU baseline done ({out['plain_autograd_gpu']['seconds']:.1f} s)")
         out["speedup_vs_plain_autograd_gpu"] = value / out["plain_autograd_gpu"]["value"]
+        torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline()
         log("CPU baseline done")
     elif not args.no_baselines:
@@ -417,6 +481,12 @@ def main():
         log("CPU baseline done")
     else:
         out["cpu_baseline"] = None
+    del obs
+    torch.cuda.empty_cache()
+    if args.serving_requests > 0:
+        out["serving"] = serving_leg(args.serving_requests, args.serving_load)
+        log(f"serving leg done: p50 {out['serving']['latency_p50_s']:.2f} s, "
+            f"p99 {out['serving']['latency_p99_s']:.2f} s")
     print(json.dumps(out))
 
 
