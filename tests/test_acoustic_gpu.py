@@ -304,3 +304,59 @@ def test_resident_images_equal_recomputed_segments(monkeypatch, rho_grad):
     for resident in ("3", "100", None):
         for k, a in out["0"].items():
             assert np.array_equal(out[resident][k], a), (resident, k, rel_l2(out[resident][k], a))
+
+
+def test_resident_budget_peak_stays_within_the_free_memory_it_assumed():
+    """The resident-image budget (80 % of the free memory less 8 GiB, less the checkpoints and the
+    backward's buffers) against the real peak allocation of a gradient: a blocker tensor leaves
+    about 24 GiB free so that the budget binds (some but not all segments resident), then one
+    32-shot 464 x 1232 gradient (nt 1000, 32-step segments of 2.4 GB of images) must allocate at
+    most 80 % of the free memory it was given, and its gradient must equal the run with every
+    segment recomputed."""
+    from adfwi_amd.propagator import acoustic_kernels as ak
+    from adfwi_amd.propagator.boundary_condition import bc_pml
+    nz, nx, nabc, nt, S = 384, 1152, 40, 1000, 32
+    rng = np.random.default_rng(8)
+    vp = (2500 + 300 * rng.random((nz, nx))).astype(np.float32)
+    rho = (310 * vp ** 0.25).astype(np.float32)
+    damp = torch.tensor(bc_pml(nx, nz, 12.5, 12.5, nabc, float(vp.max()), False), dtype=torch.float32, device=DEV)
+    Z, X = nz + 2 * nabc, nx + 2 * nabc
+    sx, sz = np.linspace(5, nx - 6, S).astype(int), np.full(S, 2)
+    rx, rz = np.arange(nx), np.full(nx, 2)
+    wav = torch.tensor(rng.standard_normal((S, nt)), dtype=torch.float32, device=DEV)
+    k = ak.checkpoint_interval(nt)
+    nseg = -(-nt // k)
+
+    def gradient(resident):
+        geom = ak.AcousticGeometry(Z, X, nt, False, nabc, sz + nabc, sx + nabc, rz + nabc, rx + nabc, DEV,
+                                   resident_images=resident)
+        v = torch.tensor(vp, device=DEV, requires_grad=True)
+        res = ak.forward_kernel(nx, nz, 12.5, 12.5, nt, 1e-3, nabc, False, torch.tensor(sx), torch.tensor(sz), S,
+                                wav, torch.tensor(rx), torch.tensor(rz), nx, damp, v, torch.tensor(rho, device=DEV),
+                                device=DEV, geometry=geom)
+        (res["p"].double() ** 2).sum().backward()
+        return v.grad.detach(), geom
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0, _ = torch.cuda.mem_get_info(DEV)
+    blocker = torch.empty(max(0, free0 - (40 << 30)) // 4, dtype=torch.float32, device=DEV)
+    try:
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated(DEV)
+        free, _ = torch.cuda.mem_get_info(DEV)
+        free += torch.cuda.memory_reserved(DEV) - base
+        torch.cuda.reset_peak_memory_stats(DEV)
+        probe = ak.AcousticGeometry(Z, X, nt, False, nabc, sz + nabc, sx + nabc, rz + nabc, rx + nabc, DEV)
+        nres = ak.resident_segments(probe, k, True, False)
+        g_res, _ = gradient(None)
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated(DEV) - base
+        print(f"free {free / 2**30:.1f} GiB, resident segments {nres} of {nseg}, peak {peak / 2**30:.1f} GiB",
+              flush=True)
+        assert 0 < nres < nseg, (nres, nseg)
+        assert peak <= 0.8 * free, (peak / 2**30, free / 2**30)
+    finally:
+        del blocker
+        torch.cuda.empty_cache()
+    g_none, _ = gradient(0)
+    assert torch.equal(g_res, g_none)
