@@ -102,6 +102,11 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
   const rsrc_t rIn = rsrc(in, (unsigned)NFIELD * fieldb);
   const rsrc_t rOut = rsrc(out, (unsigned)NFIELD * fieldb);
   const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  // per-row PML factors, read as part of each row's loads (uniform-address vector loads queue
+  // behind the row data; a separate read would make the wave wait for every load in flight)
+  const unsigned zB = 4u * (unsigned)d.Z;
+  const rsrc_t rPz = rsrc(d.pml_z, 3u * zB);
+  auto zo = [&](int k) -> unsigned { return 4u * (unsigned)min(max(k, 0), zmax); };
   const float c0 = d.c0, c1 = d.c1;
   // per-column PML factors (x); rows come from pml_z
   const float pxn = d.pml_x[min(max(c, 0), d.X - 1)];
@@ -120,6 +125,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
     const int zs = d.src_z[s], xs = d.src_x[s];
     const float wsrc = d.src[(long long)s * d.nt + t];
     const float mxx = d.src_m[3 * s], mzz = d.src_m[3 * s + 1], mxz = d.src_m[3 * s + 2];
+    const float fz = d.src_fz ? d.src_fz[s] : 0.0f;
     // windows: velocity totals rows k-2..k+2; stress totals txz/tzz rows k-4..k, txx rows k-2..k
     float vx0 = 0.f, vx1 = 0.f, vx2 = 0.f, vx3 = 0.f, vx4 = 0.f;
     float vz0 = 0.f, vz1 = 0.f, vz2 = 0.f, vz3 = 0.f, vz4 = 0.f;
@@ -135,11 +141,12 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = ok ? a : 0.f;
       vz1 = vz2; vz2 = vz3; vz3 = vz4; vz4 = ok ? b : 0.f;
     }
-    // everything iteration k reads from memory, loaded one iteration ahead: velocity splits of
+    // everything iteration k reads from memory, loaded two iterations ahead: velocity splits of
     // row k+2, stress splits and moduli of row k, buoyancies of row k-2.  The old velocity splits
     // of row k-2 come from a register window of the splits loaded four iterations before.
     struct Row {
       float v[4], s[6], cm[ANISO ? 6 : 4], b[2];
+      float pz[4];   // pml_z: 1 - dt/2 d and 1/(1 + dt/2 d) of row k; 1 - dt/2 d and 1 + dt/2 d of row k-2
     };
     float us[5][4] = {};                         // velocity splits of rows k-2 .. k+2
     auto load = [&](int k, Row &R) {
@@ -156,14 +163,14 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
       if (k - 2 >= z0 && k - 2 < end_row) {   // only rows this strip updates
         R.b[0] = bld(rC, vc, BX * planeb + om); R.b[1] = bld(rC, vc, BZ * planeb + om);
       }
+      R.pz[0] = bld(rPz, 0u, zo(k)); R.pz[1] = bld(rPz, 0u, 2u * zB + zo(k));
+      R.pz[2] = bld(rPz, 0u, zo(k - 2)); R.pz[3] = bld(rPz, 0u, zB + zo(k - 2));
     };
-    Row cur;
-    load(z0 - 2, cur);
-#pragma unroll 1
-    for (int kk = 0; kk < d.tz + 4; ++kk) {
+    // three row buffers in rotation: the loads of row k+2 go out before row k is computed, into
+    // the buffer row k-1 has just freed, so two rows of work cover each load and no register
+    // holding a load in flight is ever copied (a copy waits for every load issued before it)
+    auto step = [&](const int kk, const Row &cur) {
       const int k = z0 - 2 + kk;
-      Row nxt;
-      load(k + 1, nxt);      // next row's loads go out before this row's stores
       // 1. velocity totals of row k+2 into the window (zero outside the grid)
       {
         const bool ok = k + 2 <= zmax;
@@ -208,7 +215,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float dxf_vz = c0 * (vzp1 - vz2) + c1 * (vzp2 - vzm1);
         const float dzf_vx = c0 * (vx3 - vx2) + c1 * (vx4 - vx1);
         const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
-        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float pzn = cur.pz[0], pzi = cur.pz[1];
         float sx_xx, sz_xx, sx_zz, sz_zz, sx_xz, sz_xz;
         if constexpr (ANISO) {
           const float dxb_vz = c0 * (vz2 - vzm1) + c1 * (vzp1 - vzm2);
@@ -270,20 +277,34 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
         const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
         const float bxv = cur.b[0], bzv = cur.b[1];
-        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float pzn = cur.pz[2], pzd = cur.pz[3];
         const float nx_x = (pxn * us[0][0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
         const float nx_z = (pzn * us[0][1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
         float nz_x = (pxn * us[0][2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
         float nz_z = (pzn * us[0][3] + ((d.dt * bzv) * dzf_tzz) / d.dz) / pzd;
         if (d.src_fz && r == zs && c == xs) {   // vertical force: half of f w into each vz split
-          const float hf = 0.5f * (d.src_fz[s] * wsrc);
+          const float hf = 0.5f * (fz * wsrc);
           nz_x = nz_x + hf;
           nz_z = nz_z + hf;
         }
         bst(nx_x, rOut, vs, F(VX_X) + orr); bst(nx_z, rOut, vs, F(VX_Z) + orr);
         bst(nz_x, rOut, vs, F(VZ_X) + orr); bst(nz_z, rOut, vs, F(VZ_Z) + orr);
       }
-      cur = nxt;
+    };
+    Row R0, R1, R2;
+    load(z0 - 2, R0);
+    load(z0 - 1, R1);
+    // iterations rounded up to a multiple of three: the extra rows lie below the strip, where a
+    // step stores nothing (branch-free unrolled steps keep the compiler's load waits exact)
+    const int nit = (d.tz + 4 + 2) / 3 * 3;
+#pragma unroll 1
+    for (int kk = 0; kk < nit; kk += 3) {
+      load(z0 + kk, R2);
+      step(kk, R0);
+      load(z0 + kk + 1, R0);
+      step(kk + 1, R1);
+      load(z0 + kk + 2, R1);
+      step(kk + 2, R2);
     }
   }
 }
@@ -326,19 +347,25 @@ __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restri
 //      Lv_x[3,j] += Lv_x[1,j];
 //   f. L(v) = pn / pd L(v') + Lv for both splits.
 // Inside the window only; the splits outside it never change, so their adjoints are not needed.
-// Lanes and rows march exactly as in the forward step.  Each wavefront accumulates the
-// gradients of its shots in its own LDS slab (one slot per lane, so no synchronisation) and
-// adds the slab to its group's planes once per launch.
-constexpr int ETZA = 8;   // strip rows of the adjoint (bounded by the LDS slab)
+// Lanes and rows march exactly as in the forward step.  The four wavefronts of a workgroup
+// (four shots of one strip) march their rows in lockstep: every iteration each writes its
+// gradient terms of the rows it finished to an LDS row buffer, and after one barrier the
+// wavefront that owns a coefficient field adds the four terms, in wavefront order, to the
+// workgroup's plane of that field (its old value was loaded at the start of the iteration).
+// Nothing is held per strip, so strips can be tall: each strip re-reads four halo rows of
+// every field, 4/tz of its loads.
+constexpr int ETZA = 8;   // strip rows of the adjoints that keep a per-strip LDS gradient slab
 
 template <bool ANISO>
 __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
                                                    const float *__restrict__ fv, const float *__restrict__ fsn, int t,
                                                    float *__restrict__ gacc, float *__restrict__ gsrc) {
   constexpr int NGF = ANISO ? 8 : 6;
-  __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  // gradient terms of the rows finished in one iteration, per wavefront, double-buffered by
+  // iteration parity (one barrier per iteration keeps a buffer from being rewritten early)
+  __shared__ float rowbuf[2][EWPB][NGF][WAVE];
   int z0, x0;
-  if (!strip_of(d, z0, x0)) return;
+  if (!strip_of(d, z0, x0)) return;   // uniform over the workgroup (one strip)
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - EHL + lane;
@@ -364,15 +391,17 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
   const bool jj1 = c + 1 >= 2 && c + 1 <= d.X - 3;
   const bool jm1 = c - 1 >= 2 && c - 1 <= d.X - 3;
   const int grp = blockIdx.y * EWPB + wv;
-  float *sl = slab + wv * (NGF * ETZA * WAVE) + lane;
-  for (int q = 0; q < NGF * ETZA; ++q) sl[q * WAVE] = 0.0f;
-  auto acc = [&](int f, int row, float v) { sl[(f * ETZA + (row - z0)) * WAVE] += v; };
+  // this wavefront adds fields f = wv and wv + 4 (C fields of row k, b fields 4, 5 of row k-2)
+  const long long planef = (long long)d.Z * d.pitch;
+  float *gplane = gacc + (long long)blockIdx.y * NCOEF * planef + cc;
+  float gf[NGF];
 
-  bool any = false;
+  // every wavefront runs the same number of iterations (the barrier is in the row loop); one
+  // whose shot is past the last computes nothing and contributes zeros
   for (int si = 0; si < d.spw; ++si) {
-    const int s = grp * d.spw + si;
-    if (s >= d.S) break;
-    any = true;
+    const int s_raw = grp * d.spw + si;
+    const bool active = s_raw < d.S;
+    const int s = active ? s_raw : d.S - 1;   // in-range indices for the (skipped) shot set-up
     const unsigned so = (unsigned)s * planeb;
     auto F = [&](int f) -> unsigned { return (unsigned)f * fieldb + so; };
     const int zs = d.src_z[s], xs = d.src_x[s];
@@ -416,7 +445,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         ed = bld(rLi, vc, F(VZ_Z) + o) * (vz_w * bzv);
       }
     };
-    for (int q = -2; q <= 1; ++q) {
+    for (int q = -2; q <= 1 && active; ++q) {
       float fvx, fvz, ea, eb, ec, ed;
       load_row(z0 - 2 + q, fvx, fvz, ea, eb, ec, ed);
       vx1 = vx2; vx2 = vx3; vx3 = vx4; vx4 = fvx;
@@ -436,6 +465,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
     // velocity stage needs, so they are not loaded a second time)
     float lw[5][4] = {};
     auto load = [&](int k, Row &R) {
+      if (!active) return;
       const unsigned o2 = rb(k + 2), o0 = rb(k);
 #pragma unroll
       for (int q = 0; q < 4; ++q) R.fv[q] = bld(rFv, vc, F(VX_X + q) + o2);
@@ -457,8 +487,20 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
 #pragma unroll 1
     for (int kk = 0; kk < d.tz + 4; ++kk) {
       const int k = z0 - 2 + kk;
+      // old plane values of this wavefront's fields at the rows finished this iteration
+      float gold[2] = {0.0f, 0.0f};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int f = wv + j * EWPB;
+        const int row = (f == 4 || f == 5) ? k - 2 : k;
+        if (f < NGF && out_col && row >= z0 && row < end_row)
+          gold[j] = gplane[(long long)f * planef + (long long)row * d.pitch];
+      }
+#pragma unroll
+      for (int f = 0; f < NGF; ++f) gf[f] = 0.0f;
       Row nxt;
       load(k + 1, nxt);      // next row's loads go out before this row's stores
+      if (active) {
       {
         const int rr = k + 2;
         const bool in_grid = rr <= zmax;
@@ -560,15 +602,15 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
             const float dzb_vx = c0 * (vx2 - vx1) + c1 * (vx3 - vx0);
             const float dxf_vx = c0 * (vxp1 - vx2) + c1 * (vxp2 - vxm1);
             const float dzf_vz = c0 * (vz3 - vz2) + c1 * (vz4 - vz1);
-            acc(6, k, axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx);
-            acc(7, k, azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz);
+            gf[6] = axx_x * dxb_vz + axx_z * dzb_vx + axz_x * dxf_vx;
+            gf[7] = azz_x * dxb_vz + azz_z * dzb_vx + axz_z * dzf_vz;
           }
         }
         if (own_k) {
-          acc(0, k, axx_x * dxb_vx);
-          acc(1, k, axx_z * dzb_vz + azz_x * dxb_vx);
-          acc(2, k, azz_z * dzb_vz);
-          acc(3, k, axz_x * dxf_vz + axz_z * dzf_vx);
+          gf[0] = axx_x * dxb_vx;
+          gf[1] = axx_z * dzb_vz + azz_x * dxb_vx;
+          gf[2] = azz_z * dzb_vz;
+          gf[3] = axz_x * dxf_vz + axz_z * dzf_vx;
         }
       }
       p1a = p1b; p1b = p1c; p1c = P1;
@@ -632,28 +674,28 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float dzb_txz = c0 * (xz2 - xz1) + c1 * (xz3 - xz0);
         const float dxb_txz = c0 * (xz2 - tzm1) + c1 * (tzp1 - tzm2);
         const float dzf_tzz = c0 * (zz3 - zz2) + c1 * (zz4 - zz1);
-        acc(4, r, l1 * (vx_w * dxf_txx) + l2 * (vz_w * dzb_txz));
-        acc(5, r, l3 * (vx_w * dxb_txz) + l4 * (vz_w * dzf_tzz));
+        gf[4] = l1 * (vx_w * dxf_txx) + l2 * (vz_w * dzb_txz);
+        gf[5] = l3 * (vx_w * dxb_txz) + l4 * (vz_w * dzf_tzz);
       }
       cur = nxt;
+      }  // active
+      // the four wavefronts' terms of the finished rows, summed in wavefront order
+      const int par = kk & 1;
+#pragma unroll
+      for (int f = 0; f < NGF; ++f) rowbuf[par][wv][f][lane] = gf[f];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int f = wv + j * EWPB;
+        const int row = (f == 4 || f == 5) ? k - 2 : k;
+        if (f < NGF && out_col && row >= z0 && row < end_row) {
+          const float v = (rowbuf[par][0][f][lane] + rowbuf[par][1][f][lane]) +
+                          (rowbuf[par][2][f][lane] + rowbuf[par][3][f][lane]);
+          gplane[(long long)f * planef + (long long)row * d.pitch] = gold[j] + v;
+        }
+      }
     }
   }
-  // the four wavefronts of the workgroup (four shot sets of the same strip) add their slabs in
-  // LDS, then one read-modify-write of the workgroup's plane set: wavefront w takes the fields
-  // f = w, w+4
-  (void)any;
-  __syncthreads();
-  if (!out_col) return;
-  const long long planef = (long long)d.Z * d.pitch;
-  float *g = gacc + (long long)blockIdx.y * NCOEF * planef + c;
-  const float *sb = slab + lane;
-  for (int f = wv; f < NGF; f += EWPB)
-    for (int row = z0; row < end_row; ++row) {
-      const int o = (f * ETZA + (row - z0)) * WAVE;
-      const float v = (sb[o] + sb[NGF * ETZA * WAVE + o]) + (sb[2 * NGF * ETZA * WAVE + o] + sb[3 * NGF * ETZA * WAVE + o]);
-      float *p = g + (long long)f * planef + (long long)row * d.pitch;
-      *p = *p + v;
-    }
 }
 
 // adds the record adjoints of step t to both splits of each recorded field (free-surface tzz
@@ -2304,14 +2346,33 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   const bool recs = a->grad_rec[0] || a->grad_rec[1] || a->grad_rec[2] || a->grad_rec[3] || a->grad_rec[4];
   if (recs && d->nrec > 0) ADFWI_CHECK_ARG(d->rcv_z && d->rcv_x, "null receiver table pointer");
   ElDev v = el_dev(d, L);
-  // the adjoint marches strips of 4..8 rows (its gradient slab lives in LDS; the free-surface
-  // transposes need rows 0..3 inside the first strip)
+  // the free-surface transposes need rows 0..3 inside the first strip (every strip height here
+  // is at least 8)
   // (damping scheme: strips of ETZA rows, so that no strip but the first reaches the
   // free-surface rows, whose velocity adjoints are state there)
   // (three-term scheme: strips of ETZA rows, so that only the first strip reaches the
   // free-surface rows NN-2 .. NN+1)
-  v.tz = (d->damping || v.nn > 2) ? ETZA : min(max(L.tile_z, 4), ETZA);
-  v.nstrips = ((d->nz - 2 * v.nn + v.tz - 1) / v.tz) * v.nsx;
+  // The 4th-order PML adjoint keeps no per-strip storage: the fewest strips per column (equal
+  // heights, multiples of 4 rows, at most 64) that still give 2048 wavefronts; taller strips
+  // re-read fewer halo rows.  Measured on 242 x 680 / 32 shots and 362 x 1040 / 48 shots, this
+  // beats both 8-row strips and one round of 2048 wavefronts of the tallest strips.
+  // ADFWI_ADJ_TILE_Z sets the height.
+  const int rows = d->nz - 2 * v.nn;
+  if (d->damping || v.nn > 2) {
+    v.tz = ETZA;
+  } else {
+    const char *env = std::getenv("ADFWI_ADJ_TILE_Z");
+    int tz = env && *env ? std::atoi(env) : 0;
+    if (tz < 8 || tz > ETZ_MAX) {
+      tz = 8;
+      for (int n = 1; (rows + n - 1) / n >= 8; ++n) {
+        const int cand = ((rows + n - 1) / n + 3) / 4 * 4;
+        if (cand <= ETZ_MAX && (long long)n * v.nsx * L.ngroups * EWPB >= 2048) { tz = cand; break; }
+      }
+    }
+    v.tz = tz;
+  }
+  v.nstrips = ((rows + v.tz - 1) / v.tz) * v.nsx;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid(v.nstrips, L.ngroups), block(ENT);
   const long long ng = (long long)d->nshots * d->nrec;
