@@ -337,6 +337,10 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
   const bool recs = rec_p != nullptr && d.nrec > 0;
+  // receiver index of each point of a row (-1: none), read with the next row's loads: a separate
+  // read of the row's receiver count would make the wave wait for every load in flight
+  const rsrc_t rM = rsrc(io.rmap, recs ? 4u * ((unsigned)d.Z + (unsigned)d.Z * (unsigned)d.pitch) : 0u);
+  const unsigned mapb = 4u * (unsigned)d.Z;
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -389,6 +393,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const bool want_eu = want_u0 || want_u1;
   float ce0 = energy ? bld(rE, vc, eb + rb(kst - 5)) : 0.0f;
   float ce1 = (energy && want_eu) ? bld(rE, vc, eb + planeb + rb(kst - 5)) : 0.0f;
+  int cR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(kst - 2))) : -1;
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -412,6 +417,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const float nbk3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
     const float ne0 = energy ? bld(rE, vc, eb + rb(qq + 1)) : 0.0f;
     const float ne1 = (energy && want_eu) ? bld(rE, vc, eb + planeb + rb(qq + 1)) : 0.0f;
+    const int nR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(r + 1))) : -1;
 
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
     const bool rowu = cu && r >= fs && r <= d.Z - 2;
@@ -435,10 +441,9 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const float dm = rowp ? 1.0f : 0.0f, dm2 = rowp2 ? 1.0f : 0.0f;   // stencil-image masks
     const bool own_q = out_col && q >= z0 && q < end_row;
     const bool own_qq = out_col && qq >= z0 && qq < end_row;
-    // receiver slots of A's emitted row (row counts first: uniform test, then the lane's index)
+    // receiver slot of A's emitted row at this lane
     int ridx = -1;
-    if (recs && emit && r >= z0 && r < end_row && io.rmap[r] > 0 && own_r)
-      ridx = io.rmap[d.Z + r * d.pitch + c];
+    if (recs && emit && own_r) ridx = cR;
     float epa = 0.0f, eua = 0.0f, epb = 0.0f, eub = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
@@ -522,6 +527,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
       if (want_eu) bst(ce1 + eu, rE, vs, eb + planeb + rqq);
     }
     ce0 = ne0; ce1 = ne1;
+    cR = nR;
     epa0 = epa1; epa1 = epa2; epa2 = epa;
     eua0 = eua1; eua1 = eua2; eua2 = eua;
   }
@@ -773,6 +779,10 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   const rsrc_t rG2 = rsrc(io.grad_a2, A2 ? planeb * gridDim.y : 0u);
   const unsigned gb = planeb * blockIdx.y;
   const bool recs = io.rmap != nullptr && d.nrec > 0;
+  // receiver index of each point of a row (-1: none), read with the next row's loads: a separate
+  // read of the row's receiver count would make the wave wait for every load in flight
+  const rsrc_t rM = rsrc(io.rmap, recs ? 4u * ((unsigned)d.Z + (unsigned)d.Z * (unsigned)d.pitch) : 0u);
+  const unsigned mapb = 4u * (unsigned)d.Z;
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -843,6 +853,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   // old gradient-plane values of rows q (A1) and qq (A2), loaded one iteration ahead
   float cG1 = want_a1 ? bld(rG1, vc, gb + rb(qst)) : 0.0f;
   float cG2 = A2 ? bld(rG2, vc, gb + rb(kst - 5)) : 0.0f;
+  int cR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(kst - 2))) : -1;
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -872,6 +883,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
     const float nG1 = want_a1 ? bld(rG1, vc, gb + rb(q + 1)) : 0.0f;
     const float nG2 = A2 ? bld(rG2, vc, gb + rb(qq + 1)) : 0.0f;
+    const int nR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(r + 1))) : -1;
 
     // step B masks (rows k, r) and step A masks (rows q, qq)
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
@@ -899,7 +911,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float m1k = rowp ? 1.0f : 0.0f, m1q = rowp_q ? 1.0f : 0.0f;
     // receiver slot of B's emitted row r (step t's sources), on every lane of the strip
     int ridx = -1;
-    if (recs && r >= 0 && r < d.Z && io.rmap[r] > 0 && in_col) ridx = io.rmap[d.Z + r * d.pitch + c];
+    if (recs && r >= 0 && r < d.Z && in_col) ridx = cR;
     float ga1b = 0.0f, ga2b = 0.0f, ga1a = 0.0f, ga2a = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
@@ -993,6 +1005,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
     if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
     cG1 = nG1; cG2 = nG2;
+    cR = nR;
     ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
     ga2d0 = ga2d1; ga2d1 = ga2d2; ga2d2 = ga2b;
   }
