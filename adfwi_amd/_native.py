@@ -12,7 +12,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadfwi_amd.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 # floats of the library's per-GPU scratch array (ADFWI_NOALLOC_SCRATCH_FLOATS, 128 MiB)
 NOALLOC_SCRATCH_FLOATS = 32 << 20
 
@@ -67,7 +67,7 @@ class ElasticDesc(ctypes.Structure):
                 ("dx", _f32), ("dz", _f32), ("c0", _f32), ("c1", _f32), ("damping", _i32),
                 ("coef", _vp), ("pml_x", _vp), ("pml_z", _vp), ("src", _vp), ("src_m", _vp),
                 ("src_z", _vp), ("src_x", _vp), ("rcv_z", _vp), ("rcv_x", _vp), ("damp", _vp), ("src_fz", _vp),
-                ("nn", _i32), ("c2", _f32)]
+                ("nn", _i32), ("c2", _f32), ("rcv_map", _vp)]
 
 
 class ElasticFwdArgs(ctypes.Structure):

@@ -70,6 +70,36 @@ struct ElDev {
   const int *src_z, *src_x, *rcv_z, *rcv_x;
 };
 
+// Receivers handled inside the 4th-order PML step kernels: map [Z + Z*pitch] holds the receivers
+// of each row, then the receiver index of each point (-1: none; every receiver at its own point).
+struct ElRec {          // forward: records of txx, tzz, txz, vx, vz [S][nt][R]
+  const int *map;
+  float *r[5];
+};
+struct ElInj {          // adjoint: record adjoints [S][nt][R] (NULL: zero)
+  const int *map;
+  const float *g[5];
+};
+// Rows base .. base+127 that hold receivers, as a wavefront-uniform bit mask built once per
+// wavefront from the map's row counts: a row without receivers then costs one scalar test, and
+// a lane's receiver index is read only on rows that have some.
+struct RowMask {
+  unsigned long long lo, hi;
+  int base;
+  __device__ bool has(int row) const {
+    const int i = row - base;
+    if (i < 0 || i >= 128) return false;
+    return i < 64 ? ((lo >> i) & 1ull) != 0 : ((hi >> (i - 64)) & 1ull) != 0;
+  }
+};
+static_assert(ETZ_MAX + 8 <= 128, "a strip's rows (with the adjoint's halo) fit one receiver row mask");
+__device__ __forceinline__ RowMask receiver_rows(const int *map, int Z, int base, int lane) {
+  const int r0 = base + lane, r1 = base + 64 + lane;
+  const bool a = map && r0 >= 0 && r0 < Z && map[r0] > 0;
+  const bool b = map && r1 >= 0 && r1 < Z && map[r1] > 0;
+  return RowMask{(unsigned long long)__ballot(a), (unsigned long long)__ballot(b), base};
+}
+
 __device__ __forceinline__ bool strip_of(const ElDev &d, int &z0, int &x0) {
   const int n = (int)gridDim.x, b = (int)blockIdx.x;
   const int xcd = b & 7, per = n >> 3, rem = n & 7;
@@ -84,9 +114,9 @@ __device__ __forceinline__ bool strip_of(const ElDev &d, int &z0, int &x0) {
 // ---------------------------------------------------------------------------
 // forward step
 // ---------------------------------------------------------------------------
-template <bool ANISO>
+template <bool ANISO, bool REC>
 __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restrict__ in, float *__restrict__ out,
-                                                   int t) {
+                                                   int t, ElRec rc) {
   int z0, x0;
   if (!strip_of(d, z0, x0)) return;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
@@ -107,6 +137,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
   const unsigned zB = 4u * (unsigned)d.Z;
   const rsrc_t rPz = rsrc(d.pml_z, 3u * zB);
   auto zo = [&](int k) -> unsigned { return 4u * (unsigned)min(max(k, 0), zmax); };
+  const RowMask rrows = receiver_rows(REC ? rc.map : nullptr, d.Z, z0, lane);
   const float c0 = d.c0, c1 = d.c1;
   // per-column PML factors (x); rows come from pml_z
   const float pxn = d.pml_x[min(max(c, 0), d.X - 1)];
@@ -255,6 +286,17 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         bst(szz_x, rOut, vs, F(TZZ_X) + ok_); bst(szz_z, rOut, vs, F(TZZ_Z) + ok_);
         bst(sxz_x, rOut, vs, F(TXZ_X) + ok_); bst(sxz_z, rOut, vs, F(TXZ_Z) + ok_);
       }
+      if (REC && rrows.has(k)) {
+        // stress records of row k: the split sums of the state after the step (the free-surface
+        // tzz of row 2 is zero)
+        const int mk = own_k ? rc.map[d.Z + (long long)k * d.pitch + c] : -1;
+        if (mk >= 0) {
+          const long long ro = ((long long)s * d.nt + t) * d.nrec + mk;
+          rc.r[TXX][ro] = sxx_x + sxx_z;
+          rc.r[TZZ][ro] = (d.fsurf && k == 2) ? 0.0f : szz_x + szz_z;
+          rc.r[TXZ][ro] = sxz_x + sxz_z;
+        }
+      }
       xx0 = xx1; xx1 = xx2; xx2 = sxx_x + sxx_z;
       xz0 = xz1; xz1 = xz2; xz2 = xz3; xz3 = xz4; xz4 = sxz_x + sxz_z;
       zz0 = zz1; zz1 = zz2; zz2 = zz3; zz3 = zz4; zz4 = szz_x + szz_z;
@@ -289,6 +331,14 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
         }
         bst(nx_x, rOut, vs, F(VX_X) + orr); bst(nx_z, rOut, vs, F(VX_Z) + orr);
         bst(nz_x, rOut, vs, F(VZ_X) + orr); bst(nz_z, rOut, vs, F(VZ_Z) + orr);
+        if (REC && rrows.has(r)) {   // velocity records of row r
+          const int mr = rc.map[d.Z + (long long)r * d.pitch + c];
+          if (mr >= 0) {
+            const long long ro = ((long long)s * d.nt + t) * d.nrec + mr;
+            rc.r[VX][ro] = nx_x + nx_z;
+            rc.r[VZ][ro] = nz_x + nz_z;
+          }
+        }
       }
     };
     Row R0, R1, R2;
@@ -331,7 +381,8 @@ __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restri
 // adjoint step
 // ---------------------------------------------------------------------------
 // The transpose of one forward step, for the adjoint L of the split fields after step t
-// (record adjoints of step t already added by el_inject):
+// (record adjoints of step t added to L: by el_inject beforehand, or, with INJ, here as each
+// row of L is read, in the same order of operations):
 //   a. velocity update^T: e = L(v') * dt b / (d pd) feeds the stress-total adjoints
 //      Lt_xx = Dxf^T e_vxx, Lt_xz = Dzb^T e_vxz + Dxb^T e_vzx, Lt_zz = Dzf^T e_vzz;
 //      gradient of bx / bz: L(v') * dt / (d pd) * (D T) with T the forward stress totals;
@@ -356,16 +407,29 @@ __global__ void el_gather(ElDev d, const float *__restrict__ st, float *__restri
 // every field, 4/tz of its loads.
 constexpr int ETZA = 8;   // strip rows of the adjoints that keep a per-strip LDS gradient slab
 
-template <bool ANISO>
+template <bool ANISO, bool INJ>
 __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restrict__ lin, float *__restrict__ lout,
                                                    const float *__restrict__ fv, const float *__restrict__ fsn, int t,
-                                                   float *__restrict__ gacc, float *__restrict__ gsrc) {
+                                                   float *__restrict__ gacc, float *__restrict__ gsrc, ElInj inj) {
   constexpr int NGF = ANISO ? 8 : 6;
   // gradient terms of the rows finished in one iteration, per wavefront, double-buffered by
   // iteration parity (one barrier per iteration keeps a buffer from being rewritten early)
   __shared__ float rowbuf[2][EWPB][NGF][WAVE];
+  // PML factors 1 - dt/2 d, 1 + dt/2 d, 1/(1 + dt/2 d) of the rows the strip reads (z0-4 ..
+  // z0+tz+3), staged once: read from LDS they cost no wait on the vector loads in flight
+  constexpr int PZR = ETZ_MAX + 8;
+  __shared__ float pzs[3][PZR];
   int z0, x0;
   if (!strip_of(d, z0, x0)) return;   // uniform over the workgroup (one strip)
+  static_assert(3 * PZR <= ENT, "one PML factor per thread");
+  if (threadIdx.x < 3 * PZR) {
+    // (rows outside the grid are never read; a test rather than a clamp keeps the compiler's
+    // row limit, which every uniform row clamp uses, in a scalar register)
+    const int q = threadIdx.x / PZR, j = threadIdx.x - q * PZR, row = z0 - 4 + j;
+    pzs[q][j] = (unsigned)row < (unsigned)d.Z ? d.pml_z[q * d.Z + row] : 0.0f;
+  }
+  __syncthreads();
+  auto pzf = [&](int q, int row) -> float { return pzs[q][row - (z0 - 4)]; };
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int lane = threadIdx.x & (WAVE - 1);
   const int c = x0 - EHL + lane;
@@ -381,6 +445,8 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
   const rsrc_t rFv = rsrc(fv, (unsigned)NFIELD * fieldb);
   const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD * fieldb);
   const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
+  // rows z0-4 .. z0+tz+3 are read (tz <= 64)
+  const RowMask rrows = receiver_rows(INJ ? inj.map : nullptr, d.Z, z0 - 4, lane);
   const float c0 = d.c0, c1 = d.c1;
   const int cc = min(max(c, 0), d.X - 1);
   const float pxn = d.pml_x[cc], pxd = d.pml_x[d.pitch + cc], pxi = d.pml_x[2 * d.pitch + cc];
@@ -394,6 +460,9 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
   // this wavefront adds fields f = wv and wv + 4 (C fields of row k, b fields 4, 5 of row k-2)
   const long long planef = (long long)d.Z * d.pitch;
   float *gplane = gacc + (long long)blockIdx.y * NCOEF * planef + cc;
+  // the same planes for the old values' loads: every lane loads (rows clamped into the grid), so
+  // a load lands in its own register and nothing waits for it before the end of the iteration
+  const rsrc_t rG = rsrc(gacc + (long long)blockIdx.y * NCOEF * planef, (unsigned)NCOEF * planeb);
   float gf[NGF];
 
   // every wavefront runs the same number of iterations (the barrier is in the row loop); one
@@ -427,8 +496,23 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
     // free-surface rows kept for the transposes
     float ltzz1 = 0.f, ltxz0 = 0.f, ltxz1 = 0.f, lvz0 = 0.f, lvz1 = 0.f, lvx1 = 0.f;
 
+    // the record adjoint of one field at receiver index m in step t, added to both of its
+    // splits as the adjoint state's own values
+    auto add2 = [&](const float *g, int m, float &a, float &b) {
+      if (g) {
+        const float v = g[((long long)s * d.nt + t) * d.nrec + m];
+        a = a + v;
+        b = b + v;
+      }
+    };
+    // receiver index at this lane's point of row rr (-1: none or outside the window)
+    auto map_at = [&](int rr) -> int {
+      return (INJ && rrows.has(rr) && rr >= 2 && rr <= zw1 && cwin) ? inj.map[d.Z + (long long)rr * d.pitch + c]
+                                                                     : -1;
+    };
     // forward velocity totals and e of row rr (zero outside the grid / window)
-    auto load_row = [&](int rr, float &fvx, float &fvz, float &ea, float &eb, float &ec, float &ed) {
+    auto load_row = [&](int rr_, float &fvx, float &fvz, float &ea, float &eb, float &ec, float &ed) {
+      const int rr = __builtin_amdgcn_readfirstlane(rr_);   // uniform (a rolled loop's counter)
       const unsigned o = rb(rr);
       const bool in_grid = rr >= 0 && rr <= zmax;
       const float a = bld(rFv, vc, F(VX_X) + o) + bld(rFv, vc, F(VX_Z) + o);
@@ -438,13 +522,24 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       ea = eb = ec = ed = 0.f;
       if (rr >= 2 && rr <= zw1 && cwin) {
         const float bxv = bld(rC, vc, BX * planeb + o), bzv = bld(rC, vc, BZ * planeb + o);
-        const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
-        ea = bld(rLi, vc, F(VX_X) + o) * (vx_w * bxv);
-        eb = bld(rLi, vc, F(VX_Z) + o) * (vz_w * bxv);
-        ec = bld(rLi, vc, F(VZ_X) + o) * (vx_w * bzv);
-        ed = bld(rLi, vc, F(VZ_Z) + o) * (vz_w * bzv);
+        const float vz_w = d.dt / (d.dz * pzf(1, rr));
+        float l0 = bld(rLi, vc, F(VX_X) + o), l1 = bld(rLi, vc, F(VX_Z) + o);
+        float l2 = bld(rLi, vc, F(VZ_X) + o), l3 = bld(rLi, vc, F(VZ_Z) + o);
+        if constexpr (INJ) {
+          const int m = map_at(rr);
+          if (m >= 0) {
+            add2(inj.g[VX], m, l0, l1);
+            add2(inj.g[VZ], m, l2, l3);
+          }
+        }
+        ea = l0 * (vx_w * bxv);
+        eb = l1 * (vz_w * bxv);
+        ec = l2 * (vx_w * bzv);
+        ed = l3 * (vz_w * bzv);
       }
     };
+    // rolled: unrolled, the record-adjoint variant needs more than the 168 VGPRs of 3 waves per SIMD
+#pragma unroll 1
     for (int q = -2; q <= 1 && active; ++q) {
       float fvx, fvz, ea, eb, ec, ed;
       load_row(z0 - 2 + q, fvx, fvz, ea, eb, ec, ed);
@@ -493,14 +588,31 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       for (int j = 0; j < 2; ++j) {
         const int f = wv + j * EWPB;
         const int row = (f == 4 || f == 5) ? k - 2 : k;
-        if (f < NGF && out_col && row >= z0 && row < end_row)
-          gold[j] = gplane[(long long)f * planef + (long long)row * d.pitch];
+        gold[j] = bld(rG, 4u * (unsigned)cc, (unsigned)min(f, NGF - 1) * planeb + rb(row));   // unused if f >= NGF
       }
 #pragma unroll
       for (int f = 0; f < NGF; ++f) gf[f] = 0.0f;
       Row nxt;
       load(k + 1, nxt);      // next row's loads go out before this row's stores
       if (active) {
+      if constexpr (INJ) {
+        // record adjoints of step t: velocity fields of row k+2, stress fields of row k
+        if (rrows.has(k + 2)) {
+          const int m = map_at(k + 2);
+          if (m >= 0) {
+            add2(inj.g[VX], m, cur.lv[0], cur.lv[1]);
+            add2(inj.g[VZ], m, cur.lv[2], cur.lv[3]);
+          }
+        }
+        if (rrows.has(k)) {
+          const int m = map_at(k);
+          if (m >= 0) {
+            add2(inj.g[TXX], m, cur.ls[0], cur.ls[1]);
+            if (!(d.fsurf && k == 2)) add2(inj.g[TZZ], m, cur.ls[2], cur.ls[3]);   // free-surface tzz: none
+            add2(inj.g[TXZ], m, cur.ls[4], cur.ls[5]);
+          }
+        }
+      }
       {
         const int rr = k + 2;
         const bool in_grid = rr <= zmax;
@@ -515,7 +627,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
         const float fvz = in_grid ? cur.fv[2] + cur.fv[3] : 0.f;
         float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
         if (rr >= 2 && rr <= zw1 && cwin) {
-          const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
+          const float vz_w = d.dt / (d.dz * pzf(1, rr));
           ea = cur.lv[0] * (vx_w * cur.b[0]);
           eb = cur.lv[1] * (vz_w * cur.b[0]);
           ec = cur.lv[2] * (vx_w * cur.b[1]);
@@ -572,7 +684,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
           if (d.src_fz) gmt = gm;        // the force term of row zs follows two rows later
           else gsrc[(long long)s * d.nt + t] = gm;
         }
-        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float pzn = pzf(0, k), pzi = pzf(2, k);
         const float sz_keep = pzn * pzi, sz_a = d.dt_dz * pzi;
         if (own_k) {
           bst(sx_keep * lxx_x, rLo, vs, F(TXX_X) + o); bst(sz_keep * lxx_z, rLo, vs, F(TXX_Z) + o);
@@ -663,7 +775,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       const bool own_r = out_col && r >= z0 && r < end_row;
       if (own_r) {
         const unsigned o = rb(r);
-        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float pzn = pzf(0, r), pzd = pzf(1, r);
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
         const float l1 = lw[0][0], l2 = lw[0][1], l3 = lw[0][2], l4 = lw[0][3];
         if (d.src_fz && gsrc && r == zs && c == xs)   // vertical force: f w / 2 entered both vz splits
@@ -2278,6 +2390,13 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
   const dim3 grid(L.ntiles, L.ngroups), block(ENT);
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
+  // the 4th-order PML step takes its records itself when the receiver map is given
+  const bool fused = recs && ng > 0 && d->rcv_map && !d->damping && v.nn == 2;
+  ElRec rk{};
+  if (fused) {
+    rk.map = d->rcv_map;
+    for (int q = 0; q < 5; ++q) rk.r[q] = a->rec[q];
+  }
   const float *cur = a->state_in;
   for (int k = 0; k < a->nsteps; ++k) {
     const int t = a->t0 + k;
@@ -2308,13 +2427,17 @@ int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a
         hipLaunchKernelGGL(el3_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
       if ((rc = adfwi::check_launch("el3_fwd_step"))) return rc;
     } else {
-      if (d->aniso)
-        hipLaunchKernelGGL(el_fwd_step<true>, grid, block, 0, st, v, cur, nxt, t);
+      if (d->aniso && fused)
+        hipLaunchKernelGGL((el_fwd_step<true, true>), grid, block, 0, st, v, cur, nxt, t, rk);
+      else if (d->aniso)
+        hipLaunchKernelGGL((el_fwd_step<true, false>), grid, block, 0, st, v, cur, nxt, t, rk);
+      else if (fused)
+        hipLaunchKernelGGL((el_fwd_step<false, true>), grid, block, 0, st, v, cur, nxt, t, rk);
       else
-        hipLaunchKernelGGL(el_fwd_step<false>, grid, block, 0, st, v, cur, nxt, t);
+        hipLaunchKernelGGL((el_fwd_step<false, false>), grid, block, 0, st, v, cur, nxt, t, rk);
       if ((rc = adfwi::check_launch("el_fwd_step"))) return rc;
     }
-    if (recs && ng > 0) {
+    if (recs && ng > 0 && !fused) {
       if (d->damping)
         hipLaunchKernelGGL(el_gather5, ggrid, gblock, 0, st, v, nxt, a->rec[0], a->rec[1], a->rec[2], a->rec[3],
                            a->rec[4], t);
@@ -2378,10 +2501,17 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
   const int t_lo = a->t_hi - a->nsteps;
+  // the 4th-order PML adjoint step adds the record adjoints itself when the receiver map is given
+  const bool fused = recs && ng > 0 && d->rcv_map && !d->damping && v.nn == 2;
+  ElInj ij{};
+  if (fused) {
+    ij.map = d->rcv_map;
+    for (int q = 0; q < 5; ++q) ij.g[q] = a->grad_rec[q];
+  }
   float *cur = a->lam_in;
   for (int i = a->nsteps - 1; i >= 0; --i) {
     const int t = t_lo + i;
-    if (recs && ng > 0) {
+    if (recs && ng > 0 && !fused) {
       if (d->damping)
         hipLaunchKernelGGL(el_inject5, ggrid, gblock, 0, st, v, cur, a->grad_rec[0], a->grad_rec[1],
                            a->grad_rec[2], a->grad_rec[3], a->grad_rec[4], t);
@@ -2412,10 +2542,15 @@ int adfwi_elastic_adjoint(const adfwi_elastic_desc *d, adfwi_elastic_adj_args *a
         hipLaunchKernelGGL(el3_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
       if ((rc = adfwi::check_launch("el3_adj_step"))) return rc;
     } else {
-      if (d->aniso)
-        hipLaunchKernelGGL(el_adj_step<true>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+      float *gc = a->grad_coef, *gs = a->grad_src;
+      if (d->aniso && fused)
+        hipLaunchKernelGGL((el_adj_step<true, true>), grid, block, 0, st, v, cur, nxt, fv, fsn, t, gc, gs, ij);
+      else if (d->aniso)
+        hipLaunchKernelGGL((el_adj_step<true, false>), grid, block, 0, st, v, cur, nxt, fv, fsn, t, gc, gs, ij);
+      else if (fused)
+        hipLaunchKernelGGL((el_adj_step<false, true>), grid, block, 0, st, v, cur, nxt, fv, fsn, t, gc, gs, ij);
       else
-        hipLaunchKernelGGL(el_adj_step<false>, grid, block, 0, st, v, cur, nxt, fv, fsn, t, a->grad_coef, a->grad_src);
+        hipLaunchKernelGGL((el_adj_step<false, false>), grid, block, 0, st, v, cur, nxt, fv, fsn, t, gc, gs, ij);
       if ((rc = adfwi::check_launch("el_adj_step"))) return rc;
     }
     cur = nxt;

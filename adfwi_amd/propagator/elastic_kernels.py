@@ -94,6 +94,9 @@ class ElasticGeometry:
         i32 = lambda a: torch.tensor(a.astype(np.int32), device=self.device)
         self.src_z, self.src_x = i32(src_z), i32(src_x)
         self.rcv_z, self.rcv_x = i32(rcv_z), i32(rcv_x)
+        # receivers per row, then the receiver index of every point: with it the 4th-order PML
+        # step kernels take the records and add the record adjoints themselves
+        self.rcv_map = self._receiver_map(rcv_z, rcv_x)
         self.src_m = src_m.detach().to(device=self.device, dtype=torch.float32).contiguous()
         # vertical-force weight per shot (None: moment-tensor sources only, as in the reference)
         self.src_fz = (None if src_fz is None else
@@ -107,6 +110,20 @@ class ElasticGeometry:
         d = self.desc(None, None, None, None, (0.0,) * 7)
         nat.check(nat.load().adfwi_elastic_get_layout(ctypes.byref(d), ctypes.byref(self.layout)),
                   "adfwi_elastic_get_layout")
+
+    def _receiver_map(self, rcv_z: np.ndarray, rcv_x: np.ndarray) -> Optional[torch.Tensor]:
+        """[Z] receivers per row, then [Z][pitch] receiver index (-1 elsewhere); None without
+        receivers, when receivers repeat a point (their records are then taken and their adjoints
+        added by separate launches), or when ADFWI_EL_FUSED_RECORDS=0."""
+        if self.R == 0 or os.environ.get("ADFWI_EL_FUSED_RECORDS", "1") == "0":
+            return None
+        pts = rcv_z * self.pitch + rcv_x
+        if len(np.unique(pts)) != len(pts):
+            return None
+        m = np.full(self.Z + self.Z * self.pitch, -1, dtype=np.int32)
+        m[:self.Z] = np.bincount(rcv_z, minlength=self.Z)[:self.Z]
+        m[self.Z + pts] = np.arange(self.R, dtype=np.int32)
+        return torch.tensor(m, device=self.device)
 
     def desc(self, coef, pml_x, pml_z, src, scalars, damp=None) -> nat.ElasticDesc:
         """pml_x, pml_z: PML factor tables (PML scheme); damp: [Z][pitch] taper (damping scheme)."""
@@ -125,6 +142,7 @@ class ElasticGeometry:
         d.src_fz = None if self.src_fz is None else self.src_fz.data_ptr()
         d.src_z, d.src_x = self.src_z.data_ptr(), self.src_x.data_ptr()
         d.rcv_z, d.rcv_x = self.rcv_z.data_ptr(), self.rcv_x.data_ptr()
+        d.rcv_map = ptr(self.rcv_map)
         return d
 
     def state(self, n: int = 1) -> torch.Tensor:

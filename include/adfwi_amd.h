@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define ADFWI_ABI_VERSION 2
+#define ADFWI_ABI_VERSION 3
 
 #define ADFWI_OK 0
 #define ADFWI_E_VERSION -1   /* abi_version / struct_size mismatch */
@@ -211,6 +211,11 @@ typedef struct adfwi_elastic_desc {
                               (step_forward_PML_6order, elastic_kernels.py:426-584: window rows
                               [NN, nz-NN-1], free-surface rows NN-2 .. NN+1, padding NN rows on top) */
   float c2;                /* third coefficient of DiffCoef(NN, 's') (NN >= 3) */
+  const int32_t *rcv_map;  /* [nz + nz*pitch] or NULL: receivers per row, then the receiver index at
+                              each point (-1 elsewhere; every receiver at its own point).  With it the
+                              4th-order PML step kernels take the records and add the record
+                              adjoints themselves (elastic_kernels.py:407-411 samples inside the
+                              step too); without it, or with another scheme, separate launches do. */
 } adfwi_elastic_desc;
 
 /* Tiling and buffer sizes; layout.nfields = 10 (PML split fields) or 5 (damping), layout.reserved = shots per
@@ -237,8 +242,9 @@ typedef struct adfwi_elastic_fwd_args {
 int adfwi_elastic_forward(const adfwi_elastic_desc *d, adfwi_elastic_fwd_args *a, adfwi_stream_t stream);
 
 typedef struct adfwi_elastic_adj_args {
-  float *lam_in;           /* adjoint of the state after step t_hi-1: work[0] or work[1]; the record
-                              adjoints of each reversed step are added to it in place */
+  float *lam_in;           /* adjoint of the state after step t_hi-1: work[0] or work[1]; without
+                              rcv_map the record adjoints of each reversed step are added to it in
+                              place (with it, inside the step kernels: lam_in is then only read) */
   float *work[2];          /* [10][S][Z][pitch] each, distinct */
   int32_t t_hi;            /* steps t_hi-1 down to t_hi-nsteps are reversed */
   int32_t nsteps;

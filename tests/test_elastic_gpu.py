@@ -237,6 +237,58 @@ def test_gradients_multi_strip_match_oracle(fs, aniso, abc):
     assert rel_l2(w.grad.cpu().numpy(), ref["grad_wavelet"]) <= 1e-4
 
 
+@pytest.mark.parametrize("fs,aniso,repeat", [(True, False, False), (False, True, False), (True, True, False),
+                                             (True, False, True)])
+def test_fused_receivers_equal_separate_launches(monkeypatch, fs, aniso, repeat):
+    """Records taken, and record adjoints added, inside the 4th-order PML step kernels (through
+    the receiver map) against the separate gather and inject launches: a surface line on every
+    column (with a free surface the free-surface row, whose tzz reads zero) and a well line down
+    one column across several row strips of both kernels.  Records and gradients bit-identical,
+    records within 1e-5 of the oracle.  repeat: one receiver point twice, where the geometry
+    builds no map and the separate launches take over; their atomic adds of the two record
+    adjoints onto one point come in either order, so gradients then agree to rel-L2 1e-6."""
+    from adfwi_amd.propagator.elastic_kernels import ElasticGeometry
+    g, m = elastic_random_case(44, 150, 8, 40, 9, fs, seed=31 + int(fs) + 2 * int(aniso), aniso=aniso)
+    nz, nx = m["nz"], m["nx"]
+    rx = [np.arange(nx), np.full(nz - 2, 37)] + ([np.array([37])] if repeat else [])
+    rz = [np.zeros(nx), np.arange(1, nz - 1)] + ([np.array([5])] if repeat else [])
+    g["rcv_x"], g["rcv_z"] = np.concatenate(rx).astype(np.int64), np.concatenate(rz).astype(np.int64)
+    g["weights"] = np.random.default_rng(5).standard_normal((5, 9, m["nt"], len(g["rcv_x"]))).astype(np.float32)
+    monkeypatch.setenv("ADFWI_EL_TILE_Z", "8")        # forward strips of 8 rows
+    monkeypatch.setenv("ADFWI_ADJ_TILE_Z", "16")      # adjoint strips of 16 rows
+    built = []
+    own_init = ElasticGeometry.__init__
+
+    def spy(self, *a, **kw):
+        own_init(self, *a, **kw)
+        built.append(self.rcv_map is not None)
+    monkeypatch.setattr(ElasticGeometry, "__init__", spy)
+    out = {}
+    for fused in (True, False):
+        monkeypatch.setenv("ADFWI_EL_FUSED_RECORDS", "1" if fused else "0")
+        res, leaves, w = _dropin(g, m, want_grad=True)
+        _loss(res, g["weights"]).backward()
+        torch.cuda.synchronize()
+        out[fused] = ({k: res[k].detach().cpu().numpy() for k in ELASTIC_NAMES},
+                      {k: v.grad.cpu().numpy() for k, v in leaves.items()}, w.grad.cpu().numpy())
+    assert built == [not repeat, False], built
+    (rec_f, grad_f, w_f), (rec_s, grad_s, w_s) = out[True], out[False]
+    for k in ELASTIC_NAMES:
+        assert np.array_equal(rec_f[k], rec_s[k]), (k, rel_l2(rec_f[k], rec_s[k]))
+        assert np.abs(rec_f[k]).max() > 0, k
+    for k in grad_s:
+        if repeat:
+            assert rel_l2(grad_f[k], grad_s[k]) <= 1e-6, (k, rel_l2(grad_f[k], grad_s[k]))
+        else:
+            assert np.array_equal(grad_f[k], grad_s[k]), (k, rel_l2(grad_f[k], grad_s[k]))
+    assert np.array_equal(w_f, w_s) if not repeat else rel_l2(w_f, w_s) <= 1e-6
+    if fs:
+        assert not np.any(rec_f["tzz"][:, :, :nx]), "the free-surface row's tzz records are zero"
+    ref = _run_oracle(g, m)
+    for k in ELASTIC_NAMES:
+        assert rel_l2(rec_f[k], ref[k]) <= 1e-5, (k, rel_l2(rec_f[k], ref[k]))
+
+
 @pytest.mark.parametrize("abc", ["PML", "gerjan", "PML6", "gerjan6"])
 def test_adjoint_dot_product_in_the_wavelet(abc):
     """Records are linear in the wavelet: <J dw, dd> = <dw, J^T dd> with J^T dd the wavelet
