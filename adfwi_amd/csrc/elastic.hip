@@ -459,9 +459,9 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
   const int grp = blockIdx.y * EWPB + wv;
   // this wavefront adds fields f = wv and wv + 4 (C fields of row k, b fields 4, 5 of row k-2)
   const long long planef = (long long)d.Z * d.pitch;
-  float *gplane = gacc + (long long)blockIdx.y * NCOEF * planef + cc;
-  // the same planes for the old values' loads: every lane loads (rows clamped into the grid), so
-  // a load lands in its own register and nothing waits for it before the end of the iteration
+  // the same planes for the old values' loads, which every lane issues, so that a load lands in
+  // its own register and nothing waits for it before the end of the iteration (a load in a
+  // branch was copied at the branch's end, which waited for it)
   const rsrc_t rG = rsrc(gacc + (long long)blockIdx.y * NCOEF * planef, (unsigned)NCOEF * planeb);
   float gf[NGF];
 
@@ -588,7 +588,10 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       for (int j = 0; j < 2; ++j) {
         const int f = wv + j * EWPB;
         const int row = (f == 4 || f == 5) ? k - 2 : k;
-        gold[j] = bld(rG, 4u * (unsigned)cc, (unsigned)min(f, NGF - 1) * planeb + rb(row));   // unused if f >= NGF
+        // lanes that do not own the point take an offset past the planes: such a buffer load
+        // returns zero without touching memory, so the load needs no branch
+        const bool own = f < NGF && out_col && row >= z0 && row < end_row;   // (as at the store)
+        gold[j] = bld(rG, own ? 4u * (unsigned)cc : 0x80000000u, (unsigned)min(f, NGF - 1) * planeb + rb(row));
       }
 #pragma unroll
       for (int f = 0; f < NGF; ++f) gf[f] = 0.0f;
@@ -800,11 +803,14 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
       for (int j = 0; j < 2; ++j) {
         const int f = wv + j * EWPB;
         const int row = (f == 4 || f == 5) ? k - 2 : k;
-        if (f < NGF && out_col && row >= z0 && row < end_row) {
-          const float v = (rowbuf[par][0][f][lane] + rowbuf[par][1][f][lane]) +
-                          (rowbuf[par][2][f][lane] + rowbuf[par][3][f][lane]);
-          gplane[(long long)f * planef + (long long)row * d.pitch] = gold[j] + v;
-        }
+        float v = 0.0f;
+        if (f < NGF)
+          v = (rowbuf[par][0][f][lane] + rowbuf[par][1][f][lane]) + (rowbuf[par][2][f][lane] + rowbuf[par][3][f][lane]);
+        // stored by every lane, past the planes where the lane does not own the point (such a
+        // buffer store is dropped): the old value is then consumed on every path, and no load
+        // into its register is left in flight when the next iteration reuses it
+        const bool own = f < NGF && out_col && row >= z0 && row < end_row;
+        bst(gold[j] + v, rG, own ? 4u * (unsigned)cc : 0x80000000u, (unsigned)min(f, NGF - 1) * planeb + rb(row));
       }
     }
   }
