@@ -337,6 +337,8 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const rsrc_t rE = rsrc(energy, energy ? 2u * planeb * gridDim.y : 0u);
   const unsigned eb = 2u * planeb * blockIdx.y;
   const bool recs = rec_p != nullptr && d.nrec > 0;
+  // rows z0 .. z0+127 that hold receivers (a strip's records are of rows z0 .. end_row-1)
+  const adfwi::RowMask rrows = adfwi::receiver_rows(recs ? io.rmap : nullptr, d.Z, z0, lane);
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -435,12 +437,12 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const float dm = rowp ? 1.0f : 0.0f, dm2 = rowp2 ? 1.0f : 0.0f;   // stencil-image masks
     const bool own_q = out_col && q >= z0 && q < end_row;
     const bool own_qq = out_col && qq >= z0 && qq < end_row;
-    // receiver slots of A's emitted row (row counts first: uniform test, then the lane's index).
-    // The count read waits for the row loads in flight, but the other waves of the SIMD cover
-    // that wait: reading every row's map entries one row ahead instead measured 7 % slower.
-    int ridx = -1;
-    if (recs && emit && r >= z0 && r < end_row && io.rmap[r] > 0 && own_r)
-      ridx = io.rmap[d.Z + r * d.pitch + c];
+    // receiver slots of A's emitted row r (step t) and of B's final row qq (step t+1): the
+    // lane's map entry is read only on rows with receivers (reading every row's entries one row
+    // ahead with the other loads measured 7 % slower)
+    int ridx = -1, rfin = -1;
+    if (recs && emit && rrows.has(r) && own_r) ridx = io.rmap[d.Z + r * d.pitch + c];
+    if (recs && bstep && rrows.has(qq) && own_qq) rfin = io.rmap[d.Z + qq * d.pitch + c];
     float epa = 0.0f, eua = 0.0f, epb = 0.0f, eub = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
@@ -507,6 +509,12 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
             if (u_ok) bst(un2, oPU, vs1, so[j] + rqq);
             if (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) bst(wn2, oW, vs, so[j] + rqq);
             if (d.fsurf && qq == fs) bst(wn2, oW, vs, so[j] + rqq - rowb);
+          }
+          if (rfin >= 0) {   // records of step t+1: the values just stored (points never written read zero)
+            const long long ro = ((long long)(s0 + j) * d.nt + t + 1) * d.nrec + rfin;
+            rec_p[ro] = ps;
+            io.rec_u[ro] = u_ok ? un2 : 0.0f;
+            io.rec_w[ro] = (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) ? wn2 : 0.0f;
           }
           epb += ps * ps;                   // stored for owned points only
           eub += un2 * un2;
@@ -733,7 +741,8 @@ struct AdjPairIO {
   const float *img_p0, *img_p1;          // pressure images (A2 gradient), p1 = p0 + one field
   float *grad_a1, *grad_a2, *grad_src;
   const int *rmap;
-  const float *g_p, *g_u, *g_w;          // record adjoints [S][nt][R]; step t's are added here
+  const float *g_p, *g_u, *g_w;          // record adjoints [S][nt][R]; step t's are added here, and
+                                         // step t+1's (adj_pair_adds_first)
 };
 
 template <int SPB, bool A2, bool FULL>
@@ -775,10 +784,20 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   const rsrc_t rG2 = rsrc(io.grad_a2, A2 ? planeb * gridDim.y : 0u);
   const unsigned gb = planeb * blockIdx.y;
   const bool recs = io.rmap != nullptr && d.nrec > 0;
-  // receiver index of each point of a row (-1: none), read with the next row's loads: a separate
-  // read of the row's receiver count would make the wave wait for every load in flight
-  const rsrc_t rM = rsrc(io.rmap, recs ? 4u * ((unsigned)d.Z + (unsigned)d.Z * (unsigned)d.pitch) : 0u);
-  const unsigned mapb = 4u * (unsigned)d.Z;
+  // rows z0-6 .. z0+121 that hold receivers: the strip reads rows z0-6 .. end_row+5
+  const adfwi::RowMask rrows = adfwi::receiver_rows(recs ? io.rmap : nullptr, d.Z, z0 - 6, lane);
+  // receiver index at this lane's point of a row (-1: none); read only on rows with receivers
+  auto slot_at = [&](int row) -> int {
+    return (recs && rrows.has(row) && row >= 0 && row < d.Z && in_col) ? io.rmap[d.Z + row * d.pitch + c] : -1;
+  };
+  // the record adjoints of step t+1 at receiver slot m, added to the incoming adjoint state as
+  // its rows are taken over from the loads (the state itself is left as it was); the variant
+  // with both the A2 gradient and every record adjoint leaves them to a separate launch, as the
+  // additions would cost it a wave per SIMD (adj_pair_adds_first)
+  auto add_tb = [&](const float *g, int j, int m, float v) -> float {
+    if constexpr (A2 && FULL) return v;
+    return (g && m >= 0) ? v + g[((long long)(s0 + min(j, ns - 1)) * d.nt + t + 1) * d.nrec + m] : v;
+  };
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -789,6 +808,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     return raw;
   };
   const int kst = z0 - 4;
+  static_assert(TZ_MAX + 12 <= 128, "a strip's rows fit one receiver row mask");
 
   // step B windows (rows of k): lw* lambda_w' rows k-2..k+1, l2* lambda_p2 rows k-2..k,
   // g* A1-masked lambda_p2 rows k-3..k, lu* lambda_u rows k-2..k, pp* pressure images k-3..k;
@@ -814,12 +834,12 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const int s = s0 + min(j, ns - 1);
     zs[j] = j < ns ? d.src_z[s] : -1000;
     xs[j] = d.src_x[s];
-    lw1[j] = lwfix(so[j], kst - 2, bld(rW, vc, so[j] + rb(kst - 2)));
-    lw2[j] = lwfix(so[j], kst - 1, bld(rW, vc, so[j] + rb(kst - 1)));
-    lw3[j] = lwfix(so[j], kst, bld(rW, vc, so[j] + rb(kst)));
-    cW[j] = bld(rW, vc, so[j] + rb(kst + 1));
-    cU[j] = bld(rPU, vc1, so[j] + rb(kst));
-    cP[j] = bld(rPU, vc, so[j] + rb(kst));
+    lw1[j] = lwfix(so[j], kst - 2, add_tb(g_w, j, slot_at(kst - 2), bld(rW, vc, so[j] + rb(kst - 2))));
+    lw2[j] = lwfix(so[j], kst - 1, add_tb(g_w, j, slot_at(kst - 1), bld(rW, vc, so[j] + rb(kst - 1))));
+    lw3[j] = lwfix(so[j], kst, add_tb(g_w, j, slot_at(kst), bld(rW, vc, so[j] + rb(kst))));
+    cW[j] = add_tb(g_w, j, slot_at(kst + 1), bld(rW, vc, so[j] + rb(kst + 1)));
+    cU[j] = add_tb(g_u, j, slot_at(kst), bld(rPU, vc1, so[j] + rb(kst)));
+    cP[j] = add_tb(io.g_p, j, slot_at(kst), bld(rPU, vc, so[j] + rb(kst)));
     cI[j] = want_a1 ? bld(rI, vc1, so[j] + rb(kst)) : 0.0f;
     cIp[j] = A2 ? bld(rJ, vc1, so[j] + rb(kst)) : 0.0f;
     dI[j] = want_a1 ? bld(rI, vc, so[j] + rb(kst - 3)) : 0.0f;
@@ -849,7 +869,6 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   // old gradient-plane values of rows q (A1) and qq (A2), loaded one iteration ahead
   float cG1 = want_a1 ? bld(rG1, vc, gb + rb(qst)) : 0.0f;
   float cG2 = A2 ? bld(rG2, vc, gb + rb(kst - 5)) : 0.0f;
-  int cR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(kst - 2))) : -1;
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -879,7 +898,6 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
     const float nG1 = want_a1 ? bld(rG1, vc, gb + rb(q + 1)) : 0.0f;
     const float nG2 = A2 ? bld(rG2, vc, gb + rb(qq + 1)) : 0.0f;
-    const int nR = recs ? __builtin_bit_cast(int, bld(rM, vc, mapb + rb(r + 1))) : -1;
 
     // step B masks (rows k, r) and step A masks (rows q, qq)
     const bool rowp = cp && k >= fs + 1 && k <= d.Z - 3;
@@ -905,9 +923,9 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     // A1 image-term masks as 0/1 factors on one operand: (x*1)*y rounds as x*y, and a masked
     // term is +-0, which leaves the sum unchanged; the sums are stored for owned points only
     const float m1k = rowp ? 1.0f : 0.0f, m1q = rowp_q ? 1.0f : 0.0f;
-    // receiver slot of B's emitted row r (step t's sources), on every lane of the strip
-    int ridx = -1;
-    if (recs && r >= 0 && r < d.Z && in_col) ridx = cR;
+    // receiver slots: B's emitted row r (step t's sources, on every lane of the strip), and the
+    // rows of the incoming state B reads now, p and u of row k and w of row k+1 (step t+1's)
+    const int ridx = slot_at(r), mpn = slot_at(k + 1), mwn = slot_at(k + 2);
     float ga1b = 0.0f, ga2b = 0.0f, ga1a = 0.0f, ga2a = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
@@ -992,7 +1010,8 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         }
         eP[j] = op; eU[j] = ou; eW[j] = ow;
       }
-      cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j]; cI[j] = nI[j]; cIp[j] = nIp[j];
+      cW[j] = add_tb(g_w, j, mwn, nW[j]); cU[j] = add_tb(g_u, j, mpn, nU[j]); cP[j] = add_tb(io.g_p, j, mpn, nP[j]);
+      cI[j] = nI[j]; cIp[j] = nIp[j];
       dI[j] = mI[j]; dIp[j] = mIp[j];
     }
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
@@ -1001,7 +1020,6 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
     if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
     cG1 = nG1; cG2 = nG2;
-    cR = nR;
     ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
     ga2d0 = ga2d1; ga2d1 = ga2d2; ga2d2 = ga2b;
   }
@@ -1192,6 +1210,14 @@ void launch_adj_pair_a(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const Adj
   }
 }
 
+// whether the adjoint pair variant chosen for io adds the record adjoints of its first reversed
+// step itself (all but the A2 + FULL variant)
+bool adj_pair_adds_first(const AdjPairIO &io) {
+  const bool full = io.grad_src || io.g_u || io.g_w;
+  const bool a2 = io.grad_a2 && io.img_p0 && io.img_p1;
+  return !(a2 && full);
+}
+
 void launch_adj_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
   const bool full = io.grad_src || io.g_u || io.g_w;
   const bool a2 = io.grad_a2 && io.img_p0 && io.img_p1;
@@ -1253,7 +1279,8 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
   // two steps per launch where the first step's state is not needed in HBM (it is not a
-  // checkpoint) and its receiver samples can be written through the receiver-index map
+  // checkpoint) and the receiver samples of both steps can be written through the receiver-index
+  // map
   const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
   const AcDev w = pair_dev(v);
   const dim3 pgrid((w.nstrips + WPB - 1) / WPB, L.ngroups);
@@ -1286,7 +1313,7 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
       launch_fwd(v, grid, block, st, cur, nxt, t, a->energy, acc_u, imd, imp);
       if ((rc = adfwi::check_launch("ac_fwd_step"))) return rc;
     }
-    if (recs && ng > 0) {
+    if (recs && ng > 0 && !pair) {   // (a pair launch takes the records of both its steps)
       hipLaunchKernelGGL(ac_gather, ggrid, gblock, 0, st, v, nxt, a->rec_p, a->rec_u, a->rec_w, tl);
       if ((rc = adfwi::check_launch("ac_gather"))) return rc;
     }
@@ -1320,8 +1347,8 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   const dim3 grid = step_grid(L), block(NT);
   const long long ng = (long long)d->nshots * d->nrec;
   const dim3 ggrid((unsigned)((ng + 255) / 256)), gblock(256);
-  // two steps per launch: the later one (t) reversed first, the receiver sources of t-1 added
-  // inside the kernel through the receiver-index map
+  // two steps per launch: the later one (t) reversed first, the receiver sources of t and t-1
+  // added inside the kernel through the receiver-index map
   const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
   AcDev w = pair_dev(v);
   w.spb = L.reserved;
@@ -1337,13 +1364,9 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   for (int k = 0; k < a->nsteps;) {
     const int t = a->t_hi - k;
     const bool pair = fuse && k + 1 < a->nsteps;
-    if (recs && ng > 0) {
-      hipLaunchKernelGGL(ac_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec_p, a->grad_rec_u, a->grad_rec_w, t);
-      if ((rc = adfwi::check_launch("ac_inject"))) return rc;
-    }
     float *nxt = (cur == a->work[0]) ? a->work[1] : a->work[0];
+    AdjPairIO io{};
     if (pair) {
-      AdjPairIO io;
       io.in = cur; io.out = nxt; io.t = t - 1;
       io.img_d0 = a->image ? a->image + slot(t - 1) : nullptr;
       io.img_d1 = a->image ? a->image + slot(t) : nullptr;
@@ -1352,6 +1375,14 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
       io.grad_a1 = a->grad_a1; io.grad_a2 = a->grad_a2; io.grad_src = a->grad_src;
       io.rmap = recs ? a->rcv_map : nullptr;
       io.g_p = a->grad_rec_p; io.g_u = a->grad_rec_u; io.g_w = a->grad_rec_w;
+    }
+    // the record adjoints of step t: added in place before the launch, unless a pair kernel
+    // adds them as it reads the state
+    if (recs && ng > 0 && !(pair && adj_pair_adds_first(io))) {
+      hipLaunchKernelGGL(ac_inject, ggrid, gblock, 0, st, v, cur, a->grad_rec_p, a->grad_rec_u, a->grad_rec_w, t);
+      if ((rc = adfwi::check_launch("ac_inject"))) return rc;
+    }
+    if (pair) {
       launch_adj_pair(w, pgrid, block, st, io);
       if ((rc = adfwi::check_launch("ac_adj_pair"))) return rc;
     } else {

@@ -1,4 +1,5 @@
-// Shared host/device helpers of libadfwi_amd.so: error reporting and argument checks.
+// Shared host/device helpers of libadfwi_amd.so: error reporting, argument checks, and the
+// receiver row mask of the step kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,6 +46,25 @@ inline int check_launch(const char *what) {
     return ADFWI_E_LAUNCH;
   }
   return ADFWI_OK;
+}
+
+// Rows base .. base+127 that hold receivers, as a wavefront-uniform bit mask built once per
+// wavefront from the map's row counts: a row without receivers then costs one scalar test, and
+// a lane's receiver index is read only on rows that have some.
+struct RowMask {
+  unsigned long long lo, hi;
+  int base;
+  __device__ bool has(int row) const {
+    const int i = row - base;
+    if (i < 0 || i >= 128) return false;
+    return i < 64 ? ((lo >> i) & 1ull) != 0 : ((hi >> (i - 64)) & 1ull) != 0;
+  }
+};
+__device__ __forceinline__ RowMask receiver_rows(const int *map, int Z, int base, int lane) {
+  const int r0 = base + lane, r1 = base + 64 + lane;
+  const bool a = map && r0 >= 0 && r0 < Z && map[r0] > 0;
+  const bool b = map && r1 >= 0 && r1 < Z && map[r1] > 0;
+  return RowMask{(unsigned long long)__ballot(a), (unsigned long long)__ballot(b), base};
 }
 
 }  // namespace adfwi

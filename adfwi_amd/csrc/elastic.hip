@@ -80,25 +80,7 @@ struct ElInj {          // adjoint: record adjoints [S][nt][R] (NULL: zero)
   const int *map;
   const float *g[5];
 };
-// Rows base .. base+127 that hold receivers, as a wavefront-uniform bit mask built once per
-// wavefront from the map's row counts: a row without receivers then costs one scalar test, and
-// a lane's receiver index is read only on rows that have some.
-struct RowMask {
-  unsigned long long lo, hi;
-  int base;
-  __device__ bool has(int row) const {
-    const int i = row - base;
-    if (i < 0 || i >= 128) return false;
-    return i < 64 ? ((lo >> i) & 1ull) != 0 : ((hi >> (i - 64)) & 1ull) != 0;
-  }
-};
 static_assert(ETZ_MAX + 8 <= 128, "a strip's rows (with the adjoint's halo) fit one receiver row mask");
-__device__ __forceinline__ RowMask receiver_rows(const int *map, int Z, int base, int lane) {
-  const int r0 = base + lane, r1 = base + 64 + lane;
-  const bool a = map && r0 >= 0 && r0 < Z && map[r0] > 0;
-  const bool b = map && r1 >= 0 && r1 < Z && map[r1] > 0;
-  return RowMask{(unsigned long long)__ballot(a), (unsigned long long)__ballot(b), base};
-}
 
 __device__ __forceinline__ bool strip_of(const ElDev &d, int &z0, int &x0) {
   const int n = (int)gridDim.x, b = (int)blockIdx.x;
@@ -137,7 +119,7 @@ __global__ __launch_bounds__(ENT) void el_fwd_step(ElDev d, const float *__restr
   const unsigned zB = 4u * (unsigned)d.Z;
   const rsrc_t rPz = rsrc(d.pml_z, 3u * zB);
   auto zo = [&](int k) -> unsigned { return 4u * (unsigned)min(max(k, 0), zmax); };
-  const RowMask rrows = receiver_rows(REC ? rc.map : nullptr, d.Z, z0, lane);
+  const adfwi::RowMask rrows = adfwi::receiver_rows(REC ? rc.map : nullptr, d.Z, z0, lane);
   const float c0 = d.c0, c1 = d.c1;
   // per-column PML factors (x); rows come from pml_z
   const float pxn = d.pml_x[min(max(c, 0), d.X - 1)];
@@ -446,7 +428,7 @@ __global__ __launch_bounds__(ENT) void el_adj_step(ElDev d, const float *__restr
   const rsrc_t rFs = rsrc(fsn, (unsigned)NFIELD * fieldb);
   const rsrc_t rC = rsrc(d.coef, (unsigned)NCOEF * planeb);
   // rows z0-4 .. z0+tz+3 are read (tz <= 64)
-  const RowMask rrows = receiver_rows(INJ ? inj.map : nullptr, d.Z, z0 - 4, lane);
+  const adfwi::RowMask rrows = adfwi::receiver_rows(INJ ? inj.map : nullptr, d.Z, z0 - 4, lane);
   const float c0 = d.c0, c1 = d.c1;
   const int cc = min(max(c, 0), d.X - 1);
   const float pxn = d.pml_x[cc], pxd = d.pml_x[d.pitch + cc], pxi = d.pml_x[2 * d.pitch + cc];
