@@ -130,9 +130,11 @@ class ServingLoop:
             end = time.perf_counter() - t0
             return [Outcome(r.rid, r.arrival, start, end, {}, None, repr(exc)) for r in reqs]
 
-    def run(self, requests: List[Request], realtime: bool = True) -> StreamReport:
+    def run(self, requests: List[Request], realtime: bool = True,
+            progress: Optional[Callable[[int, int], None]] = None) -> StreamReport:
         """Serves the requests.  realtime=False submits every request at t = 0 (arrival times
-        are then taken as 0 for the latency)."""
+        are then taken as 0 for the latency).  progress(done, total) is called after every
+        completed pack."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         if not realtime:
             for r in pending:
@@ -169,6 +171,8 @@ class ServingLoop:
                         for r in futures.pop(f):
                             self.scheduler.finish(r.rid)
                         outcomes.extend(f.result())
+                        if progress is not None:
+                            progress(len(outcomes), len(pending))
                 elif timeout is not None:
                     time.sleep(timeout)
                 elif self.scheduler.waiting:
@@ -179,8 +183,10 @@ class ServingLoop:
                             packs)
 
 
-def run_alone(requests: List[Request], device="cuda:0") -> List[Outcome]:
-    """The plain path: each request by itself, one after the other, on the default stream."""
+def run_alone(requests: List[Request], device="cuda:0",
+              progress: Optional[Callable[[int, int], None]] = None) -> List[Outcome]:
+    """The plain path: each request by itself, one after the other, on the default stream;
+    progress(done, total) after every request."""
     import torch
     out = []
     t0 = time.perf_counter()
@@ -189,4 +195,6 @@ def run_alone(requests: List[Request], device="cuda:0") -> List[Outcome]:
         res, digest = execute(r, device)
         torch.cuda.synchronize()
         out.append(Outcome(r.rid, r.arrival, s, time.perf_counter() - t0, digest, res if r.keep else None))
+        if progress is not None:
+            progress(len(out), len(requests))
     return out
