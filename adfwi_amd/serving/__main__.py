@@ -4,7 +4,7 @@ it, the plain path (one request at a time, first come first served); prints one 
     python -m adfwi_amd.serving --requests 60 --load 0.5,1,2 --slots 4 [--scale-nt 0.1 --max-shots 8]
                                 [--model-pool 6 | --shape-pool 4] [--graph-steps 64] [--no-pack]
                                 [--pack-across-models] [--exclusive-s 1.0] [--burst]
-                                [--slots 4,2 --one-long-s 2.0]
+                                [--slots 4,2 --one-long-s 2.0 --long-slots-s 2.0]
 
 The plain path is run once without arrivals to measure every request's service time; its
 capacity (requests/s) sets the arrival rates (--load x capacity).  Its latency at each rate is
@@ -19,8 +19,9 @@ window of acoustic modelling (0: direct launches).  --exclusive-s S gives every 
 estimated to take longer than S seconds alone the device to itself.  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).  Several
 serving policies run on the same stream after one plain pass: every --slots value (a comma
-list), and with --one-long-s S also the loop that runs requests estimated above S seconds one
-at a time beside shorter ones (4 slots, each such request charged 0.51 of the budget).
+list), with --one-long-s S also the loop that runs requests estimated above S seconds one at
+a time beside shorter ones (4 slots, each such request charged 0.51 of the budget), and with
+--long-slots-s S the loop whose requests estimated above S seconds take two of its four slots.
 """
 import argparse
 import copy
@@ -140,6 +141,8 @@ def main():
                     help="fraction of the budget charged to those requests (0.51: one at a time)")
     ap.add_argument("--one-long-s", type=float, default=None,
                     help="also run the policy that serves requests estimated above this many seconds one at a time")
+    ap.add_argument("--long-slots-s", type=float, default=None,
+                    help="also run the policy whose requests estimated above this many seconds take two of four slots")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     policies = [(f"slots{n}", dict(max_slots=n, exclusive_s=args.exclusive_s, long_charge=args.long_charge))
@@ -147,6 +150,8 @@ def main():
     if args.one_long_s is not None:
         policies.append((f"one_long_above_{args.one_long_s:g}s",
                          dict(max_slots=4, exclusive_s=args.one_long_s, long_charge=0.51)))
+    if args.long_slots_s is not None:
+        policies.append((f"two_slots_above_{args.long_slots_s:g}s", dict(max_slots=4, long_slots_s=args.long_slots_s)))
     nt_range = (max(50, int(1000 * args.scale_nt)), max(60, int(6000 * args.scale_nt)))
     out = measure_stream(args.requests, [float(x) for x in args.load.split(",")], seed=args.seed, policies=policies,
                          nz_range=(128, args.max_nz), nx_range=(256, args.max_nx), nt_range=nt_range,

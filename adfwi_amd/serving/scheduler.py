@@ -1,10 +1,12 @@
 """Admission policy of the serving loop (host logic only, no device access).
 
-Requests wait in arrival order.  A request is admitted when a slot is free and its HBM
-footprint fits into what the running requests leave of the budget.  A request that does not
-fit may be passed by later, smaller ones, but only `max_bypass` times: after that nothing is
-admitted past it until it fits, so large requests cannot starve.  A request whose footprint
-exceeds the whole budget is rejected at submission.
+Requests wait in arrival order.  A request is admitted when its slots are free and its HBM
+footprint fits into what the running requests leave of the budget.  A request takes one slot,
+or `weight` slots when submitted so (the serving loop gives long requests two of its four, so
+that at most two of them share the device while short requests still run four at a time).  A
+request that does not fit may be passed by later, smaller ones, but only `max_bypass` times:
+after that nothing is admitted past it until it fits, so large requests cannot starve.  A
+request whose footprint exceeds the whole budget is rejected at submission.
 
 Packing: with a `pack_key`, a request being admitted takes later waiting requests with the
 same key along (up to `max_pack_shots` shots in all, each within the free budget): the pack
@@ -21,6 +23,7 @@ class _Waiting:
     req: object
     nbytes: int
     bypassed: int = 0
+    weight: int = 1
 
 
 class Scheduler:
@@ -36,6 +39,7 @@ class Scheduler:
         self.waiting: Deque[_Waiting] = deque()
         self.running: "OrderedDict[int, int]" = OrderedDict()
         self._pack_of: Dict[int, int] = {}      # running request -> id of its pack (its first member)
+        self._weight_of: Dict[int, int] = {}    # running pack -> slots it takes
         self.rejected: List[Tuple[object, str]] = []
         self.peak_bytes = 0
 
@@ -45,14 +49,16 @@ class Scheduler:
 
     @property
     def slots_used(self) -> int:
-        return len(set(self._pack_of.values()))
+        return sum(self._weight_of[p] for p in set(self._pack_of.values()))
 
-    def submit(self, req, nbytes: int) -> bool:
+    def submit(self, req, nbytes: int, weight: int = 1) -> bool:
         if nbytes > self.budget:
             self.rejected.append((req, f"footprint {nbytes / 2**30:.1f} GiB exceeds the budget "
                                        f"{self.budget / 2**30:.1f} GiB"))
             return False
-        self.waiting.append(_Waiting(req, int(nbytes)))
+        if not 1 <= weight <= self.max_slots:
+            raise ValueError(f"slot weight {weight} outside 1..{self.max_slots}")
+        self.waiting.append(_Waiting(req, int(nbytes), weight=int(weight)))
         return True
 
     def admit_packs(self) -> List[List[object]]:
@@ -68,7 +74,7 @@ class Scheduler:
         for i, w in enumerate(queue):
             if id(w) in taken:
                 continue
-            if not blocked and slots > 0 and w.nbytes <= free:
+            if not blocked and slots >= w.weight and w.nbytes <= free:
                 pack = [w]
                 free -= w.nbytes
                 taken.add(id(w))
@@ -93,7 +99,8 @@ class Scheduler:
                             between.append(e)
                     for p in passed:            # taking a member past a request passes it once
                         p.bypassed += 1
-                slots -= 1
+                slots -= w.weight
+                self._weight_of[w.req.rid] = w.weight
                 for m in pack:
                     self.running[m.req.rid] = m.nbytes
                     self._pack_of[m.req.rid] = w.req.rid
@@ -104,7 +111,7 @@ class Scheduler:
                         blocked = True  # nothing passes it any more: it is next as soon as it fits
             else:
                 kept.append(w)
-                if w.nbytes > free and w.bypassed >= self.max_bypass:
+                if (w.nbytes > free or w.weight > slots) and w.bypassed >= self.max_bypass:
                     blocked = True      # nothing passes it any more: it is next as soon as it fits
         self.waiting = kept
         self.peak_bytes = max(self.peak_bytes, self.used)
@@ -116,7 +123,9 @@ class Scheduler:
 
     def finish(self, rid: int) -> None:
         self.running.pop(rid, None)
-        self._pack_of.pop(rid, None)
+        pack = self._pack_of.pop(rid, None)
+        if pack is not None and pack not in self._pack_of.values():
+            self._weight_of.pop(pack, None)
 
     def idle(self) -> bool:
         return not self.waiting and not self.running

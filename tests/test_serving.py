@@ -245,3 +245,32 @@ def test_pack_members_taken_past_a_waiting_request_count_as_bypasses():
     s.finish(0)
     assert [[r.rid for r in p] for p in s.admit_packs()] == [[1]]   # 4 stays behind 2
     assert [(w.req.rid, w.bypassed) for w in s.waiting] == [(2, 1), (4, 0)]
+
+
+def test_weighted_slots_limit_long_requests_and_do_not_starve_them():
+    """A request submitted with weight 2 takes two slots: with four slots at most two such
+    requests run together, short ones fill the rest; a weight-2 request passed max_bypass
+    times by short ones blocks the queue until two slots are free."""
+    s = Scheduler(budget_bytes=100, max_slots=4, max_bypass=2)
+    for rid in (0, 1, 2):
+        s.submit(_req(rid), 10, weight=2)
+    s.submit(_req(3), 10)
+    assert [r.rid for r in s.admit()] == [0, 1]          # two long requests fill the four slots
+    assert s.slots_used == 4
+    s.finish(0)
+    assert [r.rid for r in s.admit()] == [2]             # the next long one takes the two slots freed
+    s.finish(1)
+    assert [r.rid for r in s.admit()] == [3]             # a short one takes one of the two freed
+    assert s.slots_used == 3
+    s.submit(_req(4), 10, weight=2)                      # one slot free: it must wait
+    for rid in (5, 6, 7):
+        s.submit(_req(rid), 10)
+    assert [r.rid for r in s.admit()] == [5]             # 5 passes 4 (once) and takes the last slot
+    s.finish(5)
+    assert [r.rid for r in s.admit()] == [6]             # passed twice: 4 now blocks the queue
+    s.finish(6)
+    assert [r.rid for r in s.admit()] == []              # 7 may not pass it; one slot is not enough
+    s.finish(3)
+    assert [r.rid for r in s.admit()] == [4]             # two slots free
+    with pytest.raises(ValueError):
+        s.submit(_req(8), 10, weight=5)
