@@ -19,7 +19,7 @@ window of acoustic modelling (0: direct launches).  --exclusive-s S gives every 
 estimated to take longer than S seconds alone the device to itself.  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).  Several
 serving policies run on the same stream after one plain pass: every --slots value (a comma
-list), with --one-long-s S also the loop that runs requests estimated above S seconds one at
+list; one slot per request), with --one-long-s S also the loop that runs requests estimated above S seconds one at
 a time beside shorter ones (4 slots, each such request charged 0.51 of the budget), and with
 --long-slots-s S the loop whose requests estimated above S seconds take two of its four slots.
 """
@@ -90,8 +90,8 @@ def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_
            "plain_service_p50_s": float(np.percentile(service, 50)),
            "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
     digests = {o.rid: o.digest for o in alone}
-    if policies is None:
-        policies = [(f"slots{slots}", dict(max_slots=slots, exclusive_s=exclusive_s, long_charge=long_charge))]
+    if policies is None:   # the loop's defaults otherwise (long requests take two slots)
+        policies = [(f"slots{slots}_default", dict(max_slots=slots, exclusive_s=exclusive_s, long_charge=long_charge))]
     out["policies"] = {name: {k: v for k, v in kw.items()} for name, kw in policies}
     for load, (pname, pkw) in [(ld, pol) for ld in ([0.0] if burst else list(loads)) for pol in policies]:
         rate = load * capacity
@@ -145,7 +145,8 @@ def main():
                     help="also run the policy whose requests estimated above this many seconds take two of four slots")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
-    policies = [(f"slots{n}", dict(max_slots=n, exclusive_s=args.exclusive_s, long_charge=args.long_charge))
+    policies = [(f"slots{n}", dict(max_slots=n, exclusive_s=args.exclusive_s, long_charge=args.long_charge,
+                                   long_slots_s=None))
                 for n in (int(x) for x in args.slots.split(","))]
     if args.one_long_s is not None:
         policies.append((f"one_long_above_{args.one_long_s:g}s",
