@@ -1390,7 +1390,18 @@ template <bool ANISO>
 __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__restrict__ in, float *__restrict__ out,
                                                     int t) {
   int z0, x0;
-  if (!strip_of3(d, z0, x0)) return;
+  // PML factors 1 - dt/2 d, 1 + dt/2 d, 1/(1 + dt/2 d) of the rows the strip updates (z0-6 ..
+  // z0+tz+2), staged once: read from LDS they cost no wait on the vector loads in flight
+  constexpr int PZR = ETZ_MAX + 12;
+  static_assert(3 * PZR <= ENT, "one PML factor per thread");
+  __shared__ float pzs[3][PZR];
+  if (!strip_of3(d, z0, x0)) return;   // uniform over the workgroup (one strip)
+  if (threadIdx.x < 3 * PZR) {
+    const int q = threadIdx.x / PZR, j = threadIdx.x - q * PZR, row = z0 - 6 + j;
+    pzs[q][j] = (unsigned)row < (unsigned)d.Z ? d.pml_z[q * d.Z + row] : 0.0f;
+  }
+  __syncthreads();
+  auto pzf = [&](int q, int row) -> float { return pzs[q][row - (z0 - 6)]; };
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int lane = threadIdx.x & (WAVE - 1);
   const int nn = d.nn;
@@ -1498,7 +1509,7 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
       if (k >= nn && k <= zw1 && cwin) {
         const float dzb_vz = D.dzb(vz, 3), dzf_vx = D.dzf(vx, 3);
         const float C11v = cur.cm[0], C13v = cur.cm[1], C33v = cur.cm[2], C55v = cur.cm[3];
-        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float pzn = pzf(0, k), pzi = pzf(2, k);
         float sx_xx, sz_xx, sx_zz, sz_zz, sx_xz, sz_xz;
         if constexpr (ANISO) {
           const float dzb_vx = D.dzb(vx, 3), dzf_vz = D.dzf(vz, 3);
@@ -1552,7 +1563,7 @@ __global__ __launch_bounds__(ENT) void el3_fwd_step(ElDev d, const float *__rest
         const unsigned o = rb(r);
         const float dzb_txz = D.dzb(xz, 3), dzf_tzz = D.dzf(zz, 3);
         const float bxv = bb[0], bzv = bb[1];
-        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float pzn = pzf(0, r), pzd = pzf(1, r);
         const float nx_x = (pxn * us[0][0] + ((d.dt * bxv) * dxf_txx) / d.dx) / pxd;
         const float nx_z = (pzn * us[0][1] + ((d.dt * bxv) * dzb_txz) / d.dz) / pzd;
         float nz_x = (pxn * us[0][2] + ((d.dt * bzv) * dxb_txz) / d.dx) / pxd;
@@ -1584,8 +1595,18 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
                                                     float *__restrict__ gacc, float *__restrict__ gsrc) {
   constexpr int NGF = ANISO ? 8 : 6;
   __shared__ float slab[EWPB * NGF * ETZA * WAVE];
+  // PML factors of the rows the strip reads (z0-6 .. z0+ETZA+5), staged once: read from LDS
+  // they cost no wait on the vector loads in flight
+  constexpr int PZR = ETZA + 12;
+  __shared__ float pzs[3][PZR];
   int z0, x0;
-  if (!strip_of3(d, z0, x0)) return;
+  if (!strip_of3(d, z0, x0)) return;   // uniform over the workgroup (one strip)
+  if (threadIdx.x < 3 * PZR) {
+    const int q = threadIdx.x / PZR, j = threadIdx.x - q * PZR, row = z0 - 6 + j;
+    pzs[q][j] = (unsigned)row < (unsigned)d.Z ? d.pml_z[q * d.Z + row] : 0.0f;
+  }
+  __syncthreads();
+  auto pzf = [&](int q, int row) -> float { return pzs[q][row - (z0 - 6)]; };
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int lane = threadIdx.x & (WAVE - 1);
   const int nn = d.nn;
@@ -1642,7 +1663,7 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
       push(vz, in_grid ? fvz : 0.f);
       float ea = 0.f, eb = 0.f, ec = 0.f, ed = 0.f;
       if (rr >= nn && rr <= zw1 && cwin) {
-        const float vz_w = d.dt / (d.dz * d.pml_z[d.Z + rr]);
+        const float vz_w = d.dt / (d.dz * pzf(1, rr));
         ea = lvxx * (vx_w * bxv);
         eb = lvxz * (vz_w * bxv);
         ec = lvzx * (vx_w * bzv);
@@ -1742,7 +1763,7 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
           if (d.src_fz) gmt = gm;        // the force term of row zs follows three rows later
           else gsrc[(long long)s * d.nt + t] = gm;
         }
-        const float pzn = d.pml_z[k], pzi = d.pml_z[2 * d.Z + k];
+        const float pzn = pzf(0, k), pzi = pzf(2, k);
         const float sz_keep = pzn * pzi, sz_a = d.dt_dz * pzi;
         if (own_k) {
           bst(sx_keep * lxx_x, rLo, vs, F(TXX_X) + o); bst(sz_keep * lxx_z, rLo, vs, F(TXX_Z) + o);
@@ -1818,7 +1839,7 @@ __global__ __launch_bounds__(ENT) void el3_adj_step(ElDev d, const float *__rest
       const float dxf_txx = D.dxf(xx[0], lane), dxb_txz = D.dxb(xz[3], lane);
       if (out_col && r >= z0 && r < end_row) {
         const unsigned o = rb(r);
-        const float pzn = d.pml_z[r], pzd = d.pml_z[d.Z + r];
+        const float pzn = pzf(0, r), pzd = pzf(1, r);
         const float vz_keep = pzn / pzd, vz_w = d.dt / (d.dz * pzd);
         const float l1 = lr[0], l2 = lr[1], l3 = lr[2], l4 = lr[3];
         if (d.src_fz && gsrc && r == zs && c == xs)
