@@ -18,8 +18,8 @@ with one of N shapes but its own model, so that acoustic modelling requests of o
 window of acoustic modelling (0: direct launches).  --exclusive-s S gives every request
 estimated to take longer than S seconds alone the device to itself.  --burst submits the whole stream at once
 (saturated throughput; the plain FIFO then starts every request at t = 0 too).  Several
-serving policies run on the same stream after one plain pass: every --slots value (a comma
-list; one slot per request), with --one-long-s S also the loop that runs requests estimated above S seconds one at
+serving policies run on the same stream after one plain pass: the loop's defaults when no
+policy is named, else every --slots value (a comma list; one slot per request), with --one-long-s S also the loop that runs requests estimated above S seconds one at
 a time beside shorter ones (4 slots, each such request charged 0.51 of the budget), and with
 --long-slots-s S the loop whose requests estimated above S seconds take two of its four slots.
 """
@@ -44,12 +44,14 @@ def fifo_latencies(arrivals, service):
 def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_range=(256, 2048),
                    nt_range=(1000, 6000), shots_range=(1, 64), model_pool=0, shape_pool=0, graph_steps=64,
                    pack=True, pack_across_models=False, exclusive_s=None, long_charge=1.0, burst=False,
-                   device="cuda:0", log=None, policies=None):
+                   device="cuda:0", log=None, policies=None, reserve=False):
     """Runs the seeded stream through the plain path (every request alone, one after the other:
     the service times and the capacity) and then through the serving loop at each load (arrival
     rate = load x plain capacity); returns the report dictionary.  policies: a list of
     (name, ServingLoop keyword arguments) run one after another at every load (default: one
-    loop with `slots`, `exclusive_s` and `long_charge`)."""
+    loop with `slots`, `exclusive_s` and `long_charge`).  reserve: first take 80 % of the free
+    device memory into torch's allocator cache in one block and release it to the cache, as a
+    long-running server's allocator holds it, so that no request waits for device allocations."""
     from .loop import ServingLoop, run_alone
     from .requests import make_stream
     log = log or (lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
@@ -70,6 +72,11 @@ def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_
     warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
                        shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
     ServingLoop(device, max_slots=2, graph_steps=graph_steps).run(warm, realtime=False)
+    if reserve:
+        import torch
+        free, _ = torch.cuda.mem_get_info(device)
+        block = torch.empty(int(0.8 * free), dtype=torch.uint8, device=device)
+        del block          # kept in torch's cache: later allocations are carved from it
     base = stream(1.0)
     t0 = time.perf_counter()
     alone = run_alone(copy.deepcopy(base), device, progress=progress("plain path"))
@@ -91,7 +98,7 @@ def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_
            "plain_service_p99_s": float(np.percentile(service, 99)), "runs": []}
     digests = {o.rid: o.digest for o in alone}
     if policies is None:   # the loop's defaults otherwise (long requests take two slots)
-        policies = [(f"slots{slots}_default", dict(max_slots=slots, exclusive_s=exclusive_s, long_charge=long_charge))]
+        policies = [("default", dict(max_slots=slots, exclusive_s=exclusive_s, long_charge=long_charge))]
     out["policies"] = {name: {k: v for k, v in kw.items()} for name, kw in policies}
     for load, (pname, pkw) in [(ld, pol) for ld in ([0.0] if burst else list(loads)) for pol in policies]:
         rate = load * capacity
@@ -122,7 +129,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--load", default="0.5,1,2", help="arrival rates as multiples of the plain capacity")
-    ap.add_argument("--slots", default="4", help="slots of the serving loop (a comma list: one policy each)")
+    ap.add_argument("--slots", default=None,
+                    help="slots of the serving loop, one slot per request (a comma list: one policy each); "
+                         "without it and the other policy options, the loop's defaults")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--scale-nt", type=float, default=0.1)
     ap.add_argument("--max-shots", type=int, default=8)
@@ -143,22 +152,28 @@ def main():
                     help="also run the policy that serves requests estimated above this many seconds one at a time")
     ap.add_argument("--long-slots-s", type=float, default=None,
                     help="also run the policy whose requests estimated above this many seconds take two of four slots")
+    ap.add_argument("--repeat", type=int, default=1, help="run every policy this many times, one after another")
+    ap.add_argument("--reserve", action="store_true",
+                    help="hold 80 %% of the free device memory in the allocator cache from the start")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     policies = [(f"slots{n}", dict(max_slots=n, exclusive_s=args.exclusive_s, long_charge=args.long_charge,
                                    long_slots_s=None))
-                for n in (int(x) for x in args.slots.split(","))]
+                for n in (int(x) for x in args.slots.split(","))] if args.slots else []
     if args.one_long_s is not None:
         policies.append((f"one_long_above_{args.one_long_s:g}s",
                          dict(max_slots=4, exclusive_s=args.one_long_s, long_charge=0.51)))
     if args.long_slots_s is not None:
         policies.append((f"two_slots_above_{args.long_slots_s:g}s", dict(max_slots=4, long_slots_s=args.long_slots_s)))
+    if not policies:       # the loop's defaults (four slots, requests estimated above 0.5 s take two)
+        policies = [("default", dict(exclusive_s=args.exclusive_s, long_charge=args.long_charge))]
+    policies = [(name if i == 0 else f"{name}#{i + 1}", kw) for name, kw in policies for i in range(args.repeat)]
     nt_range = (max(50, int(1000 * args.scale_nt)), max(60, int(6000 * args.scale_nt)))
     out = measure_stream(args.requests, [float(x) for x in args.load.split(",")], seed=args.seed, policies=policies,
                          nz_range=(128, args.max_nz), nx_range=(256, args.max_nx), nt_range=nt_range,
                          shots_range=(1, args.max_shots), model_pool=args.model_pool, shape_pool=args.shape_pool,
                          graph_steps=args.graph_steps, pack=not args.no_pack,
-                         pack_across_models=args.pack_across_models, burst=args.burst)
+                         pack_across_models=args.pack_across_models, burst=args.burst, reserve=args.reserve)
     print(json.dumps(out))
 
 
