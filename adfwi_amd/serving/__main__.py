@@ -44,14 +44,12 @@ def fifo_latencies(arrivals, service):
 def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_range=(256, 2048),
                    nt_range=(1000, 6000), shots_range=(1, 64), model_pool=0, shape_pool=0, graph_steps=64,
                    pack=True, pack_across_models=False, exclusive_s=None, long_charge=1.0, burst=False,
-                   device="cuda:0", log=None, policies=None, reserve=False):
+                   device="cuda:0", log=None, policies=None):
     """Runs the seeded stream through the plain path (every request alone, one after the other:
     the service times and the capacity) and then through the serving loop at each load (arrival
     rate = load x plain capacity); returns the report dictionary.  policies: a list of
     (name, ServingLoop keyword arguments) run one after another at every load (default: one
-    loop with `slots`, `exclusive_s` and `long_charge`).  reserve: first take 80 % of the free
-    device memory into torch's allocator cache in one block and release it to the cache, as a
-    long-running server's allocator holds it, so that no request waits for device allocations."""
+    loop with `slots`, `exclusive_s` and `long_charge`)."""
     from .loop import ServingLoop, run_alone
     from .requests import make_stream
     log = log or (lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
@@ -72,11 +70,6 @@ def measure_stream(requests, loads, *, slots=4, seed=4, nz_range=(128, 512), nx_
     warm = make_stream(4, 0.0, 99, nz_range=(128, 128), nx_range=(256, 256), nt_range=(40, 40),
                        shots_range=(1, 1), acoustic_frac=0.5, gradient_frac=0.5)
     ServingLoop(device, max_slots=2, graph_steps=graph_steps).run(warm, realtime=False)
-    if reserve:
-        import torch
-        free, _ = torch.cuda.mem_get_info(device)
-        block = torch.empty(int(0.8 * free), dtype=torch.uint8, device=device)
-        del block          # kept in torch's cache: later allocations are carved from it
     base = stream(1.0)
     t0 = time.perf_counter()
     alone = run_alone(copy.deepcopy(base), device, progress=progress("plain path"))
@@ -153,8 +146,6 @@ def main():
     ap.add_argument("--long-slots-s", type=float, default=None,
                     help="also run the policy whose requests estimated above this many seconds take two of four slots")
     ap.add_argument("--repeat", type=int, default=1, help="run every policy this many times, one after another")
-    ap.add_argument("--reserve", action="store_true",
-                    help="hold 80 %% of the free device memory in the allocator cache from the start")
     ap.add_argument("--burst", action="store_true")
     args = ap.parse_args()
     policies = [(f"slots{n}", dict(max_slots=n, exclusive_s=args.exclusive_s, long_charge=args.long_charge,
@@ -173,7 +164,7 @@ def main():
                          nz_range=(128, args.max_nz), nx_range=(256, args.max_nx), nt_range=nt_range,
                          shots_range=(1, args.max_shots), model_pool=args.model_pool, shape_pool=args.shape_pool,
                          graph_steps=args.graph_steps, pack=not args.no_pack,
-                         pack_across_models=args.pack_across_models, burst=args.burst, reserve=args.reserve)
+                         pack_across_models=args.pack_across_models, burst=args.burst)
     print(json.dumps(out))
 
 
