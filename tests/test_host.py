@@ -315,3 +315,32 @@ def test_integration_stub_matches_the_binding():
     assert v and int(v.group(1)) == nat.ABI_VERSION
     hdr = open(os.path.join(ROOT, "include", "adfwi_amd.h")).read()
     assert int(re.search(r"#define ADFWI_ABI_VERSION (\d+)", hdr).group(1)) == nat.ABI_VERSION
+
+
+def test_receiver_maps_list_rows_then_indices_and_refuse_repeated_points(monkeypatch):
+    """The receiver map both geometries hand to the step kernels: [Z] receivers per row, then
+    [Z][pitch] receiver index (-1 elsewhere); no map when a point repeats (its records are then
+    taken by separate launches), when there are no receivers, or, elastic, when switched off."""
+    from adfwi_amd.propagator.acoustic_kernels import AcousticGeometry
+    from adfwi_amd.propagator.elastic_kernels import ElasticGeometry
+    Z, X, pitch = 12, 20, 32
+    rz = np.array([3, 3, 7, 9], dtype=np.int64)
+    rx = np.array([2, 5, 5, 17], dtype=np.int64)
+    for cls in (AcousticGeometry, ElasticGeometry):
+        g = object.__new__(cls)          # the map builder only, without a device
+        g.Z, g.X, g.pitch, g.R, g.device = Z, X, pitch, len(rz), torch.device("cpu")
+        g.free_surface, g.fs = False, 2
+        m = g._receiver_map(rz, rx).numpy()
+        assert m.shape == (Z + Z * pitch,)
+        assert list(m[:Z]) == [0, 0, 0, 2, 0, 0, 0, 1, 0, 1, 0, 0]
+        idx = m[Z:].reshape(Z, pitch)
+        for r, (z, x) in enumerate(zip(rz, rx)):
+            assert idx[z, x] == r
+        assert (idx >= 0).sum() == len(rz)
+        assert g._receiver_map(np.array([3, 3]), np.array([5, 5])) is None     # a repeated point
+        g.R = 0
+        assert g._receiver_map(rz[:0], rx[:0]) is None
+    g = object.__new__(ElasticGeometry)
+    g.Z, g.X, g.pitch, g.R, g.device = Z, X, pitch, len(rz), torch.device("cpu")
+    monkeypatch.setenv("ADFWI_EL_FUSED_RECORDS", "0")
+    assert g._receiver_map(rz, rx) is None
