@@ -2344,6 +2344,11 @@ int el_layout(const adfwi_elastic_desc *d, adfwi_layout *L) {
   ADFWI_CHECK_ARG(L->field_floats * 4 * nf < ((int64_t)1 << 32) - 1,
                   "state of %d shots x %d x %d exceeds the 4 GiB of one buffer descriptor; "
                   "advance the shots in smaller launches", d->nshots, d->nz, d->pitch);
+  // the adjoint's accesses to a workgroup's gradient planes point the lanes that own no point
+  // past the planes (dropped buffer accesses), which needs the planes under 2 GiB
+  ADFWI_CHECK_ARG((int64_t)NCOEF * d->nz * d->pitch * 4 < ((int64_t)1 << 31),
+                  "grid of %d x %d (pitch %d) exceeds the 2 GiB of one set of gradient planes",
+                  d->nz, d->nx, d->pitch);
   L->state_floats = nf * L->field_floats;
   L->accum_floats = (int64_t)L->ngroups * NCOEF * d->nz * d->pitch;
   return ADFWI_OK;

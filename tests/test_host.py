@@ -344,3 +344,23 @@ def test_receiver_maps_list_rows_then_indices_and_refuse_repeated_points(monkeyp
     g.Z, g.X, g.pitch, g.R, g.device = Z, X, pitch, len(rz), torch.device("cpu")
     monkeypatch.setenv("ADFWI_EL_FUSED_RECORDS", "0")
     assert g._receiver_map(rz, rx) is None
+
+
+def test_elastic_layout_refuses_gradient_planes_over_2_gib():
+    """The elastic adjoint addresses a workgroup's gradient planes through one buffer descriptor
+    whose lanes without a point of their own aim past the planes, so one set of planes must stay
+    under 2 GiB: a single-shot 8200 x 8200 grid (its state still under the descriptor's 4 GiB)
+    is refused with a message, a 4000 x 4000 one accepted (host-side, no GPU)."""
+    from adfwi_amd import _native as nat
+    lib = nat.load()
+
+    def desc(nz, nx):
+        d = nat.ElasticDesc()
+        d.abi_version, d.struct_size = nat.ABI_VERSION, ctypes.sizeof(nat.ElasticDesc)
+        d.nshots, d.nz, d.nx, d.pitch = 1, nz, nx, -(-nx // 64) * 64
+        d.nt, d.nrec = 10, 0
+        return d
+    L = nat.Layout()
+    assert lib.adfwi_elastic_get_layout(ctypes.byref(desc(4000, 4000)), ctypes.byref(L)) == 0
+    assert lib.adfwi_elastic_get_layout(ctypes.byref(desc(8200, 8200)), ctypes.byref(L)) == -2
+    assert b"gradient planes" in lib.adfwi_last_error()
