@@ -311,6 +311,11 @@ def test_integration_stub_matches_the_binding():
     fields = eval(m.group(1), {"_i32": ct.c_int32, "_vp": ct.c_void_p})
     assert [f[0] for f in fields] == [f[0] for f in nat.AcousticDesc._fields_]
     assert [f[1] for f in fields] == [f[1] for f in nat.AcousticDesc._fields_]
+    m = re.search(r"class ElasticDesc\(ctypes\.Structure\):\s*_fields_ = (\[.*?\])\n", text, re.S)
+    assert m, "elastic stub not found"
+    fields = eval(m.group(1), {"_i32": ct.c_int32, "_vp": ct.c_void_p, "_f32": ct.c_float})
+    assert [f[0] for f in fields] == [f[0] for f in nat.ElasticDesc._fields_]
+    assert [f[1] for f in fields] == [f[1] for f in nat.ElasticDesc._fields_]
     v = re.search(r"AcousticDesc\(abi_version=(\d+)", text)
     assert v and int(v.group(1)) == nat.ABI_VERSION
     hdr = open(os.path.join(ROOT, "include", "adfwi_amd.h")).read()
