@@ -301,7 +301,7 @@ def test_pending_backward_reserves_are_counted_until_released():
 
 
 def test_integration_stub_matches_the_binding():
-    """The ctypes stub a maintainer copies from INTEGRATION.md declares the same descriptor
+    """The ctypes stub a maintainer copies from the integration notes declares the same descriptor
     fields and ABI version as the binding the package uses (and so as include/adfwi_amd.h)."""
     import ctypes as ct
     from adfwi_amd import _native as nat
