@@ -864,6 +864,21 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   float dA2u = bld(rC, vc, 3u * planeb + rb(qst)), dA1 = bld(rC, vc, planeb + rb(qst));
   float dK1 = bld(rC, vc, rb(qst - 2)), dK2 = bld(rC, vc, 2u * planeb + rb(qst - 2));
   float dK3 = bld(rC, vc, 4u * planeb + rb(qst - 2));
+  // step A at iteration k uses the coefficient rows step B used at iteration k-3: without the A2
+  // gradient B's values of the last two iterations wait in registers (h1: k-1, h2: k-2) instead
+  // of being loaded again (with it the registers would cost a wave per SIMD, and A loads them)
+  constexpr bool DELAY = !A2;
+  auto bA2w = [&](int j) { return wmask(j + 1) ? bld(rC, vc, 3u * planeb + rb(j + 1)) : 0.0f; };
+  float h1A2w = 0.f, h2A2w = 0.f, h1A2u = 0.f, h2A2u = 0.f, h1A1 = 0.f, h2A1 = 0.f;
+  float h1K1 = 0.f, h2K1 = 0.f, h1K2 = 0.f, h2K2 = 0.f, h1K3 = 0.f, h2K3 = 0.f;
+  if constexpr (DELAY) {
+    h1A2w = bA2w(kst - 1); h2A2w = bA2w(kst - 2);
+    h1A2u = bld(rC, vc, 3u * planeb + rb(kst - 1)); h2A2u = bld(rC, vc, 3u * planeb + rb(kst - 2));
+    h1A1 = bld(rC, vc, planeb + rb(kst - 1)); h2A1 = bld(rC, vc, planeb + rb(kst - 2));
+    h1K1 = bld(rC, vc, rb(kst - 3)); h2K1 = bld(rC, vc, rb(kst - 4));
+    h1K2 = bld(rC, vc, 2u * planeb + rb(kst - 3)); h2K2 = bld(rC, vc, 2u * planeb + rb(kst - 4));
+    h1K3 = bld(rC, vc, 4u * planeb + rb(kst - 3)); h2K3 = bld(rC, vc, 4u * planeb + rb(kst - 4));
+  }
   // B's image terms of rows k and r waiting three iterations for A's rows q and qq
   float ga1d0 = 0.f, ga1d1 = 0.f, ga1d2 = 0.f, ga2d0 = 0.f, ga2d1 = 0.f, ga2d2 = 0.f;
   // old gradient-plane values of rows q (A1) and qq (A2), loaded one iteration ahead
@@ -892,10 +907,13 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float nA2u = bld(rC, vc, 3u * planeb + rb(k + 1)), nA1 = bld(rC, vc, planeb + rb(k + 1));
     const float nK1 = bld(rC, vc, rb(r + 1)), nK2 = bld(rC, vc, 2u * planeb + rb(r + 1));
     const float nK3 = bld(rC, vc, 4u * planeb + rb(r + 1));
-    const float mA2w = wmask(q + 2) ? bld(rC, vc, 3u * planeb + rb(q + 2)) : 0.0f;
-    const float mA2u = bld(rC, vc, 3u * planeb + rb(q + 1)), mA1 = bld(rC, vc, planeb + rb(q + 1));
-    const float mK1 = bld(rC, vc, rb(qq + 1)), mK2 = bld(rC, vc, 2u * planeb + rb(qq + 1));
-    const float mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+    float mA2w = 0.f, mA2u = 0.f, mA1 = 0.f, mK1 = 0.f, mK2 = 0.f, mK3 = 0.f;
+    if constexpr (!DELAY) {
+      mA2w = wmask(q + 2) ? bld(rC, vc, 3u * planeb + rb(q + 2)) : 0.0f;
+      mA2u = bld(rC, vc, 3u * planeb + rb(q + 1)); mA1 = bld(rC, vc, planeb + rb(q + 1));
+      mK1 = bld(rC, vc, rb(qq + 1)); mK2 = bld(rC, vc, 2u * planeb + rb(qq + 1));
+      mK3 = bld(rC, vc, 4u * planeb + rb(qq + 1));
+    }
     const float nG1 = want_a1 ? bld(rG1, vc, gb + rb(q + 1)) : 0.0f;
     const float nG2 = A2 ? bld(rG2, vc, gb + rb(qq + 1)) : 0.0f;
 
@@ -1014,8 +1032,14 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
       cI[j] = nI[j]; cIp[j] = nIp[j];
       dI[j] = mI[j]; dIp[j] = mIp[j];
     }
+    if constexpr (DELAY) {
+      dA2w = h2A2w; dA2u = h2A2u; dA1 = h2A1; dK1 = h2K1; dK2 = h2K2; dK3 = h2K3;   // rows of iteration k-2
+      h2A2w = h1A2w; h2A2u = h1A2u; h2A1 = h1A1; h2K1 = h1K1; h2K2 = h1K2; h2K3 = h1K3;
+      h1A2w = cA2w; h1A2u = cA2u; h1A1 = cA1; h1K1 = cK1; h1K2 = cK2; h1K3 = cK3;
+    } else {
+      dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
+    }
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
-    dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
     // image terms: B's row k waits three iterations for A's row q, B's row r for A's row qq
     if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
     if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
