@@ -69,7 +69,12 @@ __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane
 // lane's column byte offset (VGPR), so no per-lane 64-bit address arithmetic is needed.
 // Every access stays inside its buffer by construction: loads use clamped rows and the clamped
 // column (vc), stores use the lane's own column (vs) only on lanes that own an output point.
-// The hardware range check is relied upon only for absent buffers (num_records 0).
+// The hardware range check is relied upon for absent buffers (num_records 0) and, in the
+// two-step kernels, for accesses that a lane must not make: their lane offset DROP points past
+// the buffer (the host runs those kernels only while every descriptor spans under 2 GiB), so
+// the access is dropped, moves no data and reads 0.  Stores issued on every lane that way
+// keep the loop free of branches around them, whose skipped path would make the loop's wait
+// for its prefetched loads also wait for the stores.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 constexpr int RSRC_DW3 = 0x00020000;  // raw 32-bit data, gfx9 family
 __device__ __forceinline__ rsrc_t rsrc(const void *p, unsigned bytes) {
@@ -82,6 +87,7 @@ __device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff) {
 __device__ __forceinline__ void bst(float v, rsrc_t r, unsigned voff, unsigned soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
+constexpr unsigned DROP = 0x80000000u;
 
 // strip index -> (z0, x0).  Workgroups are dispatched round-robin to the 8 XCDs
 // (each with its own L2); this bijection gives every XCD a contiguous run of
@@ -271,6 +277,9 @@ constexpr int SX2 = WAVE - 2 * HALO2;   // 52
 // feature sets of the two-step forward kernel: records and energy maps, stencil images,
 // pressure images
 constexpr int PF_REC = 1, PF_IMGD = 2, PF_IMGP = 4;
+// PF_BRANCH: stores in branches and receiver slots read in their own iteration, with no
+// dropped accesses (for fields whose two-field descriptors would reach 2 GiB)
+constexpr int PF_BRANCH = 8;
 
 struct PairIO {
   const float *in;
@@ -339,6 +348,17 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   const bool recs = rec_p != nullptr && d.nrec > 0;
   // rows z0 .. z0+127 that hold receivers (a strip's records are of rows z0 .. end_row-1)
   const adfwi::RowMask rrows = adfwi::receiver_rows(recs ? io.rmap : nullptr, d.Z, z0, lane);
+  // BF: stores issued on every lane (dropped where a lane owns no point), in the recompute
+  // variants (stencil images only).  The variants with records keep their stores in branches
+  // and read a receiver slot in its own iteration: with records and images the registers would
+  // cost a wave per SIMD, and with records only it measured 1.7 % slower (334 against 328 us;
+  // the recompute variant 349 against 367 us, one box)
+  constexpr bool BF = (F & PF_BRANCH) == 0 && !FR;
+  // a store that only some lanes make
+  auto put = [&](bool own, float v, rsrc_t r, unsigned voff, unsigned soff) {
+    if constexpr (BF) bst(v, r, own ? voff : DROP, soff);
+    else if (own) bst(v, r, voff, soff);
+  };
   unsigned so[SPB];
 #pragma unroll
   for (int j = 0; j < SPB; ++j) so[j] = (unsigned)(s0 + min(j, ns - 1)) * planeb;
@@ -439,14 +459,19 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
     const bool own_qq = out_col && qq >= z0 && qq < end_row;
     // receiver slots of A's emitted row r (step t) and of B's final row qq (step t+1): the
     // lane's map entry is read only on rows with receivers (reading every row's entries one row
-    // ahead with the other loads measured 7 % slower)
+    // ahead with the other loads measured 7 % slower; the variants with records keep their
+    // stores in branches, so a prefetched entry would only move the wait to the loop's end).
+    // Owned rows come after the warm-up iterations: own_r implies emit, own_qq implies bstep.
     int ridx = -1, rfin = -1;
     if (recs && emit && rrows.has(r) && own_r) ridx = io.rmap[d.Z + r * d.pitch + c];
     if (recs && bstep && rrows.has(qq) && own_qq) rfin = io.rmap[d.Z + qq * d.pitch + c];
     float epa = 0.0f, eua = 0.0f, epb = 0.0f, eub = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
-      if (j < ns) {
+      // BF: every shot slot runs (a last group's slots past ns repeat its last shot's rows and
+      // store nothing), so that no path through the loop skips the stores
+      const bool jv = j == 0 || j < ns;
+      if (BF || j < ns) {
         // ---- step A: new pressure at row k
         const float w3 = cW[j];
         float pn = cP[j];
@@ -460,16 +485,16 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
         }
         pw0[j] = pw1[j]; pw1[j] = pw2[j]; pw2[j] = pw3[j]; pw3[j] = pn;
         if (d.fsurf && k == fs + 1) pw1[j] = -pn;
-        if (img_d0 && own_k) bst(dd * dm, rI, vs, so[j] + rk);
+        if constexpr (FD) put(own_k && jv, dd * dm, rI, vs, so[j] + rk);
         // ---- step A: u', w' at row r; emit (p', u', w') of row r
         const float pr = pw1[j];
         const float pl1 = shfl(pr, lane - 1), pr1 = shfl(pr, lane + 1), pr2 = shfl(pr, lane + 2);
         const float un = ua * uw0[j] - ub * (C1 * (pr1 - pr) + C2 * (pr2 - pl1));
         const float wn = wa * ww0[j] - wb * (C1 * (pw2[j] - pr) + C2 * (pw3[j] - pw0[j]));
-        if (img_p0 && emit && own_r) bst(pr, rJ, vs, so[j] + rr);
-        epa += pr * pr;                     // stored for owned points only
-        eua += un * un;
-        if (ridx >= 0) {
+        if constexpr (FP) put(own_r && jv, pr, rJ, vs, so[j] + rr);
+        epa += jv ? pr * pr : 0.0f;         // stored for owned points only
+        eua += jv ? un * un : 0.0f;
+        if (ridx >= 0 && jv) {
           const long long ro = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
           rec_p[ro] = pr;
           io.rec_u[ro] = un;
@@ -495,7 +520,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
           }
           qw0[j] = qw1[j]; qw1[j] = qw2[j]; qw2[j] = qw3[j]; qw3[j] = pn2;
           if (d.fsurf && q == fs + 1) qw1[j] = -pn2;
-          if (img_d1 && own_q) bst(dd2 * dm2, rI, vs1, so[j] + rq);
+          if constexpr (FD) put(own_q && jv, dd2 * dm2, rI, vs1, so[j] + rq);
         }
         // ---- step B: u'', w'' at row qq = k-5; store the final rows
         {
@@ -503,21 +528,28 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
           const float sl1 = shfl(ps, lane - 1), sr1 = shfl(ps, lane + 1), sr2 = shfl(ps, lane + 2);
           const float un2 = ua2 * bu0[j] - ub2 * (C1 * (sr1 - ps) + C2 * (sr2 - sl1));
           const float wn2 = wa2 * bw0[j] - wb2 * (C1 * (qw2[j] - ps) + C2 * (qw3[j] - qw0[j]));
-          if (bstep && own_qq) {
+          if constexpr (BF) {
+            const bool oq = own_qq && jv;
+            bst(ps, oPU, oq ? vs : DROP, so[j] + rqq);
+            if constexpr (FP) bst(ps, rJ, oq ? vs1 : DROP, so[j] + rqq);
+            bst(un2, oPU, (oq && u_ok) ? vs1 : DROP, so[j] + rqq);
+            bst(wn2, oW, (oq && qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) ? vs : DROP, so[j] + rqq);
+            if (d.fsurf && qq == fs && oq) bst(wn2, oW, vs, so[j] + rqq - rowb);
+          } else if (bstep && own_qq) {
             bst(ps, oPU, vs, so[j] + rqq);
             if (img_p1) bst(ps, rJ, vs1, so[j] + rqq);
             if (u_ok) bst(un2, oPU, vs1, so[j] + rqq);
             if (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) bst(wn2, oW, vs, so[j] + rqq);
             if (d.fsurf && qq == fs) bst(wn2, oW, vs, so[j] + rqq - rowb);
           }
-          if (rfin >= 0) {   // records of step t+1: the values just stored (points never written read zero)
+          if (rfin >= 0 && jv) {   // records of step t+1: the values just stored (points never written read zero)
             const long long ro = ((long long)(s0 + j) * d.nt + t + 1) * d.nrec + rfin;
             rec_p[ro] = ps;
             io.rec_u[ro] = u_ok ? un2 : 0.0f;
             io.rec_w[ro] = (qq < d.Z - 1 && !(d.fsurf && qq == fs - 1)) ? wn2 : 0.0f;
           }
-          epb += ps * ps;                   // stored for owned points only
-          eub += un2 * un2;
+          epb += jv ? ps * ps : 0.0f;       // stored for owned points only
+          eub += jv ? un2 * un2 : 0.0f;
         }
       }
       cW[j] = nW[j]; cU[j] = nU[j]; cP[j] = nP[j];
@@ -745,7 +777,7 @@ struct AdjPairIO {
                                          // step t+1's (adj_pair_adds_first)
 };
 
-template <int SPB, bool A2, bool FULL>
+template <int SPB, bool A2, bool FULL, bool DR>
 __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   int z0, x0;
   if (!strip_of2(d, z0, x0)) return;
@@ -786,10 +818,19 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   const bool recs = io.rmap != nullptr && d.nrec > 0;
   // rows z0-6 .. z0+121 that hold receivers: the strip reads rows z0-6 .. end_row+5
   const adfwi::RowMask rrows = adfwi::receiver_rows(recs ? io.rmap : nullptr, d.Z, z0 - 6, lane);
-  // receiver index at this lane's point of a row (-1: none); read only on rows with receivers
-  auto slot_at = [&](int row) -> int {
-    return (recs && rrows.has(row) && row >= 0 && row < d.Z && in_col) ? io.rmap[d.Z + row * d.pitch + c] : -1;
-  };
+  // receiver index at this lane's point of a row (-1: none).  In the row loop the raw map
+  // entries are read one iteration ahead with the other loads (slot_ld, then slot); on rows
+  // without receivers the access is dropped (no traffic, reads 0)
+  const rsrc_t rM = rsrc(recs ? io.rmap + d.Z : nullptr, recs ? planeb : 0u);
+  auto slot_hit = [&](int row) { return recs && rrows.has(row) && row >= 0 && row < d.Z && in_col; };
+  auto slot_ld = [&](int row) -> float { return bld(rM, slot_hit(row) ? 4u * (unsigned)c : DROP, rb(row)); };
+  auto slot = [&](int row, float raw) -> int { return slot_hit(row) ? __builtin_bit_cast(int, raw) : -1; };
+  auto slot_at = [&](int row) -> int { return slot_hit(row) ? io.rmap[d.Z + row * d.pitch + c] : -1; };
+  // BF: stores issued on every lane (dropped where a lane owns no point) and slots read one
+  // iteration ahead, in the variants without the A2 gradient (with it the registers would cost
+  // a wave per SIMD; those keep their stores in branches and read a slot in its iteration) and
+  // when the host allows dropped accesses (DR)
+  constexpr bool BF = DR && !A2;
   // the record adjoints of step t+1 at receiver slot m, added to the incoming adjoint state as
   // its rows are taken over from the loads (the state itself is left as it was); the variant
   // with both the A2 gradient and every record adjoint leaves them to a separate launch, as the
@@ -884,6 +925,9 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
   // old gradient-plane values of rows q (A1) and qq (A2), loaded one iteration ahead
   float cG1 = want_a1 ? bld(rG1, vc, gb + rb(qst)) : 0.0f;
   float cG2 = A2 ? bld(rG2, vc, gb + rb(kst - 5)) : 0.0f;
+  // raw map entries of B's emitted row r and of the incoming rows k+1 (p, u) and k+2 (w)
+  float cR = 0.f, cMP = 0.f, cMW = 0.f;
+  if constexpr (BF) { cR = slot_ld(kst - 2); cMP = slot_ld(kst + 1); cMW = slot_ld(kst + 2); }
 
   const int iters = end_row - z0 + 9;   // k = z0-4 .. end_row+4
 #pragma unroll 1
@@ -892,6 +936,8 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const int r = k - 2, q = k - 3, qq = k - 5;
     const unsigned rk = rb(k), rr = rb(r), rq = rb(q), rqq = rb(qq);
     // next row's loads before this row's stores
+    float nR = 0.f, nMW = 0.f;
+    if constexpr (BF) { nR = slot_ld(r + 1); nMW = slot_ld(k + 3); }
     float nW[SPB], nU[SPB], nP[SPB], nI[SPB], nIp[SPB], mI[SPB], mIp[SPB];
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
@@ -943,11 +989,16 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
     const float m1k = rowp ? 1.0f : 0.0f, m1q = rowp_q ? 1.0f : 0.0f;
     // receiver slots: B's emitted row r (step t's sources, on every lane of the strip), and the
     // rows of the incoming state B reads now, p and u of row k and w of row k+1 (step t+1's)
-    const int ridx = slot_at(r), mpn = slot_at(k + 1), mwn = slot_at(k + 2);
+    int ridx, mpn, mwn;
+    if constexpr (BF) { ridx = slot(r, cR); mpn = slot(k + 1, cMP); mwn = slot(k + 2, cMW); }
+    else { ridx = slot_at(r); mpn = slot_at(k + 1); mwn = slot_at(k + 2); }
     float ga1b = 0.0f, ga2b = 0.0f, ga1a = 0.0f, ga2a = 0.0f;
 #pragma unroll
     for (int j = 0; j < SPB; ++j) {
-      if (j < ns) {
+      // BF: every shot slot runs (a last group's slots past ns repeat its last shot's rows and
+      // store nothing), so that no path through the loop skips the stores
+      const bool jv = j == 0 || j < ns;
+      if (BF || j < ns) {
         // ================= step B (t+1), as ac_adj_step at row k
         lw0[j] = lw1[j]; lw1[j] = lw2[j]; lw2[j] = lw3[j]; lw3[j] = lwfix(so[j], k + 1, cW[j]);
         lu0[j] = lu1[j]; lu1[j] = lu2[j]; lu2[j] = cU[j];
@@ -964,7 +1015,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         l2c[j] = l2;
         if (grad_src && own_k && k == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t + 1] = l2;
         g3[j] = a1k * l2;
-        if (want_a1) ga1b -= l2 * (cI[j] * m1k);
+        if (want_a1) ga1b -= jv ? l2 * (cI[j] * m1k) : 0.0f;
         // outputs of B at row r: the adjoint state after step t (plus step t's sources)
         const float gc = g1[j];
         const float gl1 = shfl(gc, lane - 1), gr1 = shfl(gc, lane + 1), gr2 = shfl(gc, lane + 2);
@@ -981,7 +1032,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         }
         if (c >= d.X - 1) ou = 0.0f;          // rows/columns the one-step kernel never writes
         if (r >= d.Z - 1) ow = 0.0f;
-        if (ridx >= 0) {
+        if (ridx >= 0 && jv) {
           const long long gi = ((long long)(s0 + j) * d.nt + t) * d.nrec + ridx;
           if (io.g_p) op = op + io.g_p[gi];
           if (g_u) ou = ou + g_u[gi];
@@ -1005,7 +1056,7 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
         m2c[j] = m2;
         if (grad_src && own_q && q == zs[j] && c == xs[j]) grad_src[(long long)(s0 + j) * d.nt + t] = m2;
         h3[j] = a1q * m2;
-        if (want_a1) ga1a -= m2 * (dI[j] * m1q);
+        if (want_a1) ga1a -= jv ? m2 * (dI[j] * m1q) : 0.0f;
         {
           const float hc = h1[j];
           const float hl1 = shfl(hc, lane - 1), hr1 = shfl(hc, lane + 1), hr2 = shfl(hc, lane + 2);
@@ -1020,7 +1071,12 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
               if (wm_q) ga2a -= mw0[j] * (C1 * (qp2[j] - pc) + C2 * (qp3[j] - qp0[j]));
             }
           }
-          if (own_qq) {
+          if constexpr (BF) {
+            const bool oq = own_qq && jv;
+            bst(opa, oPU, oq ? vs : DROP, so[j] + rqq);
+            bst(oua, oPU, (oq && c < d.X - 1) ? vs1 : DROP, so[j] + rqq);
+            bst(owa, oW, (oq && qq < d.Z - 1) ? vs : DROP, so[j] + rqq);
+          } else if (own_qq) {
             bst(opa, oPU, vs, so[j] + rqq);
             if (c < d.X - 1) bst(oua, oPU, vs1, so[j] + rqq);
             if (qq < d.Z - 1) bst(owa, oW, vs, so[j] + rqq);
@@ -1040,8 +1096,11 @@ __global__ __launch_bounds__(NT) void ac_adj_pair(AcDev d, AdjPairIO io) {
       dA2w = mA2w; dA2u = mA2u; dA1 = mA1; dK1 = mK1; dK2 = mK2; dK3 = mK3;
     }
     cA2w = nA2w; cA2u = nA2u; cA1 = nA1; cK1 = nK1; cK2 = nK2; cK3 = nK3;
+    cR = nR; cMP = cMW; cMW = nMW;
     // image terms: B's row k waits three iterations for A's row q, B's row r for A's row qq
-    if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
+    // (without the A1 gradient rG1 is empty and a BF store is dropped)
+    if constexpr (BF) bst(cG1 + (ga1d0 + ga1a), rG1, own_q ? vs : DROP, gb + rq);
+    else if (want_a1 && own_q) bst(cG1 + (ga1d0 + ga1a), rG1, vs, gb + rq);
     if (A2 && own_qq) bst(cG2 + (ga2d0 + ga2a), rG2, vs, gb + rqq);
     cG1 = nG1; cG2 = nG2;
     ga1d0 = ga1d1; ga1d1 = ga1d2; ga1d2 = ga1b;
@@ -1173,6 +1232,15 @@ AcDev make_dev(const adfwi_acoustic_desc *d, const adfwi_layout &L) {
   return v;
 }
 
+// the branch-free two-step variants drop the accesses a lane must not make by pointing them
+// DROP bytes past the buffer: every descriptor they use (two fields at most) must span under
+// 2 GiB.  Larger fields (and ADFWI_PAIR_DROP=0) take the variants with stores in branches.
+bool pair_drop_ok(const adfwi_layout &L) {
+  const char *e = std::getenv("ADFWI_PAIR_DROP");
+  if (e && e[0] == '0') return false;
+  return 2 * L.field_floats * 4 < ((int64_t)1 << 31);
+}
+
 dim3 step_grid(const adfwi_layout &L) { return dim3((L.ntiles + WPB - 1) / WPB, L.ngroups); }
 
 template <int SPB>
@@ -1215,22 +1283,25 @@ void launch_pair_f(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO 
 }
 
 // the variant without the features this launch does not use (fewer live scalar registers)
-void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io) {
+void launch_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const PairIO &io, bool drop_ok) {
   const bool rec = io.energy || io.rec_p;
   const bool imd = io.img_d0 || io.img_d1, imp = io.img_p0 || io.img_p1;
   if (!imd && !imp) launch_pair_f<PF_REC>(w, g, b, st, io);
-  else if (!rec && !imp) launch_pair_f<PF_IMGD>(w, g, b, st, io);
-  else if (!rec) launch_pair_f<PF_IMGD | PF_IMGP>(w, g, b, st, io);
+  else if (!rec && !imp)
+    drop_ok ? launch_pair_f<PF_IMGD>(w, g, b, st, io) : launch_pair_f<PF_IMGD | PF_BRANCH>(w, g, b, st, io);
+  else if (!rec)
+    drop_ok ? launch_pair_f<PF_IMGD | PF_IMGP>(w, g, b, st, io)
+            : launch_pair_f<PF_IMGD | PF_IMGP | PF_BRANCH>(w, g, b, st, io);
   else if (!imp) launch_pair_f<PF_REC | PF_IMGD>(w, g, b, st, io);
   else launch_pair_f<PF_REC | PF_IMGD | PF_IMGP>(w, g, b, st, io);
 }
 
-template <bool A2, bool FULL>
+template <bool A2, bool FULL, bool DR>
 void launch_adj_pair_a(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
   switch (w.spb) {
-    case 4: hipLaunchKernelGGL((ac_adj_pair<4, A2, FULL>), g, b, 0, st, w, io); break;
-    case 2: hipLaunchKernelGGL((ac_adj_pair<2, A2, FULL>), g, b, 0, st, w, io); break;
-    default: hipLaunchKernelGGL((ac_adj_pair<1, A2, FULL>), g, b, 0, st, w, io);
+    case 4: hipLaunchKernelGGL((ac_adj_pair<4, A2, FULL, DR>), g, b, 0, st, w, io); break;
+    case 2: hipLaunchKernelGGL((ac_adj_pair<2, A2, FULL, DR>), g, b, 0, st, w, io); break;
+    default: hipLaunchKernelGGL((ac_adj_pair<1, A2, FULL, DR>), g, b, 0, st, w, io);
   }
 }
 
@@ -1242,13 +1313,15 @@ bool adj_pair_adds_first(const AdjPairIO &io) {
   return !(a2 && full);
 }
 
-void launch_adj_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io) {
+void launch_adj_pair(const AcDev &w, dim3 g, dim3 b, hipStream_t st, const AdjPairIO &io, bool drop_ok) {
   const bool full = io.grad_src || io.g_u || io.g_w;
   const bool a2 = io.grad_a2 && io.img_p0 && io.img_p1;
   if (a2)
-    full ? launch_adj_pair_a<true, true>(w, g, b, st, io) : launch_adj_pair_a<true, false>(w, g, b, st, io);
+    full ? launch_adj_pair_a<true, true, false>(w, g, b, st, io) : launch_adj_pair_a<true, false, false>(w, g, b, st, io);
+  else if (drop_ok)
+    full ? launch_adj_pair_a<false, true, true>(w, g, b, st, io) : launch_adj_pair_a<false, false, true>(w, g, b, st, io);
   else
-    full ? launch_adj_pair_a<false, true>(w, g, b, st, io) : launch_adj_pair_a<false, false>(w, g, b, st, io);
+    full ? launch_adj_pair_a<false, true, false>(w, g, b, st, io) : launch_adj_pair_a<false, false, false>(w, g, b, st, io);
 }
 
 template <int SPB>
@@ -1306,6 +1379,7 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
   // checkpoint) and the receiver samples of both steps can be written through the receiver-index
   // map
   const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
+  const bool drop_ok = pair_drop_ok(L);
   const AcDev w = pair_dev(v);
   const dim3 pgrid((w.nstrips + WPB - 1) / WPB, L.ngroups);
   auto ckpt_slot = [&](int t) -> float * {   // checkpoint buffer receiving the state after step t
@@ -1330,7 +1404,7 @@ int adfwi_acoustic_forward(const adfwi_acoustic_desc *d, adfwi_acoustic_fwd_args
       io.img_p0 = imp; io.img_p1 = imp ? imp + L.field_floats : nullptr;
       io.rmap = a->rcv_map;
       io.rec_p = recs ? a->rec_p : nullptr; io.rec_u = a->rec_u; io.rec_w = a->rec_w;
-      launch_pair(w, pgrid, block, st, io);
+      launch_pair(w, pgrid, block, st, io, drop_ok);
       if ((rc = adfwi::check_launch("ac_fwd_pair"))) return rc;
     } else {
       const int acc_u = a->energy ? 1 : 0;   // from step energy_u_from on (tested in the kernel)
@@ -1374,6 +1448,7 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
   // two steps per launch: the later one (t) reversed first, the receiver sources of t and t-1
   // added inside the kernel through the receiver-index map
   const bool fuse = a->fuse_steps != 0 && L.shots_per_block <= 4 && (!recs || d->nrec == 0 || a->rcv_map != nullptr);
+  const bool drop_ok = pair_drop_ok(L);
   AcDev w = pair_dev(v);
   w.spb = L.reserved;
   const dim3 pgrid((w.nstrips + WPB - 1) / WPB, (d->nshots + w.spb - 1) / w.spb);
@@ -1407,7 +1482,7 @@ int adfwi_acoustic_adjoint(const adfwi_acoustic_desc *d, adfwi_acoustic_adj_args
       if ((rc = adfwi::check_launch("ac_inject"))) return rc;
     }
     if (pair) {
-      launch_adj_pair(w, pgrid, block, st, io);
+      launch_adj_pair(w, pgrid, block, st, io, drop_ok);
       if ((rc = adfwi::check_launch("ac_adj_pair"))) return rc;
     } else {
       const float *imd = a->image ? a->image + slot(t) : nullptr;

@@ -195,7 +195,10 @@ def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surfac
     accumulator planes).  needs: the gradients asked for, which pick the adjoint variant: "all"
     (A1, A2, wavelet, the three records' adjoints: the variant that leaves the first reversed
     step's record adjoints to a separate launch), "a1_p" (A1 from the pressure records only: the
-    lean variant), "a1_all" (A1 and the wavelet from all three records)."""
+    lean variant), "a1_all" (A1 and the wavelet from all three records).  Both forms of the
+    two-step kernels are checked: stores issued on every lane with the accesses of lanes that own
+    no point dropped (the default below 2 GiB per two fields), and stores in branches
+    (ADFWI_PAIR_DROP=0, the form for larger fields); the 7 shots leave partial shot groups."""
     from adfwi_amd.propagator import acoustic_kernels as ak
     from adfwi_amd.propagator.boundary_condition import bc_pml
     rng = np.random.default_rng(nt + spb)
@@ -214,8 +217,9 @@ def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surfac
     a1, k1, a2, k2, k3 = ak.coefficient_fields(torch.tensor(vp, device=DEV), torch.tensor(rho, device=DEV), damp,
                                                nz, nx, nabc, dt, dx, free_surface, DEV)
     outs = []
-    for fuse in ("0", "1"):
+    for fuse, drop in (("0", "1"), ("1", "1"), ("1", "0")):
         monkeypatch.setenv("ADFWI_FUSE_STEPS", fuse)
+        monkeypatch.setenv("ADFWI_PAIR_DROP", drop)
         if tz:
             monkeypatch.setenv("ADFWI_TILE_Z", str(tz))
         geom = ak.AcousticGeometry(Z, X, nt, free_surface, nabc, src_z, src_x, rcv_z, rcv_x, DEV, shots_per_block=spb)
@@ -230,14 +234,15 @@ def test_two_steps_per_launch_equal_one_step_per_launch(monkeypatch, free_surfac
         torch.cuda.synchronize()
         outs.append(([r.detach().cpu() for r in res], a1_.grad.cpu(), None if a2_.grad is None else a2_.grad.cpu(),
                      None if w_.grad is None else w_.grad.cpu()))
-    (r0, ga0, gb0, gw0), (r1, ga1, gb1, gw1) = outs
-    for i in range(3):
-        assert torch.equal(r0[i], r1[i]), ("record", i, rel_l2(r1[i].numpy(), r0[i].numpy()))
-    for i in range(3, 6):
-        assert rel_l2(r1[i].numpy(), r0[i].numpy()) <= 1e-6, ("energy", i)
-    assert (gw0 is None) == (gw1 is None) and (gw0 is None or torch.equal(gw0, gw1))
-    assert rel_l2(ga1.numpy(), ga0.numpy()) <= 1e-6
-    assert (gb0 is None) == (gb1 is None) and (gb0 is None or rel_l2(gb1.numpy(), gb0.numpy()) <= 1e-6)
+    (r0, ga0, gb0, gw0) = outs[0]
+    for form, (r1, ga1, gb1, gw1) in zip(("dropped", "branches"), outs[1:]):
+        for i in range(3):
+            assert torch.equal(r0[i], r1[i]), (form, "record", i, rel_l2(r1[i].numpy(), r0[i].numpy()))
+        for i in range(3, 6):
+            assert rel_l2(r1[i].numpy(), r0[i].numpy()) <= 1e-6, (form, "energy", i)
+        assert (gw0 is None) == (gw1 is None) and (gw0 is None or torch.equal(gw0, gw1)), form
+        assert rel_l2(ga1.numpy(), ga0.numpy()) <= 1e-6, form
+        assert (gb0 is None) == (gb1 is None) and (gb0 is None or rel_l2(gb1.numpy(), gb0.numpy()) <= 1e-6), form
 
 
 @pytest.mark.parametrize("abc", ["gerjan", "sincos"])
