@@ -70,11 +70,12 @@ __device__ __forceinline__ float shfl(float v, int lane) { return __shfl(v, lane
 // Every access stays inside its buffer by construction: loads use clamped rows and the clamped
 // column (vc), stores use the lane's own column (vs) only on lanes that own an output point.
 // The hardware range check is relied upon for absent buffers (num_records 0) and, in the
-// two-step kernels, for accesses that a lane must not make: their lane offset DROP points past
-// the buffer (the host runs those kernels only while every descriptor spans under 2 GiB), so
-// the access is dropped, moves no data and reads 0.  Stores issued on every lane that way
-// keep the loop free of branches around them, whose skipped path would make the loop's wait
-// for its prefetched loads also wait for the stores.
+// branch-free two-step variants, for accesses that a lane must not make: their lane offset
+// DROP points past the buffer, so the access is dropped, moves no data and reads 0.  That
+// needs every descriptor under 2 GiB (DROP plus the scalar offset then stays below 2^32 and
+// at or above num_records); the host picks those variants only then (pair_drop_ok).  Stores
+// issued on every lane that way keep the loop free of branches around them, whose skipped
+// path would make the loop's wait for its prefetched loads also wait for the stores.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 constexpr int RSRC_DW3 = 0x00020000;  // raw 32-bit data, gfx9 family
 __device__ __forceinline__ rsrc_t rsrc(const void *p, unsigned bytes) {
