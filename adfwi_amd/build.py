@@ -27,18 +27,30 @@ def hipcc():
     return "hipcc"
 
 
+def device_flags():
+    """Code-generation flags shared by the library build and device_asm."""
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17",
+            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+
 def build(verbose: bool = True) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-command-line-argument",
-           "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", tmp] + srcs
+    cmd = [hipcc()] + device_flags() + ["-fPIC", "-shared", "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
+
+
+def device_asm(source: str, out: str) -> str:
+    """gfx950 assembly of one source file (with its kernel metadata: registers, spills, LDS),
+    compiled with the library's flags."""
+    cmd = [hipcc()] + device_flags() + ["--cuda-device-only", "-S", "-o", out, os.path.join(CSRC, source)]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return out
 
 
 if __name__ == "__main__":
