@@ -350,7 +350,7 @@ __global__ __launch_bounds__(NT) void ac_fwd_pair(AcDev d, PairIO io) {
   // rows z0 .. z0+127 that hold receivers (a strip's records are of rows z0 .. end_row-1)
   const adfwi::RowMask rrows = adfwi::receiver_rows(recs ? io.rmap : nullptr, d.Z, z0, lane);
   // BF: stores issued on every lane (dropped where a lane owns no point), in the recompute
-  // variants (stencil images only).  The variants with records keep their stores in branches
+  // variants (images, no records).  The variants with records keep their stores in branches
   // and read a receiver slot in its own iteration: with records and images the registers would
   // cost a wave per SIMD, and with records only it measured 1.7 % slower (334 against 328 us;
   // the recompute variant 349 against 367 us, one box)
